@@ -1,0 +1,166 @@
+"""Benchmark of the explicit step on one MI355X.
+
+Workload (N=1): one 3-D block 256x128x128 with 9 synthetic species, plain
+FP64, Van Albada MUSCL + AUSM+ + central viscous flux + Wilke transport,
+SSP-RK3 with the global CFL time step (the "plain FP64 drop-in" config).
+A step is one RK3 step (3 stages) over the whole block; inputs are resident
+in HBM before the timed region.
+
+Prints one JSON line: metric/value in Mcell-stage updates/s (interior cells x
+stages / wall time), the roofline object of the dominant kernel (the fused
+RHS), and the CPU baseline (the oracle on the host cores, bounded sample).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+  python bench.py --copy-calibration   # plain copy kernel of known size, for
+                                       # scaling rocprofv3 FETCH/WRITE_SIZE
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic HBM bytes per cell-stage of the FP64 SSP-RK3 stage:
+#   B = 8 * [(8/3) n_c + n_g + 2] with n_c = 14 conserved, n_g = 10 geometric
+BYTES_PER_CELL_STAGE_FP64 = 8.0 * ((8.0 / 3.0) * 14 + 10 + 2)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def copy_calibration():
+    import torch
+    n = 1 << 28  # 2 GiB of float64
+    a = torch.empty(n, dtype=torch.float64, device="cuda")
+    a.fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8}))
+
+
+def cpu_baseline(case, threads):
+    from oracle import pyoracle
+    pyoracle.build()
+    used = pyoracle.set_threads(threads)
+    o = pyoracle.Oracle(case)
+    o.upload_initial()
+    o.init_conservative()
+    t0 = time.perf_counter()
+    o.step(1)
+    dt = time.perf_counter() - t0
+    cs = case.n_cells * 3
+    o.close()
+    return {"value": cs / dt / 1e6, "unit": "Mcell-stage updates/s", "cores": used, "kind": "port",
+            "sample": f"1 SSP-RK3 step (3 stages) of the same 256x128x128 9-species block, "
+                      f"{cs} cell-stages in {dt:.2f} s, OpenMP on {used} host threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--size", type=str, default="256,128,128")
+    ap.add_argument("--copy-calibration", action="store_true")
+    args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return 0
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU layout is "
+              f"design-only for this path", file=sys.stderr)
+        return 2
+    import numpy as np
+    from combustioncodegpu_amd import cases
+    from combustioncodegpu_amd.lib import Solver
+
+    mx, my, mz = (int(v) for v in args.size.split(","))
+    log(f"[bench] generating workload {mx}x{my}x{mz}, 9 species")
+    t0 = time.perf_counter()
+    case = cases.c1(mx, my, mz)
+    log(f"[bench] generated in {time.perf_counter() - t0:.1f} s")
+    g = Solver(case)
+    g.upload_initial()
+    g.init_conservative()
+    g.sync()
+    log("[bench] warmup")
+    g.step(args.warmup)
+    g.sync()
+    rc, nfail = g.check()
+    if rc != 0:
+        log(f"[bench] health check failed rc={rc} newton_fail={nfail}")
+        return 3
+    log("[bench] timed region")
+    g.sync()
+    t0 = time.perf_counter()
+    ev_ms = g.time_steps(args.steps)
+    g.sync()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+    cell_stages = case.n_cells * 3 * args.steps
+    value = cell_stages / wall / 1e6
+    # per-kernel durations from HIP events on the context's stream
+    g.set_profiling(True)
+    g.time_steps(3)
+    kt = g.kernel_times()
+    g.set_profiling(False)
+    rhs_ms, rhs_n = kt["rhs"]
+    rhs_avg_s = rhs_ms / max(rhs_n, 1) / 1e3
+    achieved = BYTES_PER_CELL_STAGE_FP64 * case.n_cells / rhs_avg_s / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "rhs_c1_fp64_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    dt = g.get_dt()
+    res = g.residual()
+    g.close()
+    out = {
+        "metric": "Mcell-stage updates/s, FP64 plain vs mixed+batched+surrogate; HBM GB/s",
+        "value": round(value, 3),
+        "unit": "Mcell-stage updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded smooth T/p/u fields, smoothed Dirichlet mass fractions, 9 synthetic species)",
+        "config": {"workload": "plain FP64 explicit step, 3-D block 256x128x128, 9 species, Van Albada MUSCL, "
+                               "AUSM+, viscous + Wilke, SSP-RK3 global CFL 0.5",
+                   "cells": case.n_cells, "variant": "fp64", "event_ms_per_step": round(ev_ms / args.steps, 4)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "kernel": "rhs_kernel (fused inviscid+viscous RHS)",
+                     "kernel_avg_ms": round(rhs_avg_s * 1e3, 4),
+                     "bytes_per_cell_stage": round(BYTES_PER_CELL_STAGE_FP64, 2)},
+        "kernel_ms_per_step": {k: round(v[0] / 3, 4) for k, v in kt.items()},
+        "dt": dt,
+        "residual": [float(x) for x in res],
+    }
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle)")
+        try:
+            out["cpu_baseline"] = cpu_baseline(case, args.cpu_threads)
+        except Exception as e:  # the baseline is reported, never the measurement
+            out["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,282 @@
+"""Seeded synthetic workloads (grids, species, initial fields, BCs).
+
+The reference initialises from input decks (Initialize.cu:9-110,
+Field.cu:118-169) that are not shipped; these generators produce the same
+kind of data deterministically so that the HIP path and the CPU oracle see
+identical bytes.  Random numbers come from a counter-based SplitMix64 hash
+(Steele, Lea & Flood 2014), which is easy to reproduce bit-exactly in any
+language.  Every field is filled over the ghost-padded range, as the
+reference's initialisation does (Field.cu:139-168).
+
+Workloads:
+  c0   2-D 128x64 air, Mach-2 inflow with a Gaussian density bump,
+       inflow / outflow / slip walls, MUSCL-minmod, AUSM+, viscous, RK3.
+  c1   3-D 256x128x128, 9 synthetic species (NASA-7, Lennard-Jones),
+       smooth seeded T/p/u fields, Dirichlet mass fractions smoothed,
+       outflow on all faces, Van Albada, AUSM+, viscous + Wilke, RK3.
+Both accept reduced sizes for tests.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+
+R_U = 8314.462618  # Constants.h:64
+NGG = 2
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def _mix(z: np.ndarray) -> np.ndarray:
+    z = (z ^ (z >> np.uint64(30))) * _M1
+    z = (z ^ (z >> np.uint64(27))) * _M2
+    return z ^ (z >> np.uint64(31))
+
+
+def uniform(seed: int, stream: int, n: int) -> np.ndarray:
+    """n doubles in [0,1) from SplitMix64 of counters 1..n of (seed, stream)."""
+    with np.errstate(over="ignore"):
+        base = _mix(np.array([(seed * 0x100000001B3 + stream * 0x9E3779B1 + 1) & 0xFFFFFFFFFFFFFFFF],
+                             dtype=np.uint64))[0]
+        ctr = np.arange(1, n + 1, dtype=np.uint64)
+        z = _mix(base + ctr * _GOLDEN)
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+@dataclass
+class BlockSpec:
+    mx: int
+    my: int
+    mz: int
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    bc: list = field(default_factory=list)     # (range_start, range_end, type, cond)
+    inner: list = field(default_factory=list)  # (src[6], tar[6], target_block 1-based)
+
+    @property
+    def padded_shape(self):
+        return (self.mz + 2 * NGG, self.my + 2 * NGG, self.mx + 2 * NGG)
+
+    @property
+    def n_padded(self):
+        a, b, c = self.padded_shape
+        return a * b * c
+
+    @property
+    def n_interior(self):
+        return self.mx * self.my * self.mz
+
+
+@dataclass
+class Case:
+    name: str
+    params: dict
+    species: dict | None
+    blocks: list
+    inflow: list
+    wall: list
+    bv0: list   # per block (6, pz, py, px)
+    yk0: list   # per block (ns, pz, py, px) or None
+
+    @property
+    def n_spec(self):
+        return 0 if self.species is None else len(self.species["mw"])
+
+    @property
+    def n_var(self):
+        return 5 + self.n_spec
+
+    @property
+    def n_cells(self):
+        return sum(b.n_interior for b in self.blocks)
+
+    def marshal(self):
+        return abi.Marshalled(self)
+
+
+def outflow_all_faces(mx, my, mz):
+    """Outflow on the 6 faces of a block (0-based vertex ranges)."""
+    X, Y, Z = mx - 1, my - 1, mz - 1
+    return [
+        ((0, 0, 0), (0, Y, Z), abi.BC_OUTFLOW, 0),
+        ((X, 0, 0), (X, Y, Z), abi.BC_OUTFLOW, 0),
+        ((0, 0, 0), (X, 0, Z), abi.BC_OUTFLOW, 0),
+        ((0, Y, 0), (X, Y, Z), abi.BC_OUTFLOW, 0),
+        ((0, 0, 0), (X, Y, 0), abi.BC_OUTFLOW, 0),
+        ((0, 0, Z), (X, Y, Z), abi.BC_OUTFLOW, 0),
+    ]
+
+
+def cartesian_grid(mx, my, mz, dx, perturb=0.0, seed=0):
+    """Vertex coordinates (k,j,i order flattened i-fastest); optional smooth
+    curvilinear perturbation of amplitude perturb*dx (positive Jacobian)."""
+    k, j, i = np.meshgrid(np.arange(mz), np.arange(my), np.arange(mx), indexing="ij")
+    x = i * dx
+    y = j * dx
+    z = k * dx
+    if perturb:
+        ph = uniform(seed, 77, 6) * 2 * math.pi
+        x = x + perturb * dx * np.sin(2 * math.pi * j / max(my - 1, 1) + ph[0]) * np.cos(
+            2 * math.pi * k / max(mz - 1, 1) + ph[1])
+        y = y + perturb * dx * np.sin(2 * math.pi * i / max(mx - 1, 1) + ph[2]) * np.cos(
+            2 * math.pi * k / max(mz - 1, 1) + ph[3])
+        if mz > 1:
+            z = z + perturb * dx * np.sin(2 * math.pi * i / max(mx - 1, 1) + ph[4]) * np.cos(
+                2 * math.pi * j / max(my - 1, 1) + ph[5])
+    return [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (x, y, z)]
+
+
+def padded_index_coords(mx, my, mz):
+    k, j, i = np.meshgrid(np.arange(-NGG, mz + NGG), np.arange(-NGG, my + NGG), np.arange(-NGG, mx + NGG),
+                          indexing="ij")
+    return i.astype(np.float64), j.astype(np.float64), k.astype(np.float64)
+
+
+# ------------------------------------------------------------------ species ---
+def synthetic_species(n_spec=9, seed=2024):
+    """n_spec synthetic species: W ~ U[2,44] kg/kmol; NASA-7 with one
+    coefficient set for both ranges (t_low 200, t_mid 1000, t_high 3000 K)
+    such that cp/R in [2.5, 6] on [200, 3000] K (checked on 2801 samples);
+    a5 ~ U[-3e4, 3e4] K; Lennard-Jones sigma ~ U[2.5,4] A, eps/k ~ U[30,600] K."""
+    u = uniform(seed, 1, 8 * n_spec)
+    mw = 2.0 + 42.0 * u[:n_spec]
+    sigma = 2.5 + 1.5 * u[n_spec:2 * n_spec]
+    eps_k = 30.0 + 570.0 * u[2 * n_spec:3 * n_spec]
+    a5 = -3e4 + 6e4 * u[3 * n_spec:4 * n_spec]
+    T0, D = 200.0, 2800.0
+    tt = np.linspace(200.0, 3000.0, 2801)
+    coeff = np.zeros((n_spec, 7))
+    for l in range(n_spec):
+        attempt = 0
+        while True:
+            r = uniform(seed, 100 + 31 * l + attempt, 5)
+            c = np.array([2.5 + 1.0 * r[0], 2.0 * r[1], -0.8 + 1.6 * r[2], -0.3 + 0.6 * r[3], -0.2 + 0.4 * r[4]])
+            tau = (tt - T0) / D
+            cpr = c[0] + tau * (c[1] + tau * (c[2] + tau * (c[3] + tau * c[4])))
+            if cpr.min() >= 2.5 and cpr.max() <= 6.0:
+                break
+            attempt += 1
+        # expand sum_m c_m ((T-T0)/D)^m into powers of T
+        a = np.zeros(5)
+        for m in range(5):
+            for n in range(m + 1):
+                a[n] += c[m] * math.comb(m, n) * (-T0) ** (m - n) / D ** m
+        coeff[l, :5] = a
+        coeff[l, 5] = a5[l]
+        coeff[l, 6] = 0.0
+    return {
+        "mw": mw, "t_low": np.full(n_spec, 200.0), "t_mid": np.full(n_spec, 1000.0),
+        "t_high": np.full(n_spec, 3000.0), "coeff_lo": coeff.copy(), "coeff_hi": coeff.copy(),
+        "eps_k": eps_k, "sigma": sigma,
+    }
+
+
+# ---------------------------------------------------------------------- C0 ---
+def c0(mx=128, my=64, seed=7, dx=1e-3, r_air=287.0, gamma=1.4, cfl=0.5, rk_stages=3, limiter=0):
+    """2-D air, uniform Mach-2 inflow (T=300 K, p=101325 Pa) with a seeded
+    Gaussian density bump (10 %, sigma = 8 cells) at uniform pressure;
+    inflow at i-min, outflow at i-max, slip walls at j-min / j-max."""
+    mz = 1
+    x, y, z = cartesian_grid(mx, my, mz, dx)
+    T0, p0 = 300.0, 101325.0
+    rho0 = p0 / (r_air * T0)
+    u0 = 2.0 * math.sqrt(gamma * r_air * T0)
+    ii, jj, _ = padded_index_coords(mx, my, mz)
+    c = uniform(seed, 3, 2)
+    ic = 0.25 * (mx - 1) + 0.5 * (mx - 1) * c[0]
+    jc = 0.25 * (my - 1) + 0.5 * (my - 1) * c[1]
+    sig = 8.0
+    r2 = (ii - ic) ** 2 + (jj - jc) ** 2
+    rho = rho0 * (1.0 + 0.1 * np.exp(-r2 / (2 * sig * sig)))
+    bv = np.zeros((6,) + rho.shape)
+    bv[0] = rho
+    bv[1] = u0
+    bv[4] = p0
+    bv[5] = p0 / (rho * r_air)
+    X, Y = mx - 1, my - 1
+    bc = [
+        ((0, 0, 0), (0, Y, 0), abi.BC_INFLOW, 0),
+        ((X, 0, 0), (X, Y, 0), abi.BC_OUTFLOW, 0),
+        ((0, 0, 0), (X, 0, 0), abi.BC_SYMMETRY, 0),
+        ((0, Y, 0), (X, Y, 0), abi.BC_SYMMETRY, 0),
+    ]
+    blk = BlockSpec(mx, my, mz, x, y, z, bc=bc)
+    params = dict(inviscid_scheme=3, reconstruction=2, limiter=limiter, viscous_order=2, rk_stages=rk_stages,
+                  cfl=cfl, prandtl=0.72, schmidt=0.9, gamma_air=gamma, r_air=r_air)
+    inflow = [dict(mach=2.0, pressure=p0, temperature=T0, u=1.0, v=0.0, w=0.0)]
+    return Case("c0", params, None, [blk], inflow, [], [np.ascontiguousarray(bv)], [None])
+
+
+# ---------------------------------------------------------------------- C1 ---
+def smooth_field(seed, stream, shape_idx, lengths):
+    """S = 1/4 sum_{m=1..4} sin(2 pi k_m . x / L + phi_m), k_m in {1..4}^3."""
+    ii, jj, kk = shape_idx
+    r = uniform(seed, stream, 16)
+    s = np.zeros_like(ii)
+    for m in range(4):
+        kv = [1 + int(4 * r[4 * m + d]) for d in range(3)]
+        phi = 2 * math.pi * r[4 * m + 3]
+        s += np.sin(2 * math.pi * (kv[0] * ii / lengths[0] + kv[1] * jj / lengths[1] + kv[2] * kk / lengths[2]) + phi)
+    return 0.25 * s
+
+
+def smoothed_dirichlet(seed, stream, n_spec, shape, passes=4):
+    """Per point Dirichlet(1) mass fractions (normalised exponentials), then
+    `passes` separable 3-point box filters (edge-replicated), renormalised."""
+    n = int(np.prod(shape))
+    u = uniform(seed, stream, n_spec * n).reshape((n_spec,) + shape)
+    e = -np.log1p(-u)
+    y = e / e.sum(axis=0, keepdims=True)
+    for _ in range(passes):
+        for ax in (1, 2, 3):
+            yp = np.concatenate([np.take(y, [0], axis=ax), y, np.take(y, [-1], axis=ax)], axis=ax)
+            sl = [slice(None)] * 4
+            a = []
+            for o in range(3):
+                sl[ax] = slice(o, o + y.shape[ax])
+                a.append(yp[tuple(sl)])
+            y = (a[0] + a[1] + a[2]) / 3.0
+    return y / y.sum(axis=0, keepdims=True)
+
+
+def c1(mx=256, my=128, mz=128, n_spec=9, seed=11, dx=1e-3, perturb=0.0, cfl=0.5, rk_stages=3, limiter=1,
+       bc="outflow", species_seed=2024, viscous=2, quirks=abi.QUIRKS_REFERENCE):
+    """3-D block with n_spec synthetic species and smooth seeded fields:
+    T = 1400 + 1100 S_T, p = 101325 (1 + 0.1 S_p), u_d = 200/sqrt(3) S_ud,
+    Y smoothed Dirichlet(1); outflow on all faces."""
+    sp = synthetic_species(n_spec, species_seed) if n_spec > 0 else None
+    x, y, z = cartesian_grid(mx, my, mz, dx, perturb=perturb, seed=seed)
+    ii, jj, kk = padded_index_coords(mx, my, mz)
+    L = (max(mx - 1, 1), max(my - 1, 1), max(mz - 1, 1))
+    T = 1400.0 + 1100.0 * smooth_field(seed, 10, (ii, jj, kk), L)
+    p = 101325.0 * (1.0 + 0.1 * smooth_field(seed, 11, (ii, jj, kk), L))
+    amp = 200.0 / math.sqrt(3.0)
+    u = amp * smooth_field(seed, 12, (ii, jj, kk), L)
+    v = amp * smooth_field(seed, 13, (ii, jj, kk), L)
+    w = amp * smooth_field(seed, 14, (ii, jj, kk), L)
+    if sp is not None:
+        Y = smoothed_dirichlet(seed, 20, n_spec, ii.shape)
+        r_mix = R_U * np.einsum("l...,l->...", Y, 1.0 / sp["mw"])
+    else:
+        Y = None
+        r_mix = np.full(ii.shape, R_U / 28.9647)
+    rho = p / (r_mix * T)
+    bv = np.stack([rho, u, v, w, p, T])
+    blk = BlockSpec(mx, my, mz, x, y, z, bc=outflow_all_faces(mx, my, mz))
+    params = dict(inviscid_scheme=3, reconstruction=2, limiter=limiter, viscous_order=viscous,
+                  rk_stages=rk_stages, cfl=cfl, prandtl=0.72, schmidt=0.9, quirks=quirks)
+    return Case("c1", params, sp, [blk], [], [], [np.ascontiguousarray(bv)],
+                [None if Y is None else np.ascontiguousarray(Y)])
+
+
+def cell_stages(case: Case, steps: int) -> int:
+    """Interior cells x stages x steps (the metric's work unit)."""
+    stages = 3 if case.params.get("rk_stages", 3) == 3 else 1
+    return case.n_cells * stages * steps

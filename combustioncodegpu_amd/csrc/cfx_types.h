@@ -1,0 +1,79 @@
+// cfx_types.h — plain structs shared by the host side of the library and the
+// HIP kernels: the per-block device view, the species table and the scheme
+// constants.  All device arrays are structure-of-arrays over the ghost-padded
+// block with i fastest (see include/cfx.h for the index formula).
+#pragma once
+
+#include <cstdint>
+
+#include "../../include/cfx.h"
+
+namespace cfx {
+
+constexpr double kRu = 8314.462618;  // universal gas constant, J/(kmol K) (Constants.h:64)
+constexpr double kMwAir = 28.9647;   // Constants.h:73
+constexpr int kNgg = 2;              // ghost layers for MUSCL (Parameter.cpp:32)
+constexpr int kMaxSp = CFX_MAX_SPEC;
+
+// Species table resident in device memory.  Everything the kernels need is
+// precomputed here so that the per-point loops are polynomial evaluations:
+//   cp_l(T)/1 = sum_k cpc[r][l][k] T^k,   h_l(T) = T*sum_k hc[r][l][k] T^k + hc[r][l][5]
+// with r = 0 for T < t_mid (low set) and r = 1 otherwise, already scaled by
+// R_u/W_l; below t_low the enthalpy is extended linearly with the frozen cp
+// (h_tlow, cp_tlow), as Thermo.cu:29-36 does.
+struct SpeciesDev {
+  int ns;
+  double mw[kMaxSp], inv_mw[kMaxSp], r_sp[kMaxSp];  // W, 1/W, R_u/W
+  double t_low[kMaxSp], t_mid[kMaxSp];
+  double cpc[2][kMaxSp][5];
+  double hc[2][kMaxSp][6];
+  double h_tlow[kMaxSp], cp_tlow[kMaxSp];
+  double lj_inv[kMaxSp], log_lj_inv[kMaxSp], vis_coeff[kMaxSp];
+  double cond_add[kMaxSp];              // 1.25 R_u / W_l (Transport.cu:222)
+  double wj4[kMaxSp][kMaxSp];           // (W_n/W_m)^(1/4)           (ChemData.cpp:257)
+  double sqw8[kMaxSp][kMaxSp];          // 1/sqrt(8(1+W_m/W_n))      (ChemData.cpp:258)
+};
+
+struct SchemeDev {
+  int reconstruction, limiter, viscous, quirks, rk, dim;
+  double cfl, pr, sc, gamma_air, r_air;
+  double inv_sc, inv_pr;
+};
+
+// One block as seen by the kernels.
+struct BlockView {
+  int mx, my, mz;
+  int px, py, pz;   // padded extents
+  long long np;     // padded point count
+  long long ni;     // interior point count
+  long long sy, sz; // padded strides in j and k
+  double *cv, *bv, *yk, *props;  // props: gamma, c, mu, kappa, rhoD[ns]
+  double *metric, *jac;          // metric: 9 x np (xi_x..zeta_z), jac: np
+  double *dq, *qn, *dt_local, *bv_last;
+};
+
+// Inflow state resident on device: rho,u,v,w,p,T, E, then Y[ns].
+struct InflowDev {
+  double st[7];
+  double y[kMaxSp];
+};
+
+struct WallDev {
+  int adiabatic;
+  double temperature;
+};
+
+// A registered boundary face (Boundary::register_boundary, Mesh.cpp:14-32).
+struct FaceRange {
+  int rs[3], re[3];
+  int face, direction;
+};
+
+// A registered inner face (InnerFace::register_boundary, Mesh.cpp:44-143).
+struct InnerDev {
+  int rs[3], re[3], face, direction;
+  int ts[3], target_block, src_tar[3], loop_dir[3], target_loop_dir[3];
+  int n_point[3];
+};
+
+}  // namespace cfx

@@ -1,0 +1,77 @@
+"""Loader of the HIP step library (combustioncodegpu_amd/libcfx.so).
+
+There is no fallback: if the library is missing or fails to load, every
+entry point raises.  Build it with ``__graft_entry__.build()`` or
+``python -m combustioncodegpu_amd.build``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from . import abi
+from .driver import StepContext
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcfx.so")
+_lib = None
+
+K_RHS, K_UPDATE, K_BC, K_PROPS, K_DT, K_OTHER = range(6)
+KERNEL_CLASSES = ("rhs", "update", "bc", "props", "dt", "other")
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    abi.declare_api(lib, "cfx_")
+    vp, dp = C.c_void_p, C.POINTER(C.c_double)
+    lib.cfx_version.argtypes = []
+    lib.cfx_version.restype = C.c_int
+    lib.cfx_check.argtypes = [vp, C.POINTER(C.c_longlong)]
+    lib.cfx_check.restype = C.c_int
+    lib.cfx_sync.argtypes = [vp]
+    lib.cfx_sync.restype = C.c_int
+    lib.cfx_time_steps.argtypes = [vp, C.c_int, dp]
+    lib.cfx_time_steps.restype = C.c_int
+    lib.cfx_kernel_times.argtypes = [vp, C.c_int, dp, C.POINTER(C.c_longlong)]
+    lib.cfx_kernel_times.restype = C.c_int
+    lib.cfx_set_profiling.argtypes = [vp, C.c_int]
+    lib.cfx_set_profiling.restype = C.c_int
+    _lib = lib
+    return lib
+
+
+class Solver(StepContext):
+    """One context of the HIP step library for a case."""
+
+    def __init__(self, case):
+        super().__init__(load(), "cfx_", case)
+
+    def check(self):
+        n = C.c_longlong(0)
+        rc = self.lib.cfx_check(self.h, C.byref(n))
+        return rc, n.value
+
+    def sync(self):
+        self._call("sync")
+
+    def time_steps(self, n):
+        ms = C.c_double(0.0)
+        self._call("time_steps", int(n), C.byref(ms))
+        return ms.value
+
+    def set_profiling(self, on=True):
+        self._call("set_profiling", 1 if on else 0)
+
+    def kernel_times(self):
+        out = {}
+        for c, name in enumerate(KERNEL_CLASSES):
+            ms = C.c_double(0.0)
+            n = C.c_longlong(0)
+            self.lib.cfx_kernel_times(self.h, c, C.byref(ms), C.byref(n))
+            out[name] = (ms.value, n.value)
+        return out

@@ -1,0 +1,236 @@
+/*
+ * cfx.h — C ABI of the MI355X (gfx950) explicit step of the multiblock
+ * compressible multispecies Navier–Stokes solver.
+ *
+ * The reference has no C ABI: its step path sits behind host free functions
+ * launched from Driver::steady_simulation (Driver.cu:167-246) and device
+ * "scheme" objects selected by integer tags in setup_schemes
+ * (Driver.cu:347-381).  Each entry point below names the reference routine(s)
+ * it replaces.  A caller keeps the reference's Driver loop shape: create a
+ * context from Parameter/Mesh/ChemData-equivalent descriptors, upload the
+ * initial primitive field, then call the step pieces (or cfx_step) per
+ * iteration.
+ *
+ * Conventions
+ *   - Every call returns int: 0 = OK, >0 = a hipError_t passed through,
+ *     <0 = a domain error (CFX_E_*).  cfx_last_error() gives a message.
+ *   - Host arrays are owned by the caller and only read during the call.
+ *   - The context owns all device memory and one HIP stream; calls are
+ *     stream-ordered and asynchronous, except the ones that hand data back
+ *     to the host (cfx_download, cfx_residual, cfx_get_dt, cfx_check,
+ *     cfx_time_step with a non-NULL dt pointer), which synchronise.
+ *   - A context is not thread-safe: one host thread per context.
+ *   - No C++ exception crosses the ABI.
+ *
+ * Data layout of host arrays handed to cfx_upload_* / returned by
+ * cfx_download: structure-of-arrays over the ghost-padded block, i fastest,
+ * then j, then k; point (i,j,k) with i in [-ngg, mx+ngg) etc. sits at
+ *   ((k+ngg)*(my+2ngg) + (j+ngg))*(mx+2ngg) + (i+ngg)
+ * and variable v of a multi-variable field at v*n_padded + that index.
+ * Variable orders follow the reference's DZone (Field.h:46-48):
+ *   cv: rho, rho*u, rho*v, rho*w, rho*E, rho*Y_1..rho*Y_ns
+ *   bv: rho, u, v, w, p, T
+ */
+#ifndef CFX_H
+#define CFX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CFX_ABI_VERSION 1
+#define CFX_MAX_SPEC 16
+
+/* domain errors */
+#define CFX_E_ARG        (-1) /* bad argument                                  */
+#define CFX_E_NEWTON     (-2) /* Newton temperature inversion did not converge */
+#define CFX_E_NONFINITE  (-3) /* non-finite or non-positive rho / T detected   */
+#define CFX_E_SURROGATE  (-4) /* surrogate transport requested but not ready   */
+#define CFX_E_UNSUPPORTED (-5) /* configuration not compiled in (e.g. n_spec) */
+
+/* boundary condition type labels, as in the reference (BoundCond.cu:254) */
+#define CFX_BC_WALL      2
+#define CFX_BC_SYMMETRY  3 /* slip wall: mirror with the normal velocity negated */
+#define CFX_BC_INFLOW    5
+#define CFX_BC_OUTFLOW   6
+
+/* quirk bits: 1 = reproduce the reference text, 0 = corrected form */
+#define CFX_Q_METRIC_AVG   0x1 /* Q5: fv/hv average only xi_x over the face (ViscousScheme.cu:16-24,238-246) */
+#define CFX_Q_HV_RHOD_J    0x2 /* Q6: hv averages rho*D with (i,j+1,k) (ViscousScheme.cu:317) */
+#define CFX_Q_NEWTON_REL   0x4 /* Q8: Newton stops at |1-T1/T|<=1e-3 (Thermo.cu:123,138) instead of |dT|<=1e-10 K */
+#define CFX_QUIRKS_REFERENCE (CFX_Q_METRIC_AVG | CFX_Q_HV_RHOD_J)
+
+/* precision modes */
+#define CFX_FP64   0 /* plain: everything in FP64 */
+#define CFX_MIXED  1 /* FP32 reconstruction/fluxes/transport; FP64 Newton, RK and reductions */
+
+/* transport models */
+#define CFX_TRANSPORT_WILKE 0
+#define CFX_TRANSPORT_MLP   1
+
+/* what cfx_download returns */
+#define CFX_CV     0 /* n_var x padded          */
+#define CFX_BV     1 /* 6 x padded              */
+#define CFX_YK     2 /* n_spec x padded         */
+#define CFX_PROPS  3 /* (4 + n_spec) x padded: gamma, c, mu, kappa, rho*D_1..ns */
+#define CFX_DQ     4 /* n_var x interior (mx*my*mz, i fastest) */
+#define CFX_METRIC 5 /* 10 x padded: xi_x,xi_y,xi_z,eta_x..,zeta_z, jacobian */
+#define CFX_DTLOCAL 6 /* interior: local time step of the last cfx_time_step */
+
+/* One physical boundary face, as a line of the reference's boundary file
+ * (Mesh.cpp:606-615, Boundary ctor Mesh.cpp:10-12): 0-based vertex ranges,
+ * start == end in the face-normal direction. */
+typedef struct {
+  int range_start[3];
+  int range_end[3];
+  int type;  /* CFX_BC_* */
+  int cond;  /* index into cfx_params.inflow / cfx_params.wall for that type */
+} cfx_bc;
+
+/* One block-to-block interface, as two lines of the reference's inner file
+ * (Mesh.cpp:640-665, InnerFace ctor Mesh.cpp:34-42): 1-based, signed ranges
+ * in the Plot3D/Gridgen convention, target block 1-based. */
+typedef struct {
+  int src[6];       /* x1 x2 y1 y2 z1 z2 of this block            */
+  int tar[6];       /* x1 x2 y1 y2 z1 z2 of the target block       */
+  int target_block; /* 1-based, as in the file                     */
+} cfx_inner;
+
+/* One grid block (Block, Mesh.h:109-159): vertex coordinates of the
+ * mx*my*mz points (i fastest), boundaries and interfaces. */
+typedef struct {
+  int mx, my, mz;
+  const double *x, *y, *z;
+  int n_bc;
+  const cfx_bc *bc;
+  int n_inner;
+  const cfx_inner *inner;
+} cfx_block;
+
+/* Inflow record (Inflow, BoundCond.h:14-48): give (mach, pressure,
+ * temperature) or (density, velocity, pressure); unset fields are < 0 as in
+ * the reference defaults. (u,v,w) is the unit direction. */
+typedef struct {
+  double mach, pressure, temperature, velocity, density;
+  double u, v, w;
+  const double *yk; /* n_spec mass fractions; NULL on the air path */
+} cfx_inflow;
+
+/* Wall record (Wall, BoundCond.h:50-58) */
+typedef struct {
+  int adiabatic;      /* 0 isothermal, 1 adiabatic */
+  double temperature; /* wall temperature for isothermal */
+} cfx_wall;
+
+/* Scheme tags and constants (DParameter, DParameter.h:9-39) plus the
+ * build's own switches. */
+typedef struct {
+  int inviscid_scheme;  /* 3 = AUSM+ (only scheme of the reference)      */
+  int reconstruction;   /* 2 = MUSCL kappa=1/3, otherwise first order     */
+  int limiter;          /* 0 = minmod, 1 = Van Albada                      */
+  int viscous_order;    /* 2 = 2nd-order central, otherwise inviscid      */
+  int temporal_scheme;  /* kept for the reference's tag; the stepping is chosen by rk_stages */
+  int rk_stages;        /* 1 = explicit Euler with local dt (reference), 3 = SSP-RK3 with global CFL dt */
+  double cfl, prandtl, schmidt;
+  double gamma_air, r_air; /* air path (n_spec == 0) constants; reference: 1.4, R_u/28.9647 */
+  int precision;        /* CFX_FP64 | CFX_MIXED */
+  int transport;        /* CFX_TRANSPORT_WILKE | CFX_TRANSPORT_MLP */
+  int quirks;           /* CFX_Q_* bits */
+  int n_inflow;
+  const cfx_inflow *inflow;
+  int n_wall;
+  const cfx_wall *wall;
+} cfx_params;
+
+/* Species data (Species, ChemData.h:11-46).  NASA-7 coefficient arrays are
+ * n_spec x 7 row-major as read by read_therm (ChemData.cpp:106-228); eps_k
+ * and sigma are the Lennard-Jones well depth [K] and diameter [Angstrom]
+ * read by read_tran (ChemData.cpp:230-252).  n_spec == 0 selects the
+ * calorically perfect air path. */
+typedef struct {
+  int n_spec;
+  const double *mw;                       /* kg/kmol */
+  const double *t_low, *t_mid, *t_high;   /* K */
+  const double *coeff_lo, *coeff_hi;      /* n_spec*7 */
+  const double *eps_k, *sigma;
+} cfx_species;
+
+typedef struct cfx_ctx cfx_ctx;
+
+/* Library version / capability query. */
+int cfx_version(void);
+
+/* Driver ctor + Field::setup_device_memory + DParameter + ghost grid and
+ * metrics (Driver.cu:71-111, Field.cu:182-240, DParameter.cu:118-159,
+ * Mesh.cpp:718-872 and 226-488). */
+int cfx_create(const cfx_params *params, const cfx_species *species,
+               const cfx_block *blocks, int n_block, cfx_ctx **out);
+void cfx_destroy(cfx_ctx *ctx);
+const char *cfx_last_error(const cfx_ctx *ctx);
+
+/* Upload the primitive state of one block over the padded range (the
+ * reference fills ghosts too: Field.cu:139-168). bv is 6 x padded, yk is
+ * n_spec x padded (ignored on the air path). */
+int cfx_upload_primitive(cfx_ctx *ctx, int blk, const double *bv, const double *yk);
+/* Upload the conservative field directly (checkpoint restore), n_var x padded,
+ * together with bv and yk. */
+int cfx_upload_state(cfx_ctx *ctx, int blk, const double *cv, const double *bv, const double *yk);
+int cfx_download(cfx_ctx *ctx, int blk, int what, double *dst);
+
+/* Driver::initialize_computation (Driver.cu:113-151): cv from bv, boundary
+ * conditions, inter-block exchange, physical properties. */
+int cfx_init_conservative(cfx_ctx *ctx);
+
+/* store_last_step (TimeAdvanceFunc.cu:59-70) for every block. */
+int cfx_store_last_step(cfx_ctx *ctx);
+/* compute_inviscid_flux + compute_viscous_flux (TimeAdvanceFunc.cu:84-289)
+ * for one block: dq = -(dF_inv) + (dF_visc), overwritten (set_dq_to_0 is
+ * folded in).  blk < 0 means every block. */
+int cfx_rhs(cfx_ctx *ctx, int blk);
+/* local_time_step (TimeAdvanceFunc.cu:291-299, TemporalScheme.cu:10-43) for
+ * every block; global != 0 also reduces min(dt_local) over all blocks into the
+ * device-resident dt used by cfx_update.  dt_out (may be NULL) receives it. */
+int cfx_time_step(cfx_ctx *ctx, int global, double *dt_out);
+/* update_cv_and_bv (TimeAdvanceFunc.cu:301-336) for every block.  With
+ * rk_stages == 1 the local dt is used (reference); with rk_stages == 3,
+ * stage = 0,1,2 applies the SSP-RK3 combination with the device dt. */
+int cfx_update(cfx_ctx *ctx, int stage);
+/* DBoundCond::apply_boundary_conditions (BoundCond.cu:251-313) + slip wall. */
+int cfx_apply_bc(cfx_ctx *ctx);
+/* Driver::data_communication (Driver.cu:248-271), inner faces on device. */
+int cfx_exchange(cfx_ctx *ctx);
+/* eliminate_k_gradient (Field.cu:360-384), 2-D only. */
+int cfx_eliminate_k_gradient(cfx_ctx *ctx);
+/* update_physical_properties (Field.cu:270-302) over the padded range. */
+int cfx_update_properties(cfx_ctx *ctx);
+/* compute_residual (Driver.cu:273-331) without the normalisation:
+ * res[0..3] = sqrt(sum over all blocks of (d rho, d|V|, d p, d T)^2 / n_grid_total). */
+int cfx_residual(cfx_ctx *ctx, double res[4]);
+/* The whole step loop body of Driver::steady_simulation (Driver.cu:189-244),
+ * n_steps times, without residual output. */
+int cfx_step(cfx_ctx *ctx, int n_steps);
+/* Read back the device dt of the last global reduction. */
+int cfx_get_dt(cfx_ctx *ctx, double *dt);
+/* Set the device dt (fixed-dt runs). */
+int cfx_set_dt(cfx_ctx *ctx, double dt);
+/* Health check: synchronises and reports non-finite state (CFX_E_NONFINITE)
+ * and Newton failures (CFX_E_NEWTON, count in *n_newton_fail if non-NULL). */
+int cfx_check(cfx_ctx *ctx, long long *n_newton_fail);
+/* Synchronise the context's stream. */
+int cfx_sync(cfx_ctx *ctx);
+
+/* Time k steps with HIP events on the context's stream (no host sync inside).
+ * ms_total receives the elapsed time.  Used by the benchmark. */
+int cfx_time_steps(cfx_ctx *ctx, int n_steps, double *ms_total);
+
+/* Per-kernel timing of the last cfx_time_steps call: for kernel class c
+ * (0 rhs, 1 update, 2 bc, 3 props, 4 dt, 5 other) the summed milliseconds
+ * and launch count, measured with events on the context's stream. */
+int cfx_kernel_times(cfx_ctx *ctx, int cls, double *ms, long long *launches);
+int cfx_set_profiling(cfx_ctx *ctx, int on);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CFX_H */

@@ -1,0 +1,72 @@
+"""Shared helpers of the test suite: comparisons with stated tolerances.
+
+Errors are normwise: max|a-b| over the compared points divided by a scale.
+The scale is per physical quantity, not per array row:
+  - momentum (rho u, rho v, rho w) and velocity (u, v, w) are vectors and
+    are scaled by the largest vector magnitude; a component that is nearly
+    zero (the y-momentum of the 2-D air case is ~1e-9 of |rho V|) is a
+    cancellation of O(|rho V|) fluxes and carries round-off of that size;
+  - partial densities rho*Y_l are scaled by max rho, mass fractions by 1;
+  - every other quantity by its own maximum magnitude.
+"""
+import numpy as np
+
+from combustioncodegpu_amd import abi
+
+
+def _rows(a):
+    return a.reshape(a.shape[0], -1)
+
+
+def scales(sel, ref):
+    r = _rows(np.asarray(ref, dtype=np.float64))
+    mag = np.abs(r).max(axis=1)
+    s = mag.copy()
+    if sel in (abi.CV, abi.BV) and r.shape[0] >= 4:
+        vmag = np.sqrt((r[1:4] ** 2).sum(axis=0)).max() if sel == abi.BV else np.sqrt(
+            (r[1:4] ** 2).sum(axis=0)).max()
+        s[1:4] = vmag
+        if sel == abi.CV:
+            s[5:] = mag[0]
+    if sel == abi.YK:
+        s[:] = 1.0
+    if sel == abi.DQ and r.shape[0] >= 4:
+        s[1:4] = mag[1:4].max()
+        s[5:] = np.maximum(mag[5:], mag[0]) if r.shape[0] > 5 else s[5:]
+    return np.where(s == 0, 1.0, s)
+
+
+def rel_errors(sel, a, b):
+    a = _rows(np.asarray(a, dtype=np.float64))
+    b = _rows(np.asarray(b, dtype=np.float64))
+    return np.abs(a - b).max(axis=1) / scales(sel, b)
+
+
+def assert_close_sel(sel, a, b, tol, what=""):
+    r = rel_errors(sel, a, b)
+    worst = float(np.max(r))
+    assert worst <= tol, (f"{what}: relative error {worst:.3e} > {tol:.1e} "
+                          f"(per variable: {np.array2string(np.atleast_1d(r), precision=2)})")
+    return worst
+
+
+def normwise_rel(a, b):
+    a = _rows(np.asarray(a, dtype=np.float64))
+    b = _rows(np.asarray(b, dtype=np.float64))
+    den = np.abs(b).max(axis=1)
+    return np.abs(a - b).max(axis=1) / np.where(den == 0, 1.0, den)
+
+
+def assert_close(a, b, tol, what=""):
+    r = normwise_rel(a, b)
+    worst = float(np.max(r))
+    assert worst <= tol, (f"{what}: normwise relative error {worst:.3e} > {tol:.1e} "
+                          f"(per variable: {np.array2string(np.atleast_1d(r), precision=2)})")
+    return worst
+
+
+def interior(arr, ngg=2, dim=3):
+    """Interior part of a (v, pz, py, px) padded field."""
+    if dim == 2:
+        return arr[..., ngg:ngg + 1, ngg:-ngg, ngg:-ngg]
+    return arr[..., ngg:-ngg, ngg:-ngg, ngg:-ngg]
