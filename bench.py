@@ -74,6 +74,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--size", type=str, default="256,128,128")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--rhs-only", type=int, default=0,
+                    help="profiling mode: run this many RHS launches on the workload and exit")
+    ap.add_argument("--rhs-variant", type=int, default=-1)
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -95,6 +98,15 @@ def main():
     g.upload_initial()
     g.init_conservative()
     g.sync()
+    if args.rhs_variant >= 0:
+        g.set_rhs_variant(args.rhs_variant)
+    if args.rhs_only:
+        for _ in range(args.rhs_only):
+            g.rhs(-1)
+        g.sync()
+        log(f"[bench] {args.rhs_only} RHS launches done")
+        g.close()
+        return 0
     log("[bench] warmup")
     g.step(args.warmup)
     g.sync()

@@ -630,6 +630,15 @@ int cfx_time_steps(cfx_ctx *ctx, int n_steps, double *ms_total) {
   return 0;
 }
 
+int cfx_set_rhs_variant(cfx_ctx *ctx, int variant) {
+  if (!ctx) return CFX_E_ARG;
+  if (variant < 0 || variant >= n_rhs_tiles(ctx->dim)) return ctx->fail(CFX_E_ARG, "rhs variant out of range");
+  ctx->rhs_tile = variant;
+  return 0;
+}
+
+int cfx_rhs_variants(cfx_ctx *ctx) { return ctx ? n_rhs_tiles(ctx->dim) : CFX_E_ARG; }
+
 int cfx_kernel_times(cfx_ctx *ctx, int cls, double *ms, long long *launches) {
   if (!ctx || cls < 0 || cls >= K_NCLASS) return CFX_E_ARG;
   if (ms) *ms = ctx->class_ms[cls];

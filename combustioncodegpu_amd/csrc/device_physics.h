@@ -68,19 +68,17 @@ __device__ __forceinline__ R limiter(int tag, R a, R b) {
   return (a > R(0) && b > R(0)) ? m : ((a < R(0) && b < R(0)) ? -m : R(0));
 }
 
-// MUSCL kappa = 1/3 on one variable (Reconstruction.cu:59-77), or first order.
+// MUSCL kappa = 1/3 on one variable (Reconstruction.cu:59-77), or first
+// order.  Both limiters are symmetric, lim(a,b) == lim(b,a) bit for bit, so
+// the reference's four calls per face reduce to the two limited slopes of
+// the two cells: (1-kappa)/4 + (1+kappa)/4 = 1/2.
 template <typename R>
 __device__ __forceinline__ void muscl(const SchemeDev &S, R q0, R q1, R q2, R q3, R &l, R &r) {
   if (S.reconstruction == 2) {
-    constexpr double kappa = 1.0 / 3.0;
-    const R a = R(1 - kappa), b = R(1 + kappa);
-    const R d_i = q1 - q0, d_i1 = q2 - q1, d_i2 = q3 - q2;
-    const R dnl = limiter<R>(S.limiter, d_i, d_i1);
-    const R dpl = limiter<R>(S.limiter, d_i1, d_i);
-    const R dnr = limiter<R>(S.limiter, d_i1, d_i2);
-    const R dpr = limiter<R>(S.limiter, d_i2, d_i1);
-    l = q1 + R(0.25) * (a * dnl + b * dpl);
-    r = q2 - R(0.25) * (a * dpr + b * dnr);
+    const R sl = limiter<R>(S.limiter, q1 - q0, q2 - q1);
+    const R sr = limiter<R>(S.limiter, q2 - q1, q3 - q2);
+    l = q1 + R(0.5) * sl;
+    r = q2 - R(0.5) * sr;
   } else {
     l = q1;
     r = q2;

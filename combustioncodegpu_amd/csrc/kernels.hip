@@ -135,8 +135,8 @@ __global__ void __launch_bounds__(256) props_kernel(BlockView B, const SpeciesDe
 // the face fluxes F = F_inviscid - F_viscous of the tile's (T+1) faces are
 // computed by a flat loop over the workgroup (each face once), staged in LDS,
 // and differenced into per-thread accumulators; dq is written once.
-template <int NS, int TX, int TY, int TZ>
-__global__ void __launch_bounds__(TX *TY *TZ) rhs_kernel(BlockView B, const SpeciesDev *__restrict__ sp, SchemeDev S,
+template <int NS, int TX, int TY, int TZ, int WPE>
+__global__ void __launch_bounds__(TX *TY *TZ) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) rhs_kernel(BlockView B, const SpeciesDev *__restrict__ sp, SchemeDev S,
                                                          int ntx, int nty, int ntz) {
   constexpr int NV = 5 + NS;
   constexpr int NT = TX * TY * TZ;
@@ -155,32 +155,33 @@ __global__ void __launch_bounds__(TX *TY *TZ) rhs_kernel(BlockView B, const Spec
   for (int d = 0; d < S.dim; ++d) {
     const int fx = TX + (d == 0), fy = TY + (d == 1), fz = TZ + (d == 2);
     const int nf = fx * fy * fz;
-    for (int f = tid; f < nf; f += NT) {
-      const int c[3] = {i0 + f % fx, j0 + (f / fx) % fy, k0 + f / (fx * fy)};
-      bool ok = true;
+    const int flo = ci + fx * (cj + fy * ck);
+    const int fhi = flo + (d == 0 ? 1 : (d == 1 ? fx : fx * fy));
+    // phase 0: inviscid faces, phase 1: viscous faces (separate live sets)
+    for (int ph = 0; ph < (S.viscous == 2 ? 2 : 1); ++ph) {
+      for (int f = tid; f < nf; f += NT) {
+        const int c[3] = {i0 + f % fx, j0 + (f / fx) % fy, k0 + f / (fx * fy)};
+        bool ok = true;
 #pragma unroll
-      for (int a = 0; a < 3; ++a) ok = ok && (a == d ? c[a] <= m[a] : c[a] < m[a]);
-      if (!ok) continue;
-      const long long p = pidx(B, c[0] - (d == 0), c[1] - (d == 1), c[2] - (d == 2));
-      double F[NV];
-      inviscid_flux<NS, double>(B, sp, S, d, p, st[d], F);
-      if (S.viscous == 2) {
-        double G[NV];
-        viscous_flux<NS, double>(B, sp, S, d, p, st, G);
+        for (int a = 0; a < 3; ++a) ok = ok && (a == d ? c[a] <= m[a] : c[a] < m[a]);
+        if (!ok) continue;
+        const long long p = pidx(B, c[0] - (d == 0), c[1] - (d == 1), c[2] - (d == 2));
+        double F[NV];
+        if (ph == 0)
+          inviscid_flux<NS, double>(B, sp, S, d, p, st[d], F);
+        else
+          viscous_flux<NS, double>(B, sp, S, d, p, st, F);
 #pragma unroll
-        for (int l = 0; l < NV; ++l) F[l] -= G[l];
+        for (int l = 0; l < NV; ++l) sf[l * nf + f] = F[l];
       }
+      __syncthreads();
+      if (mine) {
+        const double sgn = ph == 0 ? -1.0 : 1.0;
 #pragma unroll
-      for (int l = 0; l < NV; ++l) sf[l * nf + f] = F[l];
+        for (int l = 0; l < NV; ++l) acc[l] += sgn * (sf[l * nf + fhi] - sf[l * nf + flo]);
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    if (mine) {
-      const int flo = ci + fx * (cj + fy * ck);
-      const int fhi = flo + (d == 0 ? 1 : (d == 1 ? fx : fx * fy));
-#pragma unroll
-      for (int l = 0; l < NV; ++l) acc[l] -= sf[l * nf + fhi] - sf[l * nf + flo];
-    }
-    __syncthreads();
   }
   if (mine) {
     const long long q = ((long long)(k0 + ck) * B.my + (j0 + cj)) * B.mx + (i0 + ci);
@@ -684,30 +685,31 @@ int launch_props(const BlockView &B, const SpeciesDev *sp, const SchemeDev &S, i
   return (int)hipGetLastError();
 }
 
+// RHS variants for the autotuner: tile shape x waves per SIMD the register
+// allocation targets.  Index 0 is the default.
+struct RhsVariant {
+  int tx, ty, tz, wpe;
+};
+static const RhsVariant kRhs3[] = {{16, 4, 4, 1}, {8, 8, 4, 1}, {32, 4, 2, 1}, {16, 8, 2, 1}, {64, 2, 2, 1},
+                                   {16, 4, 4, 2}, {8, 8, 4, 2}, {32, 4, 2, 2}, {16, 4, 4, 3}, {8, 8, 2, 2}};
+static const RhsVariant kRhs2[] = {{32, 8, 1, 1}, {16, 16, 1, 1}, {64, 4, 1, 1}, {32, 8, 1, 2}, {16, 16, 1, 2}};
+
+int n_rhs_tiles(int dim) { return dim == 3 ? (int)(sizeof(kRhs3) / sizeof(kRhs3[0])) : (int)(sizeof(kRhs2) / sizeof(kRhs2[0])); }
+
 int rhs_tile(int dim, int tile, int &tx, int &ty, int &tz) {
-  // tile shapes selectable by the autotuner; index 0 is the default
-  static const int t3[][3] = {{16, 4, 4}, {8, 8, 4}, {32, 4, 2}, {16, 8, 2}, {64, 2, 2}};
-  static const int t2[][3] = {{32, 8, 1}, {16, 16, 1}, {64, 4, 1}};
-  if (dim == 3) {
-    if (tile < 0 || tile >= 5) return -1;
-    tx = t3[tile][0], ty = t3[tile][1], tz = t3[tile][2];
-  } else {
-    if (tile < 0 || tile >= 3) return -1;
-    tx = t2[tile][0], ty = t2[tile][1], tz = t2[tile][2];
-  }
+  if (tile < 0 || tile >= n_rhs_tiles(dim)) return -1;
+  const RhsVariant &v = dim == 3 ? kRhs3[tile] : kRhs2[tile];
+  tx = v.tx, ty = v.ty, tz = v.tz;
   return 0;
 }
 
-int n_rhs_tiles(int dim) { return dim == 3 ? 5 : 3; }
-
-template <int NS, int TX, int TY, int TZ>
+template <int NS, int TX, int TY, int TZ, int WPE>
 static int launch_rhs_t(const BlockView &B, const SpeciesDev *sp, const SchemeDev &S, hipStream_t st) {
   const int ntx = (B.mx + TX - 1) / TX, nty = (B.my + TY - 1) / TY, ntz = (B.mz + TZ - 1) / TZ;
-  constexpr int maxf = (TX + 1) * TY * TZ > TX * (TY + 1) * TZ
-                           ? ((TX + 1) * TY * TZ > TX * TY * (TZ + 1) ? (TX + 1) * TY * TZ : TX * TY * (TZ + 1))
-                           : (TX * (TY + 1) * TZ > TX * TY * (TZ + 1) ? TX * (TY + 1) * TZ : TX * TY * (TZ + 1));
+  constexpr int fxn = (TX + 1) * TY * TZ, fyn = TX * (TY + 1) * TZ, fzn = TX * TY * (TZ + 1);
+  constexpr int maxf = fxn > fyn ? (fxn > fzn ? fxn : fzn) : (fyn > fzn ? fyn : fzn);
   const size_t lds = sizeof(double) * (5 + NS) * maxf;
-  rhs_kernel<NS, TX, TY, TZ><<<ntx * nty * ntz, TX * TY * TZ, lds, st>>>(B, sp, S, ntx, nty, ntz);
+  rhs_kernel<NS, TX, TY, TZ, WPE><<<ntx * nty * ntz, TX * TY * TZ, lds, st>>>(B, sp, S, ntx, nty, ntz);
   return (int)hipGetLastError();
 }
 
@@ -715,18 +717,25 @@ template <int NS>
 static int launch_rhs_ns(const BlockView &B, const SpeciesDev *sp, const SchemeDev &S, int tile, hipStream_t st) {
   if (S.dim == 3) {
     switch (tile) {
-      case 0: return launch_rhs_t<NS, 16, 4, 4>(B, sp, S, st);
-      case 1: return launch_rhs_t<NS, 8, 8, 4>(B, sp, S, st);
-      case 2: return launch_rhs_t<NS, 32, 4, 2>(B, sp, S, st);
-      case 3: return launch_rhs_t<NS, 16, 8, 2>(B, sp, S, st);
-      case 4: return launch_rhs_t<NS, 64, 2, 2>(B, sp, S, st);
+      case 0: return launch_rhs_t<NS, 16, 4, 4, 1>(B, sp, S, st);
+      case 1: return launch_rhs_t<NS, 8, 8, 4, 1>(B, sp, S, st);
+      case 2: return launch_rhs_t<NS, 32, 4, 2, 1>(B, sp, S, st);
+      case 3: return launch_rhs_t<NS, 16, 8, 2, 1>(B, sp, S, st);
+      case 4: return launch_rhs_t<NS, 64, 2, 2, 1>(B, sp, S, st);
+      case 5: return launch_rhs_t<NS, 16, 4, 4, 2>(B, sp, S, st);
+      case 6: return launch_rhs_t<NS, 8, 8, 4, 2>(B, sp, S, st);
+      case 7: return launch_rhs_t<NS, 32, 4, 2, 2>(B, sp, S, st);
+      case 8: return launch_rhs_t<NS, 16, 4, 4, 3>(B, sp, S, st);
+      case 9: return launch_rhs_t<NS, 8, 8, 2, 2>(B, sp, S, st);
       default: return CFX_E_ARG;
     }
   }
   switch (tile) {
-    case 0: return launch_rhs_t<NS, 32, 8, 1>(B, sp, S, st);
-    case 1: return launch_rhs_t<NS, 16, 16, 1>(B, sp, S, st);
-    case 2: return launch_rhs_t<NS, 64, 4, 1>(B, sp, S, st);
+    case 0: return launch_rhs_t<NS, 32, 8, 1, 1>(B, sp, S, st);
+    case 1: return launch_rhs_t<NS, 16, 16, 1, 1>(B, sp, S, st);
+    case 2: return launch_rhs_t<NS, 64, 4, 1, 1>(B, sp, S, st);
+    case 3: return launch_rhs_t<NS, 32, 8, 1, 2>(B, sp, S, st);
+    case 4: return launch_rhs_t<NS, 16, 16, 1, 2>(B, sp, S, st);
     default: return CFX_E_ARG;
   }
 }

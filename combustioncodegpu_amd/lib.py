@@ -41,6 +41,10 @@ def load():
     lib.cfx_kernel_times.restype = C.c_int
     lib.cfx_set_profiling.argtypes = [vp, C.c_int]
     lib.cfx_set_profiling.restype = C.c_int
+    lib.cfx_rhs_variants.argtypes = [vp]
+    lib.cfx_rhs_variants.restype = C.c_int
+    lib.cfx_set_rhs_variant.argtypes = [vp, C.c_int]
+    lib.cfx_set_rhs_variant.restype = C.c_int
     _lib = lib
     return lib
 
@@ -66,6 +70,12 @@ class Solver(StepContext):
 
     def set_profiling(self, on=True):
         self._call("set_profiling", 1 if on else 0)
+
+    def rhs_variants(self):
+        return self.lib.cfx_rhs_variants(self.h)
+
+    def set_rhs_variant(self, v):
+        self._call("set_rhs_variant", int(v))
 
     def kernel_times(self):
         out = {}

@@ -229,6 +229,12 @@ int cfx_time_steps(cfx_ctx *ctx, int n_steps, double *ms_total);
 int cfx_kernel_times(cfx_ctx *ctx, int cls, double *ms, long long *launches);
 int cfx_set_profiling(cfx_ctx *ctx, int on);
 
+/* Kernel-variant selection used by the autotuner (no counterpart in the
+ * reference): the number of compiled variants of the RHS kernel (tile shape
+ * x target occupancy) for the context's dimension, and the selection. */
+int cfx_rhs_variants(cfx_ctx *ctx);
+int cfx_set_rhs_variant(cfx_ctx *ctx, int variant);
+
 #ifdef __cplusplus
 }
 #endif

@@ -110,8 +110,13 @@ def test_multi_step_parity(name, steps):
     o.step(steps)
     g.step(steps)
     compare_state(case, o, g, TOL_STEP * steps, f"{steps} steps")
+    # residuals are RMS of step differences (d rho, d|V|, d p, d T), i.e.
+    # cancellations of O(q) values: compare against the state bar times q
     ro, rg = o.residual(), g.residual()
-    np.testing.assert_allclose(rg, ro, rtol=1e-9, atol=1e-300)
+    bv = o.download(0, abi.BV)
+    q = np.array([np.abs(bv[0]).max(), np.sqrt((bv[1:4] ** 2).sum(axis=0)).max(), np.abs(bv[4]).max(),
+                  np.abs(bv[5]).max()])
+    assert np.all(np.abs(rg - ro) <= TOL_STEP * steps * q), (rg, ro)
     assert abs(g.get_dt() - o.get_dt()) <= 1e-12 * o.get_dt()
 
 
