@@ -280,3 +280,47 @@ def cell_stages(case: Case, steps: int) -> int:
     """Interior cells x stages x steps (the metric's work unit)."""
     stages = 3 if case.params.get("rk_stages", 3) == 3 else 1
     return case.n_cells * stages * steps
+
+
+def split_i(case: Case, ma: int) -> Case:
+    """Split a single-block case along i into two blocks that share the
+    vertex layer i = ma-1, joined by an inner face in the reference's inner
+    file convention (1-based, the first in-plane axis flagged negative).
+    Only blocks whose six faces are all outflow are supported."""
+    assert len(case.blocks) == 1
+    b = case.blocks[0]
+    assert all(t == abi.BC_OUTFLOW for _, _, t, _ in b.bc)
+    mx, my, mz = b.mx, b.my, b.mz
+    mb = mx - ma + 1
+    shape = (mz, my, mx)
+    xs = [a.reshape(shape) for a in (b.x, b.y, b.z)]
+    xa = [np.ascontiguousarray(a[:, :, :ma]).ravel() for a in xs]
+    xb = [np.ascontiguousarray(a[:, :, ma - 1:]).ravel() for a in xs]
+    dim3 = mz > 1
+    zr = (1, mz) if dim3 else (1, 1)
+
+    def faces(m_x, keep_min, keep_max):
+        out = []
+        for f in outflow_all_faces(m_x, my, mz):
+            rs, re = f[0], f[1]
+            if rs[0] == re[0] == 0 and not keep_min:
+                continue
+            if rs[0] == re[0] == m_x - 1 and not keep_max:
+                continue
+            if not dim3 and rs[2] == re[2] and (rs[0] != re[0] and rs[1] != re[1]):
+                continue
+            out.append(f)
+        return out
+
+    inner_a = ([ma, ma, -1, -my, zr[0], zr[1]], [1, 1, -1, -my, zr[0], zr[1]], 2)
+    inner_b = ([1, 1, -1, -my, zr[0], zr[1]], [ma, ma, -1, -my, zr[0], zr[1]], 1)
+    blk_a = BlockSpec(ma, my, mz, *xa, bc=faces(ma, True, False), inner=[inner_a])
+    blk_b = BlockSpec(mb, my, mz, *xb, bc=faces(mb, False, True), inner=[inner_b])
+    bv, yk = case.bv0[0], case.yk0[0]
+    sl_a = np.s_[..., 0:ma + 2 * NGG]
+    sl_b = np.s_[..., ma - 1:ma - 1 + mb + 2 * NGG]
+    bv_a, bv_b = np.ascontiguousarray(bv[sl_a]), np.ascontiguousarray(bv[sl_b])
+    yk_a = None if yk is None else np.ascontiguousarray(yk[sl_a])
+    yk_b = None if yk is None else np.ascontiguousarray(yk[sl_b])
+    return Case(case.name + "_split", dict(case.params), case.species, [blk_a, blk_b], case.inflow, case.wall,
+                [bv_a, bv_b], [yk_a, yk_b])

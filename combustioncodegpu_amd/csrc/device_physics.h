@@ -13,6 +13,13 @@
 
 namespace cfx {
 
+// The species table is read through the constant address space: every index
+// into it is uniform across the wavefront, so the loads become scalar loads
+// into SGPRs instead of vector loads into VGPRs.
+using SpeciesPtr = const __attribute__((address_space(4))) SpeciesDev *;
+
+__device__ __forceinline__ SpeciesPtr const_table(const SpeciesDev *p) { return (SpeciesPtr)p; }
+
 template <typename R>
 struct Tiny;
 template <>
@@ -29,16 +36,14 @@ struct Tiny<float> {
 // Horner form with the low/high set chosen at t_mid and the linear enthalpy
 // extension with frozen cp below t_low.
 template <typename R>
-__device__ __forceinline__ void species_h_cp(const SpeciesDev *__restrict__ sp, int l, R t, R &h, R &cp) {
+__device__ __forceinline__ void species_h_cp(SpeciesPtr sp, int l, R t, R &h, R &cp) {
   const bool hi = t >= R(sp->t_mid[l]);
-  const double *c = sp->cpc[0][l];
-  const double *ch = sp->cpc[1][l];
-  const double *e = sp->hc[0][l];
-  const double *eh = sp->hc[1][l];
-  const R c0 = hi ? R(ch[0]) : R(c[0]), c1 = hi ? R(ch[1]) : R(c[1]), c2 = hi ? R(ch[2]) : R(c[2]),
-          c3 = hi ? R(ch[3]) : R(c[3]), c4 = hi ? R(ch[4]) : R(c[4]);
-  const R e0 = hi ? R(eh[0]) : R(e[0]), e1 = hi ? R(eh[1]) : R(e[1]), e2 = hi ? R(eh[2]) : R(e[2]),
-          e3 = hi ? R(eh[3]) : R(e[3]), e4 = hi ? R(eh[4]) : R(e[4]), e5 = hi ? R(eh[5]) : R(e[5]);
+  const R c0 = hi ? R(sp->cpc[1][l][0]) : R(sp->cpc[0][l][0]), c1 = hi ? R(sp->cpc[1][l][1]) : R(sp->cpc[0][l][1]),
+          c2 = hi ? R(sp->cpc[1][l][2]) : R(sp->cpc[0][l][2]), c3 = hi ? R(sp->cpc[1][l][3]) : R(sp->cpc[0][l][3]),
+          c4 = hi ? R(sp->cpc[1][l][4]) : R(sp->cpc[0][l][4]);
+  const R e0 = hi ? R(sp->hc[1][l][0]) : R(sp->hc[0][l][0]), e1 = hi ? R(sp->hc[1][l][1]) : R(sp->hc[0][l][1]),
+          e2 = hi ? R(sp->hc[1][l][2]) : R(sp->hc[0][l][2]), e3 = hi ? R(sp->hc[1][l][3]) : R(sp->hc[0][l][3]),
+          e4 = hi ? R(sp->hc[1][l][4]) : R(sp->hc[0][l][4]), e5 = hi ? R(sp->hc[1][l][5]) : R(sp->hc[0][l][5]);
   const R cpp = c0 + t * (c1 + t * (c2 + t * (c3 + t * c4)));
   const R hp = t * (e0 + t * (e1 + t * (e2 + t * (e3 + t * e4)))) + e5;
   const R tl = R(sp->t_low[l]);
@@ -49,10 +54,12 @@ __device__ __forceinline__ void species_h_cp(const SpeciesDev *__restrict__ sp, 
 
 // cp_l(T) only, frozen below t_low (Thermo.cu:68-82)
 template <typename R>
-__device__ __forceinline__ R species_cp(const SpeciesDev *__restrict__ sp, int l, R t) {
+__device__ __forceinline__ R species_cp(SpeciesPtr sp, int l, R t) {
   const bool hi = t >= R(sp->t_mid[l]);
-  const double *c = sp->cpc[hi ? 1 : 0][l];
-  const R cpp = R(c[0]) + t * (R(c[1]) + t * (R(c[2]) + t * (R(c[3]) + t * R(c[4]))));
+  const R c0 = hi ? R(sp->cpc[1][l][0]) : R(sp->cpc[0][l][0]), c1 = hi ? R(sp->cpc[1][l][1]) : R(sp->cpc[0][l][1]),
+          c2 = hi ? R(sp->cpc[1][l][2]) : R(sp->cpc[0][l][2]), c3 = hi ? R(sp->cpc[1][l][3]) : R(sp->cpc[0][l][3]),
+          c4 = hi ? R(sp->cpc[1][l][4]) : R(sp->cpc[0][l][4]);
+  const R cpp = c0 + t * (c1 + t * (c2 + t * (c3 + t * c4)));
   return t < R(sp->t_low[l]) ? R(sp->cp_tlow[l]) : cpp;
 }
 
@@ -85,22 +92,14 @@ __device__ __forceinline__ void muscl(const SchemeDev &S, R q0, R q1, R q2, R q3
   }
 }
 
-// ------------------------------------------------------- inviscid flux ----
-// AUSM+ flux of the face between padded points p and p+s in direction d
-// (InviscidScheme.cu:220-304) with the face states from MUSCL
-// (Reconstruction.cu:79-122).  F has 5+NS entries.
+// AUSM+ flux (InviscidScheme.cu:235-304) from the reconstructed face states
+// pl/pr = (rho,u,v,w,p,Y..) and the face area vector k = 0.5(J_L grad_L +
+// J_R grad_R) of the face direction; the face temperature, energy and gamma
+// follow Reconstruction.cu:79-122.
 template <int NS, typename R>
-__device__ __forceinline__ void inviscid_flux(const BlockView &B, const SpeciesDev *__restrict__ sp,
-                                              const SchemeDev &S, int d, long long p, long long s, R *F) {
+__device__ __forceinline__ void ausm_from_states(SpeciesPtr sp, const SchemeDev &S, const R *pl,
+                                                 const R *pr, R k1, R k2, R k3, R *F) {
   constexpr int NV = 5 + NS;
-  const long long np = B.np;
-  R pl[NV], pr[NV];
-#pragma unroll
-  for (int l = 0; l < NV; ++l) {
-    const double *q = l < 5 ? B.bv + l * np : B.yk + (l - 5) * np;
-    const R q0 = R(q[p - s]), q1 = R(q[p]), q2 = R(q[p + s]), q3 = R(q[p + 2 * s]);
-    muscl<R>(S, q0, q1, q2, q3, pl[l], pr[l]);
-  }
   R el = R(0.5) * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]);
   R er = R(0.5) * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]);
   R gam_l, gam_r, E_l, E_r;
@@ -135,11 +134,6 @@ __device__ __forceinline__ void inviscid_flux(const BlockView &B, const SpeciesD
     E_r = er * pr[0] + pr[4] * g1;
     gam_l = gam_r = R(S.gamma_air);
   }
-  const double *M = B.metric + 3 * d * np;
-  const R jl = R(B.jac[p]), jr = R(B.jac[p + s]);
-  const R k1 = R(0.5) * (jl * R(M[p]) + jr * R(M[p + s]));
-  const R k2 = R(0.5) * (jl * R(M[np + p]) + jr * R(M[np + p + s]));
-  const R k3 = R(0.5) * (jl * R(M[2 * np + p]) + jr * R(M[2 * np + p + s]));
   const R gk = sqrt(k1 * k1 + k2 * k2 + k3 * k3);
   const R igk = R(1) / gk;
   const R ul = (k1 * pl[1] + k2 * pl[2] + k3 * pl[3]) * igk;
@@ -184,13 +178,37 @@ __device__ __forceinline__ void inviscid_flux(const BlockView &B, const SpeciesD
   for (int l = 5; l < NV; ++l) F[l] = coeff * (left ? pl[l] : pr[l]);
 }
 
+// ------------------------------------------------------- inviscid flux ----
+// AUSM+ flux of the face between padded points p and p+s in direction d
+// (InviscidScheme.cu:220-304) with the face states from MUSCL
+// (Reconstruction.cu:79-122).  F has 5+NS entries.
+template <int NS, typename R>
+__device__ __forceinline__ void inviscid_flux(const BlockView &B, SpeciesPtr sp,
+                                              const SchemeDev &S, int d, long long p, long long s, R *F) {
+  constexpr int NV = 5 + NS;
+  const long long np = B.np;
+  R pl[NV], pr[NV];
+#pragma unroll
+  for (int l = 0; l < NV; ++l) {
+    const double *q = l < 5 ? B.bv + l * np : B.yk + (l - 5) * np;
+    const R q0 = R(q[p - s]), q1 = R(q[p]), q2 = R(q[p + s]), q3 = R(q[p + 2 * s]);
+    muscl<R>(S, q0, q1, q2, q3, pl[l], pr[l]);
+  }
+  const double *M = B.metric + 3 * d * np;
+  const R jl = R(B.jac[p]), jr = R(B.jac[p + s]);
+  const R k1 = R(0.5) * (jl * R(M[p]) + jr * R(M[p + s]));
+  const R k2 = R(0.5) * (jl * R(M[np + p]) + jr * R(M[np + p + s]));
+  const R k3 = R(0.5) * (jl * R(M[2 * np + p]) + jr * R(M[2 * np + p + s]));
+  ausm_from_states<NS, R>(sp, S, pl, pr, k1, k2, k3, F);
+}
+
 // -------------------------------------------------------- viscous flux ----
 // Central 2nd-order viscous flux of the face between p and p+s_d
 // (SecOrdViscScheme::compute_fv/gv/hv, ViscousScheme.cu:11-342), generic in
 // the face direction d.  Quirk bits reproduce the reference's face-metric
 // averaging (fv, hv) and the (i,j+1,k) rho*D neighbour of hv.
 template <int NS, typename R>
-__device__ __forceinline__ void viscous_flux(const BlockView &B, const SpeciesDev *__restrict__ sp,
+__device__ __forceinline__ void viscous_flux(const BlockView &B, SpeciesPtr sp,
                                              const SchemeDev &S, int d, long long p, const long long st[3],
                                              R *G) {
   const long long np = B.np;

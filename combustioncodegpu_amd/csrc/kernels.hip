@@ -6,6 +6,7 @@
 
 #include "device_physics.h"
 #include "kernels.h"
+#include "rhs_pencil.h"
 
 namespace cfx {
 
@@ -37,7 +38,7 @@ __device__ __forceinline__ int xcd_tile(int b, int nt) {
 // ---------------------------------------------------------- total energy ---
 // compute_total_energy (Thermo.cu:85-105) from bv/yk/cv at padded point p.
 template <int NS>
-__device__ __forceinline__ void total_energy(const BlockView &B, const SpeciesDev *__restrict__ sp,
+__device__ __forceinline__ void total_energy(const BlockView &B, SpeciesPtr sp,
                                              const SchemeDev &S, long long p) {
   const long long np = B.np;
   const double u = B.bv[np + p], v = B.bv[2 * np + p], w = B.bv[3 * np + p];
@@ -64,7 +65,8 @@ __device__ __forceinline__ void total_energy(const BlockView &B, const SpeciesDe
 // (NS square roots instead of NS^2) and T*^-0.145 = exp(-0.145 log T*) with
 // one shared log T.
 template <int NS>
-__global__ void __launch_bounds__(256) props_kernel(BlockView B, const SpeciesDev *__restrict__ sp, SchemeDev S) {
+__global__ void __launch_bounds__(256) props_kernel(BlockView B, const SpeciesDev *__restrict__ spg, SchemeDev S) {
+  const SpeciesPtr sp = const_table(spg);
   const long long np = B.np;
   for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (long long)gridDim.x * blockDim.x) {
     const double t = B.bv[5 * np + p];
@@ -136,8 +138,9 @@ __global__ void __launch_bounds__(256) props_kernel(BlockView B, const SpeciesDe
 // computed by a flat loop over the workgroup (each face once), staged in LDS,
 // and differenced into per-thread accumulators; dq is written once.
 template <int NS, int TX, int TY, int TZ, int WPE>
-__global__ void __launch_bounds__(TX *TY *TZ) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) rhs_kernel(BlockView B, const SpeciesDev *__restrict__ sp, SchemeDev S,
+__global__ void __launch_bounds__(TX *TY *TZ) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) rhs_kernel(BlockView B, const SpeciesDev *__restrict__ spg, SchemeDev S,
                                                          int ntx, int nty, int ntz) {
+  const SpeciesPtr sp = const_table(spg);
   constexpr int NV = 5 + NS;
   constexpr int NT = TX * TY * TZ;
   extern __shared__ double sf[];
@@ -246,9 +249,10 @@ __global__ void set_u64_kernel(unsigned long long *dst, unsigned long long v) { 
 // combination q = a*qn + b*(q + dt*dq/J) and the Newton inversion of T
 // (Thermo.cu:108-144, stop |dT| <= 1e-10 K unless CFX_Q_NEWTON_REL).
 template <int NS>
-__global__ void __launch_bounds__(256) update_kernel(BlockView B, const SpeciesDev *__restrict__ sp, SchemeDev S,
+__global__ void __launch_bounds__(256) update_kernel(BlockView B, const SpeciesDev *__restrict__ spg, SchemeDev S,
                                                      int stage, const double *__restrict__ dt_dev, int *flags,
                                                      unsigned long long *newton_fail) {
+  const SpeciesPtr sp = const_table(spg);
   constexpr int NV = 5 + NS;
   const long long np = B.np, ni = B.ni;
   const double A = stage == 1 ? 0.75 : (stage == 2 ? 1.0 / 3.0 : 0.0);
@@ -335,7 +339,8 @@ __global__ void __launch_bounds__(256) update_kernel(BlockView B, const SpeciesD
 // ------------------------------------------------------------ init / last ---
 // compute_cv_from_bv (Field.cu:242-268) over the padded range
 template <int NS>
-__global__ void cv_from_bv_kernel(BlockView B, const SpeciesDev *__restrict__ sp, SchemeDev S) {
+__global__ void cv_from_bv_kernel(BlockView B, const SpeciesDev *__restrict__ spg, SchemeDev S) {
+  const SpeciesPtr sp = const_table(spg);
   const long long np = B.np;
   for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (long long)gridDim.x * blockDim.x) {
     const double rho = B.bv[p];
@@ -467,7 +472,7 @@ __global__ void bc_inflow_kernel(BlockView B, FaceRange f, int ns, const InflowD
 }
 
 template <int NS>
-__device__ __forceinline__ double mix_mw(const BlockView &B, const SpeciesDev *__restrict__ sp, const SchemeDev &S,
+__device__ __forceinline__ double mix_mw(const BlockView &B, SpeciesPtr sp, const SchemeDev &S,
                                          long long p) {
   if constexpr (NS == 0) {
     return kRu / S.r_air;
@@ -482,8 +487,9 @@ __device__ __forceinline__ double mix_mw(const BlockView &B, const SpeciesDev *_
 // apply_wall (BoundCond.cu:529-629) over the face's own vertex range
 // (the registered range shrunk back by ngg, BoundCond.cu:229-239).
 template <int NS>
-__global__ void bc_wall_kernel(BlockView B, FaceRange f, const SpeciesDev *__restrict__ sp, SchemeDev S,
+__global__ void bc_wall_kernel(BlockView B, FaceRange f, const SpeciesDev *__restrict__ spg, SchemeDev S,
                                WallDev wl) {
+  const SpeciesPtr sp = const_table(spg);
   int i, j, k;
   if (!face_point(f, i, j, k)) return;
   int dir[3] = {0, 0, 0};
@@ -544,7 +550,8 @@ __global__ void bc_wall_kernel(BlockView B, FaceRange f, const SpeciesDev *__res
 // n = grad(xi_face)/|grad(xi_face)|; ghost g mirrors interior point g with
 // the normal velocity negated and rho, p, T, Y copied.
 template <int NS>
-__global__ void bc_symmetry_kernel(BlockView B, FaceRange f, const SpeciesDev *__restrict__ sp, SchemeDev S) {
+__global__ void bc_symmetry_kernel(BlockView B, FaceRange f, const SpeciesDev *__restrict__ spg, SchemeDev S) {
+  const SpeciesPtr sp = const_table(spg);
   int i, j, k;
   if (!face_point(f, i, j, k)) return;
   int dir[3] = {0, 0, 0};
@@ -685,20 +692,40 @@ int launch_props(const BlockView &B, const SpeciesDev *sp, const SchemeDev &S, i
   return (int)hipGetLastError();
 }
 
-// RHS variants for the autotuner: tile shape x waves per SIMD the register
-// allocation targets.  Index 0 is the default.
-struct RhsVariant {
+// RHS variants for the autotuner.  Pencil variants (one LDS-staged kernel
+// per face direction, tile TA x TB for each direction) come first; the fused
+// all-direction kernel with its tile shape and target occupancy follows.
+struct FusedVariant {
   int tx, ty, tz, wpe;
 };
-static const RhsVariant kRhs3[] = {{16, 4, 4, 1}, {8, 8, 4, 1}, {32, 4, 2, 1}, {16, 8, 2, 1}, {64, 2, 2, 1},
-                                   {16, 4, 4, 2}, {8, 8, 4, 2}, {32, 4, 2, 2}, {16, 4, 4, 3}, {8, 8, 2, 2}};
-static const RhsVariant kRhs2[] = {{32, 8, 1, 1}, {16, 16, 1, 1}, {64, 4, 1, 1}, {32, 8, 1, 2}, {16, 16, 1, 2}};
+// pencil variants: per face direction d the tile ta[d] x tb[d] x tc[d]
+struct PencilVariant {
+  int ta[3], tb[3], tc[3], wpe;
+};
+static const PencilVariant kPencil3[] = {{{32, 16, 16}, {4, 8, 8}, {2, 2, 2}, 2},
+                                         {{32, 16, 16}, {8, 16, 16}, {1, 1, 1}, 2},
+                                         {{64, 32, 32}, {2, 4, 4}, {2, 2, 2}, 2},
+                                         {{32, 16, 16}, {4, 8, 8}, {2, 2, 2}, 1}};
+static const PencilVariant kPencil2[] = {{{32, 16, 0}, {8, 16, 0}, {1, 1, 0}, 2},
+                                         {{64, 32, 0}, {4, 8, 0}, {1, 1, 0}, 2},
+                                         {{32, 16, 0}, {8, 16, 0}, {1, 1, 0}, 1}};
+static constexpr int kNPencil = 4;  // slots reserved for pencil variants in both dimensions
+static const FusedVariant kRhs3[] = {{16, 4, 4, 1}, {8, 8, 4, 1}, {32, 4, 2, 1}, {16, 8, 2, 1}, {64, 2, 2, 1},
+                                     {16, 4, 4, 2}, {8, 8, 4, 2}, {32, 4, 2, 2}, {16, 4, 4, 3}, {8, 8, 2, 2}};
+static const FusedVariant kRhs2[] = {{32, 8, 1, 1}, {16, 16, 1, 1}, {64, 4, 1, 1}, {32, 8, 1, 2}, {16, 16, 1, 2}};
 
-int n_rhs_tiles(int dim) { return dim == 3 ? (int)(sizeof(kRhs3) / sizeof(kRhs3[0])) : (int)(sizeof(kRhs2) / sizeof(kRhs2[0])); }
+int n_rhs_tiles(int dim) {
+  return kNPencil + (dim == 3 ? (int)(sizeof(kRhs3) / sizeof(kRhs3[0])) : (int)(sizeof(kRhs2) / sizeof(kRhs2[0])));
+}
 
 int rhs_tile(int dim, int tile, int &tx, int &ty, int &tz) {
   if (tile < 0 || tile >= n_rhs_tiles(dim)) return -1;
-  const RhsVariant &v = dim == 3 ? kRhs3[tile] : kRhs2[tile];
+  if (tile < kNPencil) {
+    const PencilVariant &v = dim == 3 ? kPencil3[tile] : kPencil2[tile < 3 ? tile : 2];
+    tx = v.ta[0], ty = v.tb[0], tz = v.tc[0];
+    return 0;
+  }
+  const FusedVariant &v = dim == 3 ? kRhs3[tile - kNPencil] : kRhs2[tile - kNPencil];
   tx = v.tx, ty = v.ty, tz = v.tz;
   return 0;
 }
@@ -713,8 +740,48 @@ static int launch_rhs_t(const BlockView &B, const SpeciesDev *sp, const SchemeDe
   return (int)hipGetLastError();
 }
 
+template <int NS, int DIR, int TA, int TB, int TC, int WPE>
+static int launch_pencil_t(const BlockView &B, const SpeciesDev *sp, const SchemeDev &S, int accumulate,
+                           hipStream_t st) {
+  constexpr int A = Axes<DIR>::a, Bx = Axes<DIR>::b, Cx = Axes<DIR>::c;
+  const int m[3] = {B.mx, B.my, B.mz};
+  const int nta = (m[A] + TA - 2) / (TA - 1);  // TA faces cover TA-1 cells
+  const int ntb = (m[Bx] + TB - 1) / TB;
+  const int ntc = (m[Cx] + TC - 1) / TC;
+  rhs_pencil_kernel<NS, DIR, TA, TB, TC, WPE><<<nta * ntb * ntc, TA * TB * TC, 0, st>>>(B, sp, S, accumulate, nta, ntb,
+                                                                                     ntc);
+  return (int)hipGetLastError();
+}
+
+template <int NS, int TA0, int TB0, int TC0, int TA1, int TB1, int TC1, int TA2, int TB2, int TC2, int WPE>
+static int launch_pencils(const BlockView &B, const SpeciesDev *sp, const SchemeDev &S, hipStream_t st) {
+  int rc = launch_pencil_t<NS, 0, TA0, TB0, TC0, WPE>(B, sp, S, 0, st);
+  if (rc) return rc;
+  rc = launch_pencil_t<NS, 1, TA1, TB1, TC1, WPE>(B, sp, S, 1, st);
+  if (rc || S.dim == 2) return rc;
+  return launch_pencil_t<NS, 2, TA2, TB2, TC2, WPE>(B, sp, S, 1, st);
+}
+
 template <int NS>
 static int launch_rhs_ns(const BlockView &B, const SpeciesDev *sp, const SchemeDev &S, int tile, hipStream_t st) {
+  if (S.dim == 3) {
+    switch (tile) {
+      case 0: return launch_pencils<NS, 32, 4, 2, 16, 8, 2, 16, 8, 2, 2>(B, sp, S, st);
+      case 1: return launch_pencils<NS, 32, 8, 1, 16, 16, 1, 16, 16, 1, 2>(B, sp, S, st);
+      case 2: return launch_pencils<NS, 64, 2, 2, 32, 4, 2, 32, 4, 2, 2>(B, sp, S, st);
+      case 3: return launch_pencils<NS, 32, 4, 2, 16, 8, 2, 16, 8, 2, 1>(B, sp, S, st);
+      default: break;
+    }
+  } else {
+    switch (tile) {
+      case 0: return launch_pencils<NS, 32, 8, 1, 16, 16, 1, 16, 16, 1, 2>(B, sp, S, st);
+      case 1: return launch_pencils<NS, 64, 4, 1, 32, 8, 1, 32, 8, 1, 2>(B, sp, S, st);
+      case 2:
+      case 3: return launch_pencils<NS, 32, 8, 1, 16, 16, 1, 16, 16, 1, 1>(B, sp, S, st);
+      default: break;
+    }
+  }
+  tile -= kNPencil;
   if (S.dim == 3) {
     switch (tile) {
       case 0: return launch_rhs_t<NS, 16, 4, 4, 1>(B, sp, S, st);

@@ -1,0 +1,293 @@
+// rhs_pencil.h — the RHS of one face direction per launch, with the stencil
+// staged through LDS in batches of variables.
+//
+// A workgroup owns a tile of TA x TB x TC faces (TA along the face direction
+// "a", TB/TC along the two other directions "b"/"c" in increasing axis
+// order).  Consecutive faces along a share a cell, so TA faces difference
+// into TA-1 cells and neighbouring tiles overlap by one face.
+//
+// Inviscid phase (InviscidScheme.cu:220-304 via Reconstruction.cu:45-122):
+// the 5+NS reconstructed variables are staged over the box (TA+3) x TB x TC
+// (the 4-point MUSCL stencil along a), KI variables per pair of barriers.
+// Viscous phase (ViscousScheme.cu:11-342): u, v, w, T and the mass fractions
+// are staged over (TA+1) x (TB+2) x (TC+2) (the face points plus the
+// tangential neighbours of the 4-point averages), KV per batch.  Face-point
+// data (metric, Jacobian, mu, kappa, rho*D) is read directly with coalesced
+// loads: threads are laid out x-fastest for every direction.  Each thread
+// keeps its face flux F_inv - F_visc in an LDS slot; the cell difference is
+// taken once at the end and dq written (first direction) or accumulated.
+#pragma once
+
+#include "device_physics.h"
+
+namespace cfx {
+
+template <int DIR>
+struct Axes;
+template <>
+struct Axes<0> {
+  static constexpr int a = 0, b = 1, c = 2;
+};
+template <>
+struct Axes<1> {
+  static constexpr int a = 1, b = 0, c = 2;
+};
+template <>
+struct Axes<2> {
+  static constexpr int a = 2, b = 0, c = 1;
+};
+
+// A staging box: extents (Ea, Eb, Ec) along the face axes a, b, c, laid out
+// x-fastest in LDS.
+template <int DIR, int Ea, int Eb, int Ec>
+struct Box {
+  static constexpr int A = Axes<DIR>::a, Bx = Axes<DIR>::b, Cx = Axes<DIR>::c;
+  static constexpr int L0 = A == 0 ? Ea : (Bx == 0 ? Eb : Ec);
+  static constexpr int L1 = A == 1 ? Ea : (Bx == 1 ? Eb : Ec);
+  static constexpr int L2 = A == 2 ? Ea : (Bx == 2 ? Eb : Ec);
+  static constexpr int V = L0 * L1 * L2;
+  static constexpr int str[3] = {1, L0, L0 * L1};
+  static constexpr int sa = str[A], sb = str[Bx], sc = str[Cx];
+};
+
+template <int NS, int DIR, int TA, int TB, int TC>
+struct PencilShape {
+  static constexpr int A = Axes<DIR>::a, Bx = Axes<DIR>::b, Cx = Axes<DIR>::c;
+  static constexpr int NV = 5 + NS;
+  static constexpr int NT = TA * TB * TC;
+  static constexpr int T0 = A == 0 ? TA : (Bx == 0 ? TB : TC);
+  static constexpr int T1 = A == 1 ? TA : (Bx == 1 ? TB : TC);
+  static constexpr int T2 = A == 2 ? TA : (Bx == 2 ? TB : TC);
+  static constexpr int tx[3] = {1, T0, T0 * T1};
+  using BI = Box<DIR, TA + 3, TB, TC>;
+  using BV = Box<DIR, TA + 1, TB + 2, TC + 2>;
+  static constexpr int KI = NV < 7 ? NV : 7;  // inviscid variables per batch
+  static constexpr int KV = 4;                // viscous variables per batch
+  static constexpr int STG = KI * BI::V > KV * BV::V ? KI * BI::V : KV * BV::V;
+};
+
+template <int NS, int DIR, int TA, int TB, int TC, int WPE>
+__global__ void __launch_bounds__(TA *TB *TC) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
+    rhs_pencil_kernel(BlockView B, const SpeciesDev *__restrict__ spg, SchemeDev S, int accumulate, int nta, int ntb,
+                      int ntc) {
+  const SpeciesPtr sp = const_table(spg);
+  using P = PencilShape<NS, DIR, TA, TB, TC>;
+  using BI = typename P::BI;
+  using BV = typename P::BV;
+  constexpr int NV = P::NV, NT = P::NT, A = P::A, Bx = P::Bx, Cx = P::Cx;
+  constexpr int ta_stride = P::tx[A];
+  __shared__ double stg[P::STG];    // a batch of staged variables
+  __shared__ double fbuf[NV * NT];  // each thread's face flux F_inv - F_visc
+
+  const long long np = B.np;
+  const int m[3] = {B.mx, B.my, B.mz};
+  const long long st[3] = {1, B.sy, B.sz};
+  const int nt = nta * ntb * ntc;
+  const int bid = blockIdx.x;
+  const int lid = nt % 8 == 0 ? (bid % 8) * (nt / 8) + bid / 8 : bid;
+  int o[3];
+  o[A] = (lid % nta) * (TA - 1);
+  o[Bx] = ((lid / nta) % ntb) * TB;
+  o[Cx] = (lid / (nta * ntb)) * TC;
+  const int tid = threadIdx.x;
+  int t[3];
+  t[0] = tid % P::T0;
+  t[1] = (tid / P::T0) % P::T1;
+  t[2] = tid / (P::T0 * P::T1);
+  int g[3];
+  g[A] = o[A] - 1 + t[A];
+  g[Bx] = o[Bx] + t[Bx];
+  g[Cx] = o[Cx] + t[Cx];
+  const bool face_ok = g[A] + 1 <= m[A] && g[Bx] < m[Bx] && g[Cx] < m[Cx];
+  const bool cell_ok = face_ok && t[A] <= TA - 2 && g[A] + 1 < m[A];
+  const long long pl_i = face_ok ? (long long)(g[2] + kNgg) * B.sz + (long long)(g[1] + kNgg) * B.sy + (g[0] + kNgg) : 0;
+  const long long pr_i = pl_i + st[A];
+
+  // stage K variables (source pointers) over box BX with global origin bo;
+  // points beyond the padded range are left unwritten (no valid face reads them)
+  auto stage = [&](auto box, const double *const *src, int K, const int bo[3]) {
+    using BX = decltype(box);
+    __syncthreads();
+    for (int e = tid; e < BX::V; e += NT) {
+      const int x = bo[0] + e % BX::L0;
+      const int y = bo[1] + (e / BX::L0) % BX::L1;
+      const int z = bo[2] + e / (BX::L0 * BX::L1);
+      if (x <= m[0] + 1 && y <= m[1] + 1 && z <= m[2] + 1) {
+        const long long gi = (long long)(z + kNgg) * B.sz + (long long)(y + kNgg) * B.sy + (x + kNgg);
+        for (int k = 0; k < K; ++k) stg[k * BX::V + e] = src[k][gi];
+      }
+    }
+    __syncthreads();
+  };
+
+  double k1 = 0, k2 = 0, k3 = 0;
+
+  // ---- inviscid phase: F_inv into this thread's face slot
+  {
+    int bo[3];
+    bo[A] = o[A] - 2;
+    bo[Bx] = o[Bx];
+    bo[Cx] = o[Cx];
+    int l[3];
+    l[A] = t[A] + 1;
+    l[Bx] = t[Bx];
+    l[Cx] = t[Cx];
+    const int li = l[0] + BI::L0 * (l[1] + BI::L1 * l[2]);
+    constexpr int sa = BI::sa;
+    double pl[NV], pr[NV];
+#pragma unroll
+    for (int v0 = 0; v0 < NV; v0 += P::KI) {
+      const double *src[P::KI];
+      const int K = NV - v0 < P::KI ? NV - v0 : P::KI;
+#pragma unroll
+      for (int k = 0; k < P::KI; ++k) {
+        const int v = v0 + k < NV ? v0 + k : NV - 1;
+        src[k] = v < 5 ? B.bv + v * np : B.yk + (v - 5) * np;
+      }
+      stage(BI{}, src, K, bo);
+#pragma unroll
+      for (int k = 0; k < P::KI; ++k) {
+        if (v0 + k >= NV) break;
+        const double *q = stg + k * BI::V + li;
+        muscl<double>(S, q[-sa], q[0], q[sa], q[2 * sa], pl[v0 + k], pr[v0 + k]);
+      }
+    }
+    double F[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) F[v] = 0.0;
+    if (face_ok) {
+      const double *M = B.metric + 3 * A * np;
+      const double jl = B.jac[pl_i], jr = B.jac[pr_i];
+      k1 = 0.5 * (jl * M[pl_i] + jr * M[pr_i]);
+      k2 = 0.5 * (jl * M[np + pl_i] + jr * M[np + pr_i]);
+      k3 = 0.5 * (jl * M[2 * np + pl_i] + jr * M[2 * np + pr_i]);
+      ausm_from_states<NS, double>(sp, S, pl, pr, k1, k2, k3, F);
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) fbuf[v * NT + tid] = F[v];
+  }
+
+  // ---- viscous phase: F_visc subtracted from the slot as it is formed
+  if (S.viscous == 2) {
+    int bo[3];
+    bo[A] = o[A] - 1;
+    bo[Bx] = o[Bx] - 1;
+    bo[Cx] = o[Cx] - 1;
+    int l[3];
+    l[A] = t[A];
+    l[Bx] = t[Bx] + 1;
+    l[Cx] = t[Cx] + 1;
+    const int li = l[0] + BV::L0 * (l[1] + BV::L1 * l[2]);
+    constexpr int sa = BV::sa, sb = BV::sb, sc = BV::sc;
+    double mf[9];
+    if (face_ok) {
+      const bool q5 = (S.quirks & CFX_Q_METRIC_AVG) && DIR != 1;
+#pragma unroll
+      for (int rc = 0; rc < 9; ++rc) {
+        const double a = B.metric[rc * np + pl_i];
+        mf[rc] = (q5 && rc != 0) ? 0.5 * (a + a) : 0.5 * (a + B.metric[rc * np + pr_i]);
+      }
+    } else {
+#pragma unroll
+      for (int rc = 0; rc < 9; ++rc) mf[rc] = 0.0;
+    }
+    // computational-space derivatives of staged variable k at the face
+    auto grad = [&](int k, double gc[3]) {
+      const double *q = stg + k * BV::V + li;
+      gc[A] = q[sa] - q[0];
+      gc[Bx] = 0.25 * (q[sb] - q[-sb] + q[sa + sb] - q[sa - sb]);
+      gc[Cx] = 0.25 * (q[sc] - q[-sc] + q[sa + sc] - q[sa - sc]);
+    };
+    auto chain = [&](const double gc[3], double out[3]) {
+      out[0] = gc[0] * mf[0] + gc[1] * mf[3] + gc[2] * mf[6];
+      out[1] = gc[0] * mf[1] + gc[1] * mf[4] + gc[2] * mf[7];
+      out[2] = gc[0] * mf[2] + gc[1] * mf[5] + gc[2] * mf[8];
+    };
+    double gcv[3], du[3], dv[3], dw[3], dT[3], avg[4];
+    {
+      const double *src[P::KV] = {B.bv + np, B.bv + 2 * np, B.bv + 3 * np, B.bv + 5 * np};
+      stage(BV{}, src, 4, bo);
+      grad(0, gcv);
+      chain(gcv, du);
+      grad(1, gcv);
+      chain(gcv, dv);
+      grad(2, gcv);
+      chain(gcv, dw);
+      grad(3, gcv);
+      chain(gcv, dT);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) avg[q] = 0.5 * (stg[q * BV::V + li] + stg[q * BV::V + li + sa]);
+    }
+    {
+      const double vis = face_ok ? 0.5 * (B.props[2 * np + pl_i] + B.props[2 * np + pr_i]) : 0.0;
+      const double cond = face_ok ? 0.5 * (B.props[3 * np + pl_i] + B.props[3 * np + pr_i]) : 0.0;
+      const double third = 1.0 / 3.0;
+      const double txx = vis * (4 * du[0] - 2 * dv[1] - 2 * dw[2]) * third;
+      const double tyy = vis * (4 * dv[1] - 2 * du[0] - 2 * dw[2]) * third;
+      const double tzz = vis * (4 * dw[2] - 2 * du[0] - 2 * dv[1]) * third;
+      const double txy = vis * (du[1] + dv[0]);
+      const double txz = vis * (du[2] + dw[0]);
+      const double tyz = vis * (dv[2] + dw[1]);
+      const double G1 = k1 * txx + k2 * txy + k3 * txz;
+      const double G2 = k1 * txy + k2 * tyy + k3 * tyz;
+      const double G3 = k1 * txz + k2 * tyz + k3 * tzz;
+      const double G4 = avg[0] * G1 + avg[1] * G2 + avg[2] * G3 + cond * (k1 * dT[0] + k2 * dT[1] + k3 * dT[2]);
+      fbuf[1 * NT + tid] -= G1;
+      fbuf[2 * NT + tid] -= G2;
+      fbuf[3 * NT + tid] -= G3;
+      fbuf[4 * NT + tid] -= G4;
+    }
+    if constexpr (NS > 0) {
+      const double tm = avg[3];
+      const long long pD = (DIR == 2 && (S.quirks & CFX_Q_HV_RHOD_J)) ? pl_i + B.sy : pr_i;
+      double dp[NS], yb[NS];
+      double ucx = 0, ucy = 0, ucz = 0, e4 = 0;
+#pragma unroll
+      for (int s0 = 0; s0 < NS; s0 += P::KV) {
+        const double *src[P::KV];
+        const int K = NS - s0 < P::KV ? NS - s0 : P::KV;
+#pragma unroll
+        for (int k = 0; k < P::KV; ++k) src[k] = B.yk + (s0 + k < NS ? s0 + k : NS - 1) * np;
+        stage(BV{}, src, K, bo);
+#pragma unroll
+        for (int k = 0; k < P::KV; ++k) {
+          const int s = s0 + k;
+          if (s >= NS) break;
+          grad(k, gcv);
+          double dy[3];
+          chain(gcv, dy);
+          yb[s] = 0.5 * (stg[k * BV::V + li] + stg[k * BV::V + li + sa]);
+          const double dif = face_ok ? 0.5 * (B.props[(4 + s) * np + pl_i] + B.props[(4 + s) * np + pD]) : 0.0;
+          double h, cp;
+          species_h_cp<double>(sp, s, tm, h, cp);
+          const double proj = k1 * dy[0] + k2 * dy[1] + k3 * dy[2];
+          e4 += dif * h * proj;
+          ucx += dif * dy[0];
+          ucy += dif * dy[1];
+          ucz += dif * dy[2];
+          dp[s] = dif * proj;
+        }
+      }
+      const double uc = k1 * ucx + k2 * ucy + k3 * ucz;
+      fbuf[4 * NT + tid] -= e4;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) fbuf[(5 + s) * NT + tid] -= dp[s] - yb[s] * uc;
+    }
+  }
+
+  // ---- cell difference of the face slots, dq written once per direction
+  __syncthreads();
+  if (cell_ok) {
+    int c[3] = {g[0], g[1], g[2]};
+    c[A] += 1;
+    const long long q = ((long long)c[2] * B.my + c[1]) * B.mx + c[0];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const double d = -(fbuf[v * NT + tid + ta_stride] - fbuf[v * NT + tid]);
+      if (accumulate) B.dq[v * B.ni + q] += d;
+      else B.dq[v * B.ni + q] = d;
+    }
+  }
+}
+
+}  // namespace cfx
