@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--rhs-only", type=int, default=0,
                     help="profiling mode: run this many RHS launches on the workload and exit")
     ap.add_argument("--rhs-variant", type=int, default=-1)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="precision of the headline line; the other variant is measured beside it")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()

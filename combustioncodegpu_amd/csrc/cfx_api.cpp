@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/cfx.h"
@@ -54,6 +55,7 @@ struct cfx_ctx {
   int n_partial = 0;
   long long n_grid_total = 0;
   int rhs_tile = 0;
+  bool mixed = false;
   bool profiling = false;
   std::vector<TimedLaunch> timed;
   std::vector<hipEvent_t> event_pool;
@@ -105,7 +107,7 @@ namespace {
 constexpr double kRuH = 8314.462618;
 
 // Host NASA-7 evaluation, identical to species_h_cp (device_physics.h).
-void host_h_cp(const SpeciesDev &s, int l, double t, double &h, double &cp) {
+void host_h_cp(const SpeciesT<double> &s, int l, double t, double &h, double &cp) {
   const int r = t >= s.t_mid[l] ? 1 : 0;
   const double *c = s.cpc[r][l], *e = s.hc[r][l];
   const double cpp = c[0] + t * (c[1] + t * (c[2] + t * (c[3] + t * c[4])));
@@ -119,8 +121,24 @@ void host_h_cp(const SpeciesDev &s, int l, double t, double &h, double &cp) {
   }
 }
 
-int build_species(const cfx_species *sp, SpeciesDev &s, std::string &err) {
-  std::memset(&s, 0, sizeof(s));
+template <typename A, typename B>
+void round_table(const A &a, B &b) {
+  b.ns = a.ns;
+  using T = std::remove_reference_t<decltype(b.mw[0])>;
+  auto cp1 = [](const auto &src, auto &dst) {
+    const double *x = reinterpret_cast<const double *>(&src);
+    T *y = reinterpret_cast<T *>(&dst);
+    for (size_t i = 0; i < sizeof(src) / sizeof(double); ++i) y[i] = (T)x[i];
+  };
+  cp1(a.mw, b.mw); cp1(a.inv_mw, b.inv_mw); cp1(a.r_sp, b.r_sp); cp1(a.t_low, b.t_low); cp1(a.t_mid, b.t_mid);
+  cp1(a.cpc, b.cpc); cp1(a.hc, b.hc); cp1(a.h_tlow, b.h_tlow); cp1(a.cp_tlow, b.cp_tlow); cp1(a.lj_inv, b.lj_inv);
+  cp1(a.log_lj_inv, b.log_lj_inv); cp1(a.vis_coeff, b.vis_coeff); cp1(a.cond_add, b.cond_add); cp1(a.wj4, b.wj4);
+  cp1(a.sqw8, b.sqw8);
+}
+
+int build_species(const cfx_species *sp, SpeciesDev &tables, std::string &err) {
+  std::memset(&tables, 0, sizeof(tables));
+  SpeciesT<double> &s = tables.d;
   s.ns = sp ? sp->n_spec : 0;
   if (s.ns < 0 || s.ns > kMaxSp) {
     err = "n_spec out of range";
@@ -161,11 +179,13 @@ int build_species(const cfx_species *sp, SpeciesDev &s, std::string &err) {
       s.wj4[m][n] = std::pow(s.mw[n] / s.mw[m], 0.25);
       s.sqw8[m][n] = 1.0 / std::sqrt(8 * (1 + s.mw[m] / s.mw[n]));
     }
+  round_table(tables.d, tables.f);
   return 0;
 }
 
 // Inflow::Inflow state (BoundCond.cu:13-105): (mach|velocity, p, T|rho).
-int build_inflow(const cfx_inflow &in, const SpeciesDev &s, const SchemeDev &S, InflowDev &out, std::string &err) {
+int build_inflow(const cfx_inflow &in, const SpeciesT<double> &s, const SchemeDev &S, InflowDev &out,
+                 std::string &err) {
   std::memset(&out, 0, sizeof(out));
   const int ns = s.ns;
   double mw = kRuH / S.r_air;
@@ -337,11 +357,13 @@ int cfx_create(const cfx_params *params, const cfx_species *species, const cfx_b
   };
   int rc = build_species(species, ctx->sp_host, ctx->err);
   if (rc) return bail(rc);
-  ctx->ns = ctx->sp_host.ns;
+  ctx->ns = ctx->sp_host.d.ns;
   ctx->nv = 5 + ctx->ns;
   if (!ns_supported(ctx->ns)) return bail(CFX_E_UNSUPPORTED);
   if (params->inviscid_scheme != 3 && params->inviscid_scheme != 0) return bail(CFX_E_UNSUPPORTED);
-  if (params->precision != CFX_FP64 || params->transport != CFX_TRANSPORT_WILKE) return bail(CFX_E_UNSUPPORTED);
+  if ((params->precision != CFX_FP64 && params->precision != CFX_MIXED) || params->transport != CFX_TRANSPORT_WILKE)
+    return bail(CFX_E_UNSUPPORTED);
+  ctx->mixed = params->precision == CFX_MIXED;
   rc = build_mesh(blocks, n_block, ctx->hb, ctx->dim, ctx->err);
   if (rc) return bail(rc);
   SchemeDev &S = ctx->S;
@@ -361,7 +383,7 @@ int cfx_create(const cfx_params *params, const cfx_species *species, const cfx_b
   ctx->rk = S.rk;
   for (int i = 0; i < params->n_inflow; ++i) {
     InflowDev d;
-    rc = build_inflow(params->inflow[i], ctx->sp_host, S, d, ctx->err);
+    rc = build_inflow(params->inflow[i], ctx->sp_host.d, S, d, ctx->err);
     if (rc) return bail(rc);
     ctx->inflow_host.push_back(d);
   }
@@ -435,6 +457,23 @@ int cfx_create(const cfx_params *params, const cfx_species *species, const cfx_b
     if (rc) return bail(ctx->fail(rc, "hipMalloc block arrays"));
     (void)hipMemcpy(V.metric, H.metric.data(), sizeof(double) * 9 * np, hipMemcpyHostToDevice);
     (void)hipMemcpy(V.jac, H.jac.data(), sizeof(double) * np, hipMemcpyHostToDevice);
+    V.mixed = ctx->mixed ? 1 : 0;
+    if (ctx->mixed) {
+      V.bv32 = ctx->dalloc<float>(6 * np, rc);
+      V.yk32 = ctx->dalloc<float>(ctx->ns * np, rc);
+      V.props32 = ctx->dalloc<float>((4 + ctx->ns) * np, rc);
+      V.metric32 = ctx->dalloc<float>(9 * np, rc);
+      V.jac32 = ctx->dalloc<float>(np, rc);
+      V.dq32 = ctx->dalloc<float>(ctx->nv * ni, rc);
+      if (rc) return bail(ctx->fail(rc, "hipMalloc FP32 arrays"));
+      std::vector<float> m32(9 * np), j32(np);
+      for (size_t i = 0; i < 9 * np; ++i) m32[i] = (float)H.metric[i];
+      for (size_t i = 0; i < np; ++i) j32[i] = (float)H.jac[i];
+      (void)hipMemcpy(V.metric32, m32.data(), sizeof(float) * 9 * np, hipMemcpyHostToDevice);
+      (void)hipMemcpy(V.jac32, j32.data(), sizeof(float) * np, hipMemcpyHostToDevice);
+      (void)hipMemset(V.props32, 0, sizeof(float) * (4 + ctx->ns) * np);
+      (void)hipMemset(V.dq32, 0, sizeof(float) * ctx->nv * ni);
+    }
     (void)hipMemset(V.props, 0, sizeof(double) * (4 + ctx->ns) * np);
     (void)hipMemset(V.dq, 0, sizeof(double) * ctx->nv * ni);
     ctx->views.push_back(V);
@@ -497,6 +536,14 @@ int cfx_download(cfx_ctx *ctx, int blk, int what, double *dst) {
   const size_t np = (size_t)V.np, ni = (size_t)V.ni;
   const double *src = nullptr;
   size_t n = 0;
+  if (V.mixed && (what == CFX_PROPS || what == CFX_DQ)) {  // FP32 arrays, returned as double
+    const float *s32 = what == CFX_PROPS ? V.props32 : V.dq32;
+    const size_t n32 = what == CFX_PROPS ? (4 + ctx->ns) * np : ctx->nv * ni;
+    std::vector<float> tmp(n32);
+    CFX_TRY((int)hipMemcpy(tmp.data(), s32, sizeof(float) * n32, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n32; ++i) dst[i] = tmp[i];
+    return 0;
+  }
   switch (what) {
     case CFX_CV: src = V.cv, n = ctx->nv * np; break;
     case CFX_BV: src = V.bv, n = 6 * np; break;

@@ -4,6 +4,8 @@
 // block with i fastest (see include/cfx.h for the index formula).
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
 #include "../../include/cfx.h"
@@ -17,21 +19,29 @@ constexpr int kMaxSp = CFX_MAX_SPEC;
 
 // Species table resident in device memory.  Everything the kernels need is
 // precomputed here so that the per-point loops are polynomial evaluations:
-//   cp_l(T)/1 = sum_k cpc[r][l][k] T^k,   h_l(T) = T*sum_k hc[r][l][k] T^k + hc[r][l][5]
+//   cp_l(T) = sum_k cpc[r][l][k] T^k,   h_l(T) = T*sum_k hc[r][l][k] T^k + hc[r][l][5]
 // with r = 0 for T < t_mid (low set) and r = 1 otherwise, already scaled by
 // R_u/W_l; below t_low the enthalpy is extended linearly with the frozen cp
-// (h_tlow, cp_tlow), as Thermo.cu:29-36 does.
-struct SpeciesDev {
+// (h_tlow, cp_tlow), as Thermo.cu:29-36 does.  Held in the arithmetic type
+// of the kernel that reads it, so that every constant is a scalar operand.
+template <typename T>
+struct SpeciesT {
   int ns;
-  double mw[kMaxSp], inv_mw[kMaxSp], r_sp[kMaxSp];  // W, 1/W, R_u/W
-  double t_low[kMaxSp], t_mid[kMaxSp];
-  double cpc[2][kMaxSp][5];
-  double hc[2][kMaxSp][6];
-  double h_tlow[kMaxSp], cp_tlow[kMaxSp];
-  double lj_inv[kMaxSp], log_lj_inv[kMaxSp], vis_coeff[kMaxSp];
-  double cond_add[kMaxSp];              // 1.25 R_u / W_l (Transport.cu:222)
-  double wj4[kMaxSp][kMaxSp];           // (W_n/W_m)^(1/4)           (ChemData.cpp:257)
-  double sqw8[kMaxSp][kMaxSp];          // 1/sqrt(8(1+W_m/W_n))      (ChemData.cpp:258)
+  T mw[kMaxSp], inv_mw[kMaxSp], r_sp[kMaxSp];  // W, 1/W, R_u/W
+  T t_low[kMaxSp], t_mid[kMaxSp];
+  T cpc[2][kMaxSp][5];
+  T hc[2][kMaxSp][6];
+  T h_tlow[kMaxSp], cp_tlow[kMaxSp];
+  T lj_inv[kMaxSp], log_lj_inv[kMaxSp], vis_coeff[kMaxSp];
+  T cond_add[kMaxSp];        // 1.25 R_u / W_l (Transport.cu:222)
+  T wj4[kMaxSp][kMaxSp];     // (W_n/W_m)^(1/4)           (ChemData.cpp:257)
+  T sqw8[kMaxSp][kMaxSp];    // 1/sqrt(8(1+W_m/W_n))      (ChemData.cpp:258)
+};
+
+// The FP64 table and its FP32 rounding, side by side in one allocation.
+struct SpeciesDev {
+  SpeciesT<double> d;
+  SpeciesT<float> f;
 };
 
 struct SchemeDev {
@@ -50,6 +60,32 @@ struct BlockView {
   double *cv, *bv, *yk, *props;  // props: gamma, c, mu, kappa, rhoD[ns]
   double *metric, *jac;          // metric: 9 x np (xi_x..zeta_z), jac: np
   double *dq, *qn, *dt_local, *bv_last;
+  // mixed precision: FP32 copies of the RHS inputs (refreshed by the
+  // properties kernel over the padded range) and the FP32 dq; null in FP64 mode
+  int mixed;
+  float *bv32, *yk32, *props32, *metric32, *jac32, *dq32;
+};
+
+// Field selection by the arithmetic type of the flux work.
+template <typename R>
+struct Fields;
+template <>
+struct Fields<double> {
+  __host__ __device__ static const double *bv(const BlockView &B) { return B.bv; }
+  __host__ __device__ static const double *yk(const BlockView &B) { return B.yk; }
+  __host__ __device__ static const double *props(const BlockView &B) { return B.props; }
+  __host__ __device__ static const double *metric(const BlockView &B) { return B.metric; }
+  __host__ __device__ static const double *jac(const BlockView &B) { return B.jac; }
+  __host__ __device__ static double *dq(const BlockView &B) { return B.dq; }
+};
+template <>
+struct Fields<float> {
+  __host__ __device__ static const float *bv(const BlockView &B) { return B.bv32; }
+  __host__ __device__ static const float *yk(const BlockView &B) { return B.yk32; }
+  __host__ __device__ static const float *props(const BlockView &B) { return B.props32; }
+  __host__ __device__ static const float *metric(const BlockView &B) { return B.metric32; }
+  __host__ __device__ static const float *jac(const BlockView &B) { return B.jac32; }
+  __host__ __device__ static float *dq(const BlockView &B) { return B.dq32; }
 };
 
 // Inflow state resident on device: rho,u,v,w,p,T, E, then Y[ns].

@@ -16,9 +16,19 @@ namespace cfx {
 // The species table is read through the constant address space: every index
 // into it is uniform across the wavefront, so the loads become scalar loads
 // into SGPRs instead of vector loads into VGPRs.
-using SpeciesPtr = const __attribute__((address_space(4))) SpeciesDev *;
+template <typename R>
+using SpeciesPtrT = const __attribute__((address_space(4))) SpeciesT<R> *;
 
-__device__ __forceinline__ SpeciesPtr const_table(const SpeciesDev *p) { return (SpeciesPtr)p; }
+template <typename R>
+__device__ __forceinline__ SpeciesPtrT<R> const_table(const SpeciesDev *p);
+template <>
+__device__ __forceinline__ SpeciesPtrT<double> const_table<double>(const SpeciesDev *p) {
+  return (SpeciesPtrT<double>)&p->d;
+}
+template <>
+__device__ __forceinline__ SpeciesPtrT<float> const_table<float>(const SpeciesDev *p) {
+  return (SpeciesPtrT<float>)&p->f;
+}
 
 template <typename R>
 struct Tiny;
@@ -36,7 +46,7 @@ struct Tiny<float> {
 // Horner form with the low/high set chosen at t_mid and the linear enthalpy
 // extension with frozen cp below t_low.
 template <typename R>
-__device__ __forceinline__ void species_h_cp(SpeciesPtr sp, int l, R t, R &h, R &cp) {
+__device__ __forceinline__ void species_h_cp(SpeciesPtrT<R> sp, int l, R t, R &h, R &cp) {
   const bool hi = t >= R(sp->t_mid[l]);
   const R c0 = hi ? R(sp->cpc[1][l][0]) : R(sp->cpc[0][l][0]), c1 = hi ? R(sp->cpc[1][l][1]) : R(sp->cpc[0][l][1]),
           c2 = hi ? R(sp->cpc[1][l][2]) : R(sp->cpc[0][l][2]), c3 = hi ? R(sp->cpc[1][l][3]) : R(sp->cpc[0][l][3]),
@@ -54,7 +64,7 @@ __device__ __forceinline__ void species_h_cp(SpeciesPtr sp, int l, R t, R &h, R 
 
 // cp_l(T) only, frozen below t_low (Thermo.cu:68-82)
 template <typename R>
-__device__ __forceinline__ R species_cp(SpeciesPtr sp, int l, R t) {
+__device__ __forceinline__ R species_cp(SpeciesPtrT<R> sp, int l, R t) {
   const bool hi = t >= R(sp->t_mid[l]);
   const R c0 = hi ? R(sp->cpc[1][l][0]) : R(sp->cpc[0][l][0]), c1 = hi ? R(sp->cpc[1][l][1]) : R(sp->cpc[0][l][1]),
           c2 = hi ? R(sp->cpc[1][l][2]) : R(sp->cpc[0][l][2]), c3 = hi ? R(sp->cpc[1][l][3]) : R(sp->cpc[0][l][3]),
@@ -97,7 +107,7 @@ __device__ __forceinline__ void muscl(const SchemeDev &S, R q0, R q1, R q2, R q3
 // J_R grad_R) of the face direction; the face temperature, energy and gamma
 // follow Reconstruction.cu:79-122.
 template <int NS, typename R>
-__device__ __forceinline__ void ausm_from_states(SpeciesPtr sp, const SchemeDev &S, const R *pl,
+__device__ __forceinline__ void ausm_from_states(SpeciesPtrT<R> sp, const SchemeDev &S, const R *pl,
                                                  const R *pr, R k1, R k2, R k3, R *F) {
   constexpr int NV = 5 + NS;
   R el = R(0.5) * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]);
@@ -183,7 +193,7 @@ __device__ __forceinline__ void ausm_from_states(SpeciesPtr sp, const SchemeDev 
 // (InviscidScheme.cu:220-304) with the face states from MUSCL
 // (Reconstruction.cu:79-122).  F has 5+NS entries.
 template <int NS, typename R>
-__device__ __forceinline__ void inviscid_flux(const BlockView &B, SpeciesPtr sp,
+__device__ __forceinline__ void inviscid_flux(const BlockView &B, SpeciesPtrT<R> sp,
                                               const SchemeDev &S, int d, long long p, long long s, R *F) {
   constexpr int NV = 5 + NS;
   const long long np = B.np;
@@ -208,7 +218,7 @@ __device__ __forceinline__ void inviscid_flux(const BlockView &B, SpeciesPtr sp,
 // the face direction d.  Quirk bits reproduce the reference's face-metric
 // averaging (fv, hv) and the (i,j+1,k) rho*D neighbour of hv.
 template <int NS, typename R>
-__device__ __forceinline__ void viscous_flux(const BlockView &B, SpeciesPtr sp,
+__device__ __forceinline__ void viscous_flux(const BlockView &B, SpeciesPtrT<R> sp,
                                              const SchemeDev &S, int d, long long p, const long long st[3],
                                              R *G) {
   const long long np = B.np;

@@ -38,7 +38,7 @@ __device__ __forceinline__ int xcd_tile(int b, int nt) {
 // ---------------------------------------------------------- total energy ---
 // compute_total_energy (Thermo.cu:85-105) from bv/yk/cv at padded point p.
 template <int NS>
-__device__ __forceinline__ void total_energy(const BlockView &B, SpeciesPtr sp,
+__device__ __forceinline__ void total_energy(const BlockView &B, SpeciesPtrT<double> sp,
                                              const SchemeDev &S, long long p) {
   const long long np = B.np;
   const double u = B.bv[np + p], v = B.bv[2 * np + p], w = B.bv[3 * np + p];
@@ -64,69 +64,83 @@ __device__ __forceinline__ void total_energy(const BlockView &B, SpeciesPtr sp,
 // The Wilke sums use s_l = sqrt(mu_l) so that sqrt(mu_m/mu_n) = s_m/s_n
 // (NS square roots instead of NS^2) and T*^-0.145 = exp(-0.145 log T*) with
 // one shared log T.
-template <int NS>
+template <int NS, typename R>
 __global__ void __launch_bounds__(256) props_kernel(BlockView B, const SpeciesDev *__restrict__ spg, SchemeDev S) {
-  const SpeciesPtr sp = const_table(spg);
+  const SpeciesPtrT<R> sp = const_table<R>(spg);
   const long long np = B.np;
+  constexpr bool f32 = sizeof(R) == 4;
   for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (long long)gridDim.x * blockDim.x) {
-    const double t = B.bv[5 * np + p];
-    double *P = B.props;
+    const R t = R(B.bv[5 * np + p]);
+    R out[4 + (NS > 0 ? NS : 0)];
     if constexpr (NS == 0) {
-      const double c_temp = S.gamma_air * S.r_air;
-      P[p] = S.gamma_air;
-      P[np + p] = sqrt(c_temp * t);
-      const double mu = 1.716e-5 * pow(t / 273, 1.5) * (273 + 111) / (t + 111);
-      P[2 * np + p] = mu;
-      P[3 * np + p] = mu * c_temp / (S.gamma_air - 1) * S.inv_pr;
+      const R c_temp = R(S.gamma_air * S.r_air);
+      out[0] = R(S.gamma_air);
+      out[1] = sqrt(c_temp * t);
+      const R mu = R(1.716e-5) * pow(t / R(273), R(1.5)) * R(273 + 111) / (t + R(111));
+      out[2] = mu;
+      out[3] = mu * c_temp / R(S.gamma_air - 1) * R(S.inv_pr);
     } else {
-      double Y[NS], cp[NS];
-      double winv = 0, cpt = 0;
+      R Y[NS], cp[NS];
+      R winv = 0, cpt = 0;
 #pragma unroll
       for (int l = 0; l < NS; ++l) {
-        Y[l] = B.yk[l * np + p];
-        cp[l] = species_cp<double>(sp, l, t);
-        winv += Y[l] * sp->inv_mw[l];
+        Y[l] = R(B.yk[l * np + p]);
+        cp[l] = species_cp<R>(sp, l, t);
+        winv += Y[l] * R(sp->inv_mw[l]);
         cpt += Y[l] * cp[l];
       }
-      const double gam = cpt / (cpt - kRu * winv);
-      P[p] = gam;
-      P[np + p] = sqrt(gam * kRu * t * winv);
+      const R Ru = R(kRu);
+      const R gam = cpt / (cpt - Ru * winv);
+      out[0] = gam;
+      out[1] = sqrt(gam * Ru * t * winv);
       // Chapman-Enskog species viscosity (Transport.cu:215-220)
-      const double lt = log(t), st = sqrt(t);
-      double mu_s[NS], X[NS], s[NS], rs[NS];
-      const double imw = 1.0 / winv;
+      const R lt = log(t), st = sqrt(t);
+      R mu_s[NS], X[NS], s[NS], rs[NS];
+      const R imw = R(1) / winv;
 #pragma unroll
       for (int l = 0; l < NS; ++l) {
-        X[l] = Y[l] * imw * sp->inv_mw[l];
-        const double tdl = t * sp->lj_inv[l];
-        const double a = tdl + 0.5;
-        const double omega = 1.147 * exp(-0.145 * (lt + sp->log_lj_inv[l])) + 1.0 / (a * a);
-        mu_s[l] = sp->vis_coeff[l] * st / omega;
+        X[l] = Y[l] * imw * R(sp->inv_mw[l]);
+        const R tdl = t * R(sp->lj_inv[l]);
+        const R a = tdl + R(0.5);
+        const R omega = R(1.147) * exp(R(-0.145) * (lt + R(sp->log_lj_inv[l]))) + R(1) / (a * a);
+        mu_s[l] = R(sp->vis_coeff[l]) * st / omega;
         s[l] = sqrt(mu_s[l]);
-        rs[l] = 1.0 / s[l];
+        rs[l] = R(1) / s[l];
       }
       // Wilke mixing and Mason-Saxena conductivity (Transport.cu:221-246)
-      double mu = 0, kappa = 0;
+      R mu = 0, kappa = 0;
 #pragma unroll
       for (int m = 0; m < NS; ++m) {
-        double vt = X[m];
+        R vt = X[m];
 #pragma unroll
         for (int n = 0; n < NS; ++n) {
           if (n == m) continue;
-          const double q = 1.0 + s[m] * rs[n] * sp->wj4[m][n];
-          vt += q * q * sp->sqw8[m][n] * X[n];
+          const R q = R(1) + s[m] * rs[n] * R(sp->wj4[m][n]);
+          vt += q * q * R(sp->sqw8[m][n]) * X[n];
         }
-        const double lam = mu_s[m] * (cp[m] + sp->cond_add[m]);
+        const R lam = mu_s[m] * (cp[m] + R(sp->cond_add[m]));
         mu += mu_s[m] * X[m] / vt;
-        kappa += lam * X[m] / (1.065 * vt - 0.065 * X[m]);
+        kappa += lam * X[m] / (R(1.065) * vt - R(0.065) * X[m]);
       }
-      P[2 * np + p] = mu;
-      P[3 * np + p] = kappa;
+      out[2] = mu;
+      out[3] = kappa;
       // constant-Schmidt diffusivity (Transport.cu:249-255, X[l] in the test)
-      const double mus = mu * S.inv_sc;
+      const R mus = mu * R(S.inv_sc);
 #pragma unroll
       for (int l = 0; l < NS; ++l)
-        P[(4 + l) * np + p] = fabs(X[l] - 1.0) < 1e-3 ? mus : (1.0 - Y[l]) * mu / ((1.0 - X[l]) * S.sc);
+        out[4 + l] = fabs(X[l] - R(1)) < R(1e-3) ? mus : (R(1) - Y[l]) * mu / ((R(1) - X[l]) * R(S.sc));
+    }
+    if constexpr (f32) {
+#pragma unroll
+      for (int v = 0; v < 4 + NS; ++v) B.props32[v * np + p] = out[v];
+      // FP32 copies of the primitives for the FP32 RHS
+#pragma unroll
+      for (int v = 0; v < 6; ++v) B.bv32[v * np + p] = float(B.bv[v * np + p]);
+#pragma unroll
+      for (int l = 0; l < NS; ++l) B.yk32[l * np + p] = float(B.yk[l * np + p]);
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4 + NS; ++v) B.props[v * np + p] = out[v];
     }
   }
 }
@@ -140,7 +154,7 @@ __global__ void __launch_bounds__(256) props_kernel(BlockView B, const SpeciesDe
 template <int NS, int TX, int TY, int TZ, int WPE>
 __global__ void __launch_bounds__(TX *TY *TZ) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) rhs_kernel(BlockView B, const SpeciesDev *__restrict__ spg, SchemeDev S,
                                                          int ntx, int nty, int ntz) {
-  const SpeciesPtr sp = const_table(spg);
+  const SpeciesPtrT<double> sp = const_table<double>(spg);
   constexpr int NV = 5 + NS;
   constexpr int NT = TX * TY * TZ;
   extern __shared__ double sf[];
@@ -218,15 +232,16 @@ __global__ void __launch_bounds__(256) dt_kernel(BlockView B, SchemeDev S, int n
     const double U = u * mm[0] + v * mm[1] + w * mm[2];
     const double V = u * mm[3] + v * mm[4] + w * mm[5];
     const double W = u * mm[6] + v * mm[7] + w * mm[8];
-    const double a = B.props[np + p];
+    const double a = B.mixed ? (double)B.props32[np + p] : B.props[np + p];
     double sr = fabs(U) + fabs(V) + a * (gx + gy);
     double sv = gx * gx + gy * gy;
     if (S.dim == 3) {
       sr += fabs(W) + a * gz;
       sv += gz * gz;
     }
-    const double gam = ns > 0 ? B.props[p] : S.gamma_air;
-    sv *= fmax(gam, 4.0 / 3.0) * (B.props[2 * np + p] / B.bv[p] * S.inv_pr);
+    const double gam = ns > 0 ? (B.mixed ? (double)B.props32[p] : B.props[p]) : S.gamma_air;
+    const double mu = B.mixed ? (double)B.props32[2 * np + p] : B.props[2 * np + p];
+    sv *= fmax(gam, 4.0 / 3.0) * (mu / B.bv[p] * S.inv_pr);
     const double dt = S.cfl / (sr + sv);
     if (store_local) B.dt_local[q] = dt;
     dmin = fmin(dmin, dt);
@@ -252,7 +267,7 @@ template <int NS>
 __global__ void __launch_bounds__(256) update_kernel(BlockView B, const SpeciesDev *__restrict__ spg, SchemeDev S,
                                                      int stage, const double *__restrict__ dt_dev, int *flags,
                                                      unsigned long long *newton_fail) {
-  const SpeciesPtr sp = const_table(spg);
+  const SpeciesPtrT<double> sp = const_table<double>(spg);
   constexpr int NV = 5 + NS;
   const long long np = B.np, ni = B.ni;
   const double A = stage == 1 ? 0.75 : (stage == 2 ? 1.0 / 3.0 : 0.0);
@@ -269,7 +284,7 @@ __global__ void __launch_bounds__(256) update_kernel(BlockView B, const SpeciesD
 #pragma unroll
     for (int l = 0; l < NV; ++l) {
       const double cur = B.cv[l * np + p];
-      const double dq = B.dq[l * ni + q];
+      const double dq = B.mixed ? (double)B.dq32[l * ni + q] : B.dq[l * ni + q];
       if (S.rk == 3) {
         double qn;
         if (stage == 0) {
@@ -340,7 +355,7 @@ __global__ void __launch_bounds__(256) update_kernel(BlockView B, const SpeciesD
 // compute_cv_from_bv (Field.cu:242-268) over the padded range
 template <int NS>
 __global__ void cv_from_bv_kernel(BlockView B, const SpeciesDev *__restrict__ spg, SchemeDev S) {
-  const SpeciesPtr sp = const_table(spg);
+  const SpeciesPtrT<double> sp = const_table<double>(spg);
   const long long np = B.np;
   for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (long long)gridDim.x * blockDim.x) {
     const double rho = B.bv[p];
@@ -472,7 +487,7 @@ __global__ void bc_inflow_kernel(BlockView B, FaceRange f, int ns, const InflowD
 }
 
 template <int NS>
-__device__ __forceinline__ double mix_mw(const BlockView &B, SpeciesPtr sp, const SchemeDev &S,
+__device__ __forceinline__ double mix_mw(const BlockView &B, SpeciesPtrT<double> sp, const SchemeDev &S,
                                          long long p) {
   if constexpr (NS == 0) {
     return kRu / S.r_air;
@@ -489,7 +504,7 @@ __device__ __forceinline__ double mix_mw(const BlockView &B, SpeciesPtr sp, cons
 template <int NS>
 __global__ void bc_wall_kernel(BlockView B, FaceRange f, const SpeciesDev *__restrict__ spg, SchemeDev S,
                                WallDev wl) {
-  const SpeciesPtr sp = const_table(spg);
+  const SpeciesPtrT<double> sp = const_table<double>(spg);
   int i, j, k;
   if (!face_point(f, i, j, k)) return;
   int dir[3] = {0, 0, 0};
@@ -551,7 +566,7 @@ __global__ void bc_wall_kernel(BlockView B, FaceRange f, const SpeciesDev *__res
 // the normal velocity negated and rho, p, T, Y copied.
 template <int NS>
 __global__ void bc_symmetry_kernel(BlockView B, FaceRange f, const SpeciesDev *__restrict__ spg, SchemeDev S) {
-  const SpeciesPtr sp = const_table(spg);
+  const SpeciesPtrT<double> sp = const_table<double>(spg);
   int i, j, k;
   if (!face_point(f, i, j, k)) return;
   int dir[3] = {0, 0, 0};
@@ -688,7 +703,11 @@ bool ns_supported(int ns) { return ns == 0 || ns == 1 || ns == 2 || ns == 3 || n
 
 int launch_props(const BlockView &B, const SpeciesDev *sp, const SchemeDev &S, int ns, hipStream_t st) {
   const int g = grid_for(B.np, 256);
-  CFX_SWITCH_NS(ns, props_kernel<NS_><<<g, 256, 0, st>>>(B, sp, S));
+  if (B.mixed) {
+    CFX_SWITCH_NS(ns, props_kernel<NS_, float><<<g, 256, 0, st>>>(B, sp, S));
+  } else {
+    CFX_SWITCH_NS(ns, props_kernel<NS_, double><<<g, 256, 0, st>>>(B, sp, S));
+  }
   return (int)hipGetLastError();
 }
 
@@ -740,7 +759,7 @@ static int launch_rhs_t(const BlockView &B, const SpeciesDev *sp, const SchemeDe
   return (int)hipGetLastError();
 }
 
-template <int NS, int DIR, int TA, int TB, int TC, int WPE>
+template <int NS, int DIR, int TA, int TB, int TC, int WPE, typename R>
 static int launch_pencil_t(const BlockView &B, const SpeciesDev *sp, const SchemeDev &S, int accumulate,
                            hipStream_t st) {
   constexpr int A = Axes<DIR>::a, Bx = Axes<DIR>::b, Cx = Axes<DIR>::c;
@@ -748,18 +767,28 @@ static int launch_pencil_t(const BlockView &B, const SpeciesDev *sp, const Schem
   const int nta = (m[A] + TA - 2) / (TA - 1);  // TA faces cover TA-1 cells
   const int ntb = (m[Bx] + TB - 1) / TB;
   const int ntc = (m[Cx] + TC - 1) / TC;
-  rhs_pencil_kernel<NS, DIR, TA, TB, TC, WPE><<<nta * ntb * ntc, TA * TB * TC, 0, st>>>(B, sp, S, accumulate, nta, ntb,
-                                                                                     ntc);
+  rhs_pencil_kernel<NS, DIR, TA, TB, TC, WPE, R>
+      <<<nta * ntb * ntc, TA * TB * TC, 0, st>>>(B, sp, S, accumulate, nta, ntb, ntc);
   return (int)hipGetLastError();
 }
 
+template <int NS, int TA0, int TB0, int TC0, int TA1, int TB1, int TC1, int TA2, int TB2, int TC2, int WPE,
+          typename R>
+static int launch_pencils_r(const BlockView &B, const SpeciesDev *sp, const SchemeDev &S, hipStream_t st) {
+  int rc = launch_pencil_t<NS, 0, TA0, TB0, TC0, WPE, R>(B, sp, S, 0, st);
+  if (rc) return rc;
+  rc = launch_pencil_t<NS, 1, TA1, TB1, TC1, WPE, R>(B, sp, S, 1, st);
+  if (rc || S.dim == 2) return rc;
+  return launch_pencil_t<NS, 2, TA2, TB2, TC2, WPE, R>(B, sp, S, 1, st);
+}
+
+// FP64 with the variant's target occupancy; FP32 (mixed precision) needs
+// half the registers and targets one more wave per SIMD
 template <int NS, int TA0, int TB0, int TC0, int TA1, int TB1, int TC1, int TA2, int TB2, int TC2, int WPE>
 static int launch_pencils(const BlockView &B, const SpeciesDev *sp, const SchemeDev &S, hipStream_t st) {
-  int rc = launch_pencil_t<NS, 0, TA0, TB0, TC0, WPE>(B, sp, S, 0, st);
-  if (rc) return rc;
-  rc = launch_pencil_t<NS, 1, TA1, TB1, TC1, WPE>(B, sp, S, 1, st);
-  if (rc || S.dim == 2) return rc;
-  return launch_pencil_t<NS, 2, TA2, TB2, TC2, WPE>(B, sp, S, 1, st);
+  if (B.mixed)
+    return launch_pencils_r<NS, TA0, TB0, TC0, TA1, TB1, TC1, TA2, TB2, TC2, WPE + 1, float>(B, sp, S, st);
+  return launch_pencils_r<NS, TA0, TB0, TC0, TA1, TB1, TC1, TA2, TB2, TC2, WPE, double>(B, sp, S, st);
 }
 
 template <int NS>
@@ -782,6 +811,7 @@ static int launch_rhs_ns(const BlockView &B, const SpeciesDev *sp, const SchemeD
     }
   }
   tile -= kNPencil;
+  if (B.mixed) return CFX_E_UNSUPPORTED;  // the fused kernel is FP64 only
   if (S.dim == 3) {
     switch (tile) {
       case 0: return launch_rhs_t<NS, 16, 4, 4, 1>(B, sp, S, st);

@@ -66,18 +66,24 @@ struct PencilShape {
   static constexpr int STG = KI * BI::V > KV * BV::V ? KI * BI::V : KV * BV::V;
 };
 
-template <int NS, int DIR, int TA, int TB, int TC, int WPE>
+template <int NS, int DIR, int TA, int TB, int TC, int WPE, typename R>
 __global__ void __launch_bounds__(TA *TB *TC) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
     rhs_pencil_kernel(BlockView B, const SpeciesDev *__restrict__ spg, SchemeDev S, int accumulate, int nta, int ntb,
                       int ntc) {
-  const SpeciesPtr sp = const_table(spg);
+  const SpeciesPtrT<R> sp = const_table<R>(spg);
   using P = PencilShape<NS, DIR, TA, TB, TC>;
   using BI = typename P::BI;
   using BV = typename P::BV;
   constexpr int NV = P::NV, NT = P::NT, A = P::A, Bx = P::Bx, Cx = P::Cx;
   constexpr int ta_stride = P::tx[A];
-  __shared__ double stg[P::STG];    // a batch of staged variables
-  __shared__ double fbuf[NV * NT];  // each thread's face flux F_inv - F_visc
+  __shared__ R stg[P::STG];    // a batch of staged variables
+  __shared__ R fbuf[NV * NT];  // each thread's face flux F_inv - F_visc
+  using Fd = Fields<R>;
+  const R *__restrict__ BV_ = Fd::bv(B);
+  const R *__restrict__ YK_ = Fd::yk(B);
+  const R *__restrict__ PR_ = Fd::props(B);
+  const R *__restrict__ MT_ = Fd::metric(B);
+  const R *__restrict__ JC_ = Fd::jac(B);
 
   const long long np = B.np;
   const int m[3] = {B.mx, B.my, B.mz};
@@ -105,7 +111,7 @@ __global__ void __launch_bounds__(TA *TB *TC) __attribute__((amdgpu_waves_per_eu
 
   // stage K variables (source pointers) over box BX with global origin bo;
   // points beyond the padded range are left unwritten (no valid face reads them)
-  auto stage = [&](auto box, const double *const *src, int K, const int bo[3]) {
+  auto stage = [&](auto box, const R *const *src, int K, const int bo[3]) {
     using BX = decltype(box);
     __syncthreads();
     for (int e = tid; e < BX::V; e += NT) {
@@ -120,7 +126,7 @@ __global__ void __launch_bounds__(TA *TB *TC) __attribute__((amdgpu_waves_per_eu
     __syncthreads();
   };
 
-  double k1 = 0, k2 = 0, k3 = 0;
+  R k1 = 0, k2 = 0, k3 = 0;
 
   // ---- inviscid phase: F_inv into this thread's face slot
   {
@@ -134,34 +140,34 @@ __global__ void __launch_bounds__(TA *TB *TC) __attribute__((amdgpu_waves_per_eu
     l[Cx] = t[Cx];
     const int li = l[0] + BI::L0 * (l[1] + BI::L1 * l[2]);
     constexpr int sa = BI::sa;
-    double pl[NV], pr[NV];
+    R pl[NV], pr[NV];
 #pragma unroll
     for (int v0 = 0; v0 < NV; v0 += P::KI) {
-      const double *src[P::KI];
+      const R *src[P::KI];
       const int K = NV - v0 < P::KI ? NV - v0 : P::KI;
 #pragma unroll
       for (int k = 0; k < P::KI; ++k) {
         const int v = v0 + k < NV ? v0 + k : NV - 1;
-        src[k] = v < 5 ? B.bv + v * np : B.yk + (v - 5) * np;
+        src[k] = v < 5 ? BV_ + v * np : YK_ + (v - 5) * np;
       }
       stage(BI{}, src, K, bo);
 #pragma unroll
       for (int k = 0; k < P::KI; ++k) {
         if (v0 + k >= NV) break;
-        const double *q = stg + k * BI::V + li;
-        muscl<double>(S, q[-sa], q[0], q[sa], q[2 * sa], pl[v0 + k], pr[v0 + k]);
+        const R *q = stg + k * BI::V + li;
+        muscl<R>(S, q[-sa], q[0], q[sa], q[2 * sa], pl[v0 + k], pr[v0 + k]);
       }
     }
-    double F[NV];
+    R F[NV];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) F[v] = 0.0;
+    for (int v = 0; v < NV; ++v) F[v] = R(0);
     if (face_ok) {
-      const double *M = B.metric + 3 * A * np;
-      const double jl = B.jac[pl_i], jr = B.jac[pr_i];
-      k1 = 0.5 * (jl * M[pl_i] + jr * M[pr_i]);
-      k2 = 0.5 * (jl * M[np + pl_i] + jr * M[np + pr_i]);
-      k3 = 0.5 * (jl * M[2 * np + pl_i] + jr * M[2 * np + pr_i]);
-      ausm_from_states<NS, double>(sp, S, pl, pr, k1, k2, k3, F);
+      const R *M = MT_ + 3 * A * np;
+      const R jl = JC_[pl_i], jr = JC_[pr_i];
+      k1 = R(0.5) * (jl * M[pl_i] + jr * M[pr_i]);
+      k2 = R(0.5) * (jl * M[np + pl_i] + jr * M[np + pr_i]);
+      k3 = R(0.5) * (jl * M[2 * np + pl_i] + jr * M[2 * np + pr_i]);
+      ausm_from_states<NS, R>(sp, S, pl, pr, k1, k2, k3, F);
     }
 #pragma unroll
     for (int v = 0; v < NV; ++v) fbuf[v * NT + tid] = F[v];
@@ -179,33 +185,33 @@ __global__ void __launch_bounds__(TA *TB *TC) __attribute__((amdgpu_waves_per_eu
     l[Cx] = t[Cx] + 1;
     const int li = l[0] + BV::L0 * (l[1] + BV::L1 * l[2]);
     constexpr int sa = BV::sa, sb = BV::sb, sc = BV::sc;
-    double mf[9];
+    R mf[9];
     if (face_ok) {
       const bool q5 = (S.quirks & CFX_Q_METRIC_AVG) && DIR != 1;
 #pragma unroll
       for (int rc = 0; rc < 9; ++rc) {
-        const double a = B.metric[rc * np + pl_i];
-        mf[rc] = (q5 && rc != 0) ? 0.5 * (a + a) : 0.5 * (a + B.metric[rc * np + pr_i]);
+        const R a = MT_[rc * np + pl_i];
+        mf[rc] = (q5 && rc != 0) ? R(0.5) * (a + a) : R(0.5) * (a + MT_[rc * np + pr_i]);
       }
     } else {
 #pragma unroll
-      for (int rc = 0; rc < 9; ++rc) mf[rc] = 0.0;
+      for (int rc = 0; rc < 9; ++rc) mf[rc] = R(0);
     }
     // computational-space derivatives of staged variable k at the face
-    auto grad = [&](int k, double gc[3]) {
-      const double *q = stg + k * BV::V + li;
+    auto grad = [&](int k, R gc[3]) {
+      const R *q = stg + k * BV::V + li;
       gc[A] = q[sa] - q[0];
-      gc[Bx] = 0.25 * (q[sb] - q[-sb] + q[sa + sb] - q[sa - sb]);
-      gc[Cx] = 0.25 * (q[sc] - q[-sc] + q[sa + sc] - q[sa - sc]);
+      gc[Bx] = R(0.25) * (q[sb] - q[-sb] + q[sa + sb] - q[sa - sb]);
+      gc[Cx] = R(0.25) * (q[sc] - q[-sc] + q[sa + sc] - q[sa - sc]);
     };
-    auto chain = [&](const double gc[3], double out[3]) {
+    auto chain = [&](const R gc[3], R out[3]) {
       out[0] = gc[0] * mf[0] + gc[1] * mf[3] + gc[2] * mf[6];
       out[1] = gc[0] * mf[1] + gc[1] * mf[4] + gc[2] * mf[7];
       out[2] = gc[0] * mf[2] + gc[1] * mf[5] + gc[2] * mf[8];
     };
-    double gcv[3], du[3], dv[3], dw[3], dT[3], avg[4];
+    R gcv[3], du[3], dv[3], dw[3], dT[3], avg[4];
     {
-      const double *src[P::KV] = {B.bv + np, B.bv + 2 * np, B.bv + 3 * np, B.bv + 5 * np};
+      const R *src[P::KV] = {BV_ + np, BV_ + 2 * np, BV_ + 3 * np, BV_ + 5 * np};
       stage(BV{}, src, 4, bo);
       grad(0, gcv);
       chain(gcv, du);
@@ -216,51 +222,51 @@ __global__ void __launch_bounds__(TA *TB *TC) __attribute__((amdgpu_waves_per_eu
       grad(3, gcv);
       chain(gcv, dT);
 #pragma unroll
-      for (int q = 0; q < 4; ++q) avg[q] = 0.5 * (stg[q * BV::V + li] + stg[q * BV::V + li + sa]);
+      for (int q = 0; q < 4; ++q) avg[q] = R(0.5) * (stg[q * BV::V + li] + stg[q * BV::V + li + sa]);
     }
     {
-      const double vis = face_ok ? 0.5 * (B.props[2 * np + pl_i] + B.props[2 * np + pr_i]) : 0.0;
-      const double cond = face_ok ? 0.5 * (B.props[3 * np + pl_i] + B.props[3 * np + pr_i]) : 0.0;
-      const double third = 1.0 / 3.0;
-      const double txx = vis * (4 * du[0] - 2 * dv[1] - 2 * dw[2]) * third;
-      const double tyy = vis * (4 * dv[1] - 2 * du[0] - 2 * dw[2]) * third;
-      const double tzz = vis * (4 * dw[2] - 2 * du[0] - 2 * dv[1]) * third;
-      const double txy = vis * (du[1] + dv[0]);
-      const double txz = vis * (du[2] + dw[0]);
-      const double tyz = vis * (dv[2] + dw[1]);
-      const double G1 = k1 * txx + k2 * txy + k3 * txz;
-      const double G2 = k1 * txy + k2 * tyy + k3 * tyz;
-      const double G3 = k1 * txz + k2 * tyz + k3 * tzz;
-      const double G4 = avg[0] * G1 + avg[1] * G2 + avg[2] * G3 + cond * (k1 * dT[0] + k2 * dT[1] + k3 * dT[2]);
+      const R vis = face_ok ? R(0.5) * (PR_[2 * np + pl_i] + PR_[2 * np + pr_i]) : R(0);
+      const R cond = face_ok ? R(0.5) * (PR_[3 * np + pl_i] + PR_[3 * np + pr_i]) : R(0);
+      const R third = R(1.0 / 3.0);
+      const R txx = vis * (R(4) * du[0] - R(2) * dv[1] - R(2) * dw[2]) * third;
+      const R tyy = vis * (R(4) * dv[1] - R(2) * du[0] - R(2) * dw[2]) * third;
+      const R tzz = vis * (R(4) * dw[2] - R(2) * du[0] - R(2) * dv[1]) * third;
+      const R txy = vis * (du[1] + dv[0]);
+      const R txz = vis * (du[2] + dw[0]);
+      const R tyz = vis * (dv[2] + dw[1]);
+      const R G1 = k1 * txx + k2 * txy + k3 * txz;
+      const R G2 = k1 * txy + k2 * tyy + k3 * tyz;
+      const R G3 = k1 * txz + k2 * tyz + k3 * tzz;
+      const R G4 = avg[0] * G1 + avg[1] * G2 + avg[2] * G3 + cond * (k1 * dT[0] + k2 * dT[1] + k3 * dT[2]);
       fbuf[1 * NT + tid] -= G1;
       fbuf[2 * NT + tid] -= G2;
       fbuf[3 * NT + tid] -= G3;
       fbuf[4 * NT + tid] -= G4;
     }
     if constexpr (NS > 0) {
-      const double tm = avg[3];
+      const R tm = avg[3];
       const long long pD = (DIR == 2 && (S.quirks & CFX_Q_HV_RHOD_J)) ? pl_i + B.sy : pr_i;
-      double dp[NS], yb[NS];
-      double ucx = 0, ucy = 0, ucz = 0, e4 = 0;
+      R dp[NS], yb[NS];
+      R ucx = 0, ucy = 0, ucz = 0, e4 = 0;
 #pragma unroll
       for (int s0 = 0; s0 < NS; s0 += P::KV) {
-        const double *src[P::KV];
+        const R *src[P::KV];
         const int K = NS - s0 < P::KV ? NS - s0 : P::KV;
 #pragma unroll
-        for (int k = 0; k < P::KV; ++k) src[k] = B.yk + (s0 + k < NS ? s0 + k : NS - 1) * np;
+        for (int k = 0; k < P::KV; ++k) src[k] = YK_ + (s0 + k < NS ? s0 + k : NS - 1) * np;
         stage(BV{}, src, K, bo);
 #pragma unroll
         for (int k = 0; k < P::KV; ++k) {
           const int s = s0 + k;
           if (s >= NS) break;
           grad(k, gcv);
-          double dy[3];
+          R dy[3];
           chain(gcv, dy);
-          yb[s] = 0.5 * (stg[k * BV::V + li] + stg[k * BV::V + li + sa]);
-          const double dif = face_ok ? 0.5 * (B.props[(4 + s) * np + pl_i] + B.props[(4 + s) * np + pD]) : 0.0;
-          double h, cp;
-          species_h_cp<double>(sp, s, tm, h, cp);
-          const double proj = k1 * dy[0] + k2 * dy[1] + k3 * dy[2];
+          yb[s] = R(0.5) * (stg[k * BV::V + li] + stg[k * BV::V + li + sa]);
+          const R dif = face_ok ? R(0.5) * (PR_[(4 + s) * np + pl_i] + PR_[(4 + s) * np + pD]) : R(0);
+          R h, cp;
+          species_h_cp<R>(sp, s, tm, h, cp);
+          const R proj = k1 * dy[0] + k2 * dy[1] + k3 * dy[2];
           e4 += dif * h * proj;
           ucx += dif * dy[0];
           ucy += dif * dy[1];
@@ -268,7 +274,7 @@ __global__ void __launch_bounds__(TA *TB *TC) __attribute__((amdgpu_waves_per_eu
           dp[s] = dif * proj;
         }
       }
-      const double uc = k1 * ucx + k2 * ucy + k3 * ucz;
+      const R uc = k1 * ucx + k2 * ucy + k3 * ucz;
       fbuf[4 * NT + tid] -= e4;
 #pragma unroll
       for (int s = 0; s < NS; ++s) fbuf[(5 + s) * NT + tid] -= dp[s] - yb[s] * uc;
@@ -281,11 +287,12 @@ __global__ void __launch_bounds__(TA *TB *TC) __attribute__((amdgpu_waves_per_eu
     int c[3] = {g[0], g[1], g[2]};
     c[A] += 1;
     const long long q = ((long long)c[2] * B.my + c[1]) * B.mx + c[0];
+    R *__restrict__ DQ = Fd::dq(B);
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-      const double d = -(fbuf[v * NT + tid + ta_stride] - fbuf[v * NT + tid]);
-      if (accumulate) B.dq[v * B.ni + q] += d;
-      else B.dq[v * B.ni + q] = d;
+      const R d = -(fbuf[v * NT + tid + ta_stride] - fbuf[v * NT + tid]);
+      if (accumulate) DQ[v * B.ni + q] += d;
+      else DQ[v * B.ni + q] = d;
     }
   }
 }
