@@ -721,12 +721,15 @@ struct FusedVariant {
 struct PencilVariant {
   int ta[3], tb[3], tc[3], wpe;
 };
+// (variant 0 and 2/3 visit tiles in 4x4 tangential groups, variant 1 in
+// plain order)
 static const PencilVariant kPencil3[] = {{{32, 16, 16}, {4, 8, 8}, {2, 2, 2}, 2},
-                                         {{32, 16, 16}, {8, 16, 16}, {1, 1, 1}, 2},
-                                         {{64, 32, 32}, {2, 4, 4}, {2, 2, 2}, 2},
-                                         {{32, 16, 16}, {4, 8, 8}, {2, 2, 2}, 1}};
+                                         {{32, 16, 16}, {4, 8, 8}, {2, 2, 2}, 2},
+                                         {{32, 16, 16}, {4, 4, 4}, {4, 4, 4}, 2},
+                                         {{64, 32, 32}, {2, 4, 4}, {2, 2, 2}, 2}};
 static const PencilVariant kPencil2[] = {{{32, 16, 0}, {8, 16, 0}, {1, 1, 0}, 2},
                                          {{64, 32, 0}, {4, 8, 0}, {1, 1, 0}, 2},
+                                         {{32, 16, 0}, {8, 16, 0}, {1, 1, 0}, 2},
                                          {{32, 16, 0}, {8, 16, 0}, {1, 1, 0}, 1}};
 static constexpr int kNPencil = 4;  // slots reserved for pencil variants in both dimensions
 static const FusedVariant kRhs3[] = {{16, 4, 4, 1}, {8, 8, 4, 1}, {32, 4, 2, 1}, {16, 8, 2, 1}, {64, 2, 2, 1},
@@ -740,7 +743,7 @@ int n_rhs_tiles(int dim) {
 int rhs_tile(int dim, int tile, int &tx, int &ty, int &tz) {
   if (tile < 0 || tile >= n_rhs_tiles(dim)) return -1;
   if (tile < kNPencil) {
-    const PencilVariant &v = dim == 3 ? kPencil3[tile] : kPencil2[tile < 3 ? tile : 2];
+    const PencilVariant &v = dim == 3 ? kPencil3[tile] : kPencil2[tile];
     tx = v.ta[0], ty = v.tb[0], tz = v.tc[0];
     return 0;
   }
@@ -759,7 +762,7 @@ static int launch_rhs_t(const BlockView &B, const SpeciesDev *sp, const SchemeDe
   return (int)hipGetLastError();
 }
 
-template <int NS, int DIR, int TA, int TB, int TC, int WPE, typename R>
+template <int NS, int DIR, int TA, int TB, int TC, int WPE, typename R, int G>
 static int launch_pencil_t(const BlockView &B, const SpeciesDev *sp, const SchemeDev &S, int accumulate,
                            hipStream_t st) {
   constexpr int A = Axes<DIR>::a, Bx = Axes<DIR>::b, Cx = Axes<DIR>::c;
@@ -767,46 +770,48 @@ static int launch_pencil_t(const BlockView &B, const SpeciesDev *sp, const Schem
   const int nta = (m[A] + TA - 2) / (TA - 1);  // TA faces cover TA-1 cells
   const int ntb = (m[Bx] + TB - 1) / TB;
   const int ntc = (m[Cx] + TC - 1) / TC;
-  rhs_pencil_kernel<NS, DIR, TA, TB, TC, WPE, R>
-      <<<nta * ntb * ntc, TA * TB * TC, 0, st>>>(B, sp, S, accumulate, nta, ntb, ntc);
+  const int ngrid = nta * G * G * ((ntb + G - 1) / G) * ((ntc + G - 1) / G);
+  rhs_pencil_kernel<NS, DIR, TA, TB, TC, WPE, R, G>
+      <<<ngrid, TA * TB * TC, 0, st>>>(B, sp, S, accumulate, nta, ntb, ntc);
   return (int)hipGetLastError();
 }
 
 template <int NS, int TA0, int TB0, int TC0, int TA1, int TB1, int TC1, int TA2, int TB2, int TC2, int WPE,
-          typename R>
+          typename R, int G>
 static int launch_pencils_r(const BlockView &B, const SpeciesDev *sp, const SchemeDev &S, hipStream_t st) {
-  int rc = launch_pencil_t<NS, 0, TA0, TB0, TC0, WPE, R>(B, sp, S, 0, st);
+  int rc = launch_pencil_t<NS, 0, TA0, TB0, TC0, WPE, R, G>(B, sp, S, 0, st);
   if (rc) return rc;
-  rc = launch_pencil_t<NS, 1, TA1, TB1, TC1, WPE, R>(B, sp, S, 1, st);
+  rc = launch_pencil_t<NS, 1, TA1, TB1, TC1, WPE, R, G>(B, sp, S, 1, st);
   if (rc || S.dim == 2) return rc;
-  return launch_pencil_t<NS, 2, TA2, TB2, TC2, WPE, R>(B, sp, S, 1, st);
+  return launch_pencil_t<NS, 2, TA2, TB2, TC2, WPE, R, G>(B, sp, S, 1, st);
 }
 
 // FP64 with the variant's target occupancy; FP32 (mixed precision) needs
 // half the registers and targets one more wave per SIMD
-template <int NS, int TA0, int TB0, int TC0, int TA1, int TB1, int TC1, int TA2, int TB2, int TC2, int WPE>
+template <int NS, int TA0, int TB0, int TC0, int TA1, int TB1, int TC1, int TA2, int TB2, int TC2, int WPE,
+          int G = 1>
 static int launch_pencils(const BlockView &B, const SpeciesDev *sp, const SchemeDev &S, hipStream_t st) {
   if (B.mixed)
-    return launch_pencils_r<NS, TA0, TB0, TC0, TA1, TB1, TC1, TA2, TB2, TC2, WPE + 1, float>(B, sp, S, st);
-  return launch_pencils_r<NS, TA0, TB0, TC0, TA1, TB1, TC1, TA2, TB2, TC2, WPE, double>(B, sp, S, st);
+    return launch_pencils_r<NS, TA0, TB0, TC0, TA1, TB1, TC1, TA2, TB2, TC2, WPE + 1, float, G>(B, sp, S, st);
+  return launch_pencils_r<NS, TA0, TB0, TC0, TA1, TB1, TC1, TA2, TB2, TC2, WPE, double, G>(B, sp, S, st);
 }
 
 template <int NS>
 static int launch_rhs_ns(const BlockView &B, const SpeciesDev *sp, const SchemeDev &S, int tile, hipStream_t st) {
   if (S.dim == 3) {
     switch (tile) {
-      case 0: return launch_pencils<NS, 32, 4, 2, 16, 8, 2, 16, 8, 2, 2>(B, sp, S, st);
-      case 1: return launch_pencils<NS, 32, 8, 1, 16, 16, 1, 16, 16, 1, 2>(B, sp, S, st);
-      case 2: return launch_pencils<NS, 64, 2, 2, 32, 4, 2, 32, 4, 2, 2>(B, sp, S, st);
-      case 3: return launch_pencils<NS, 32, 4, 2, 16, 8, 2, 16, 8, 2, 1>(B, sp, S, st);
+      case 0: return launch_pencils<NS, 32, 4, 2, 16, 8, 2, 16, 8, 2, 2, 4>(B, sp, S, st);
+      case 1: return launch_pencils<NS, 32, 4, 2, 16, 8, 2, 16, 8, 2, 2, 1>(B, sp, S, st);
+      case 2: return launch_pencils<NS, 32, 4, 4, 16, 4, 4, 16, 4, 4, 2, 4>(B, sp, S, st);
+      case 3: return launch_pencils<NS, 64, 2, 2, 32, 4, 2, 32, 4, 2, 2, 4>(B, sp, S, st);
       default: break;
     }
   } else {
     switch (tile) {
-      case 0: return launch_pencils<NS, 32, 8, 1, 16, 16, 1, 16, 16, 1, 2>(B, sp, S, st);
-      case 1: return launch_pencils<NS, 64, 4, 1, 32, 8, 1, 32, 8, 1, 2>(B, sp, S, st);
-      case 2:
-      case 3: return launch_pencils<NS, 32, 8, 1, 16, 16, 1, 16, 16, 1, 1>(B, sp, S, st);
+      case 0: return launch_pencils<NS, 32, 8, 1, 16, 16, 1, 16, 16, 1, 2, 4>(B, sp, S, st);
+      case 1: return launch_pencils<NS, 64, 4, 1, 32, 8, 1, 32, 8, 1, 2, 4>(B, sp, S, st);
+      case 2: return launch_pencils<NS, 32, 8, 1, 16, 16, 1, 16, 16, 1, 2, 1>(B, sp, S, st);
+      case 3: return launch_pencils<NS, 32, 8, 1, 16, 16, 1, 16, 16, 1, 1, 4>(B, sp, S, st);
       default: break;
     }
   }

@@ -66,7 +66,28 @@ struct PencilShape {
   static constexpr int STG = KI * BI::V > KV * BV::V ? KI * BI::V : KV * BV::V;
 };
 
-template <int NS, int DIR, int TA, int TB, int TC, int WPE, typename R>
+// Tile order: workgroups are dealt to the 8 XCDs round-robin, so each XCD
+// receives a contiguous run of the tile sequence; inside it the tiles of the
+// (b, c) plane are visited in G x G groups with a fastest, so that tiles
+// sharing a tangential halo run close together and reuse it from L2.
+// Returns false for the padding tiles of partial groups.
+template <int G>
+__device__ __forceinline__ bool pencil_tile(int bid, int nta, int ntb, int ntc, int &ia, int &ib, int &ic) {
+  const int gb = (ntb + G - 1) / G, gc = (ntc + G - 1) / G;
+  const int nt = nta * G * G * gb * gc;
+  const int lid = nt % 8 == 0 ? (bid % 8) * (nt / 8) + bid / 8 : bid;
+  ia = lid % nta;
+  int q = lid / nta;
+  const int bin = q % G;
+  q /= G;
+  const int cin = q % G;
+  q /= G;
+  ib = (q % gb) * G + bin;
+  ic = (q / gb) * G + cin;
+  return ib < ntb && ic < ntc;
+}
+
+template <int NS, int DIR, int TA, int TB, int TC, int WPE, typename R, int G = 1>
 __global__ void __launch_bounds__(TA *TB *TC) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
     rhs_pencil_kernel(BlockView B, const SpeciesDev *__restrict__ spg, SchemeDev S, int accumulate, int nta, int ntb,
                       int ntc) {
@@ -88,13 +109,12 @@ __global__ void __launch_bounds__(TA *TB *TC) __attribute__((amdgpu_waves_per_eu
   const long long np = B.np;
   const int m[3] = {B.mx, B.my, B.mz};
   const long long st[3] = {1, B.sy, B.sz};
-  const int nt = nta * ntb * ntc;
-  const int bid = blockIdx.x;
-  const int lid = nt % 8 == 0 ? (bid % 8) * (nt / 8) + bid / 8 : bid;
+  int ia, ib, ic;
+  if (!pencil_tile<G>(blockIdx.x, nta, ntb, ntc, ia, ib, ic)) return;  // uniform per workgroup
   int o[3];
-  o[A] = (lid % nta) * (TA - 1);
-  o[Bx] = ((lid / nta) % ntb) * TB;
-  o[Cx] = (lid / (nta * ntb)) * TC;
+  o[A] = ia * (TA - 1);
+  o[Bx] = ib * TB;
+  o[Cx] = ic * TC;
   const int tid = threadIdx.x;
   int t[3];
   t[0] = tid % P::T0;
