@@ -22,7 +22,17 @@ ARCH = os.environ.get("CFX_OFFLOAD_ARCH", "gfx950")
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-SOURCES = ["cfx_api.cpp", "mesh.cpp", "kernels.hip"]
+# Species counts with compiled kernels (the species loops are unrolled at
+# compile time); kernels_ns.hip is built once per count and precision.
+SPECIES_COUNTS = (9, 5, 3, 2, 1, 0)
+SOURCES = ["cfx_api.cpp", "mesh.cpp", "kernels_common.hip"]
+
+
+def _units():
+    units = [(f"kernels_ns{n}_f{prec}", "kernels_ns.hip", [f"-DCFX_NS={n}", f"-DCFX_PREC={prec}"])
+             for n in SPECIES_COUNTS for prec in (64, 32)]
+    units += [(os.path.splitext(s)[0], s, []) for s in SOURCES]
+    return units
 
 
 def _hipcc():
@@ -32,19 +42,22 @@ def _hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def _compile(src, extra):
-    out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [_hipcc(), *COMMON, *extra, "-c", os.path.join(CSRC, src), "-o", out]
+def _compile(unit, extra):
+    name, src, defs = unit
+    out = os.path.join(OBJ, name + ".o")
+    cmd = [_hipcc(), *COMMON, *defs, *extra, "-c", os.path.join(CSRC, src), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     return out, r.stderr
 
 
-def build(verbose=False, extra=()):
+def build(verbose=False, extra=(), jobs=None):
     os.makedirs(OBJ, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
-        results = list(ex.map(lambda s: _compile(s, list(extra)), SOURCES))
+    units = _units()
+    jobs = jobs or min(16, os.cpu_count() or 4, len(units))
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        results = list(ex.map(lambda u: _compile(u, list(extra)), units))
     objs = [o for o, _ in results]
     if verbose:
         for _, err in results:

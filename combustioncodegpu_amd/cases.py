@@ -14,7 +14,9 @@ Workloads:
   c1   3-D 256x128x128, 9 synthetic species (NASA-7, Lennard-Jones),
        smooth seeded T/p/u fields, Dirichlet mass fractions smoothed,
        outflow on all faces, Van Albada, AUSM+, viscous + Wilke, RK3.
-Both accept reduced sizes for tests.
+  c3   64 blocks on a 4x4x4 face-matched lattice, extents drawn from
+       [16, 160] per lattice axis (about 40M points), c1 species and fields.
+All accept reduced sizes for tests.
 """
 from __future__ import annotations
 
@@ -39,12 +41,13 @@ def _mix(z: np.ndarray) -> np.ndarray:
     return z ^ (z >> np.uint64(31))
 
 
-def uniform(seed: int, stream: int, n: int) -> np.ndarray:
-    """n doubles in [0,1) from SplitMix64 of counters 1..n of (seed, stream)."""
+def uniform(seed: int, stream: int, n: int, offset: int = 0) -> np.ndarray:
+    """n doubles in [0,1) from SplitMix64 of counters offset+1..offset+n of
+    (seed, stream)."""
     with np.errstate(over="ignore"):
         base = _mix(np.array([(seed * 0x100000001B3 + stream * 0x9E3779B1 + 1) & 0xFFFFFFFFFFFFFFFF],
                              dtype=np.uint64))[0]
-        ctr = np.arange(1, n + 1, dtype=np.uint64)
+        ctr = np.arange(offset + 1, offset + n + 1, dtype=np.uint64)
         z = _mix(base + ctr * _GOLDEN)
     return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
 
@@ -229,32 +232,32 @@ def smooth_field(seed, stream, shape_idx, lengths):
 
 def smoothed_dirichlet(seed, stream, n_spec, shape, passes=4):
     """Per point Dirichlet(1) mass fractions (normalised exponentials), then
-    `passes` separable 3-point box filters (edge-replicated), renormalised."""
+    `passes` separable 3-point box filters (edge-replicated), renormalised.
+    Species are drawn and filtered one at a time (the filter is linear per
+    species), which bounds the temporaries on large grids."""
     n = int(np.prod(shape))
-    u = uniform(seed, stream, n_spec * n).reshape((n_spec,) + shape)
-    e = -np.log1p(-u)
-    y = e / e.sum(axis=0, keepdims=True)
-    for _ in range(passes):
-        for ax in (1, 2, 3):
-            yp = np.concatenate([np.take(y, [0], axis=ax), y, np.take(y, [-1], axis=ax)], axis=ax)
-            sl = [slice(None)] * 4
-            a = []
-            for o in range(3):
-                sl[ax] = slice(o, o + y.shape[ax])
-                a.append(yp[tuple(sl)])
-            y = (a[0] + a[1] + a[2]) / 3.0
+    y = np.empty((n_spec,) + tuple(shape))
+    for l in range(n_spec):
+        y[l] = -np.log1p(-uniform(seed, stream, n, offset=l * n)).reshape(shape)
+    y /= y.sum(axis=0, keepdims=True)
+    for l in range(n_spec):
+        a = y[l]
+        for _ in range(passes):
+            for ax in (0, 1, 2):
+                ap = np.concatenate([np.take(a, [0], axis=ax), a, np.take(a, [-1], axis=ax)], axis=ax)
+                sl = [slice(None)] * 3
+                parts = []
+                for o in range(3):
+                    sl[ax] = slice(o, o + a.shape[ax])
+                    parts.append(ap[tuple(sl)])
+                a = (parts[0] + parts[1] + parts[2]) / 3.0
+        y[l] = a
     return y / y.sum(axis=0, keepdims=True)
 
 
-def c1(mx=256, my=128, mz=128, n_spec=9, seed=11, dx=1e-3, perturb=0.0, cfl=0.5, rk_stages=3, limiter=1,
-       bc="outflow", species_seed=2024, viscous=2, quirks=abi.QUIRKS_REFERENCE):
-    """3-D block with n_spec synthetic species and smooth seeded fields:
-    T = 1400 + 1100 S_T, p = 101325 (1 + 0.1 S_p), u_d = 200/sqrt(3) S_ud,
-    Y smoothed Dirichlet(1); outflow on all faces."""
-    sp = synthetic_species(n_spec, species_seed) if n_spec > 0 else None
-    x, y, z = cartesian_grid(mx, my, mz, dx, perturb=perturb, seed=seed)
-    ii, jj, kk = padded_index_coords(mx, my, mz)
-    L = (max(mx - 1, 1), max(my - 1, 1), max(mz - 1, 1))
+def _smooth_fields(seed, n_spec, sp, idx, L):
+    """The c1 field generator on index coordinates idx = (i, j, k) arrays."""
+    ii, jj, kk = idx
     T = 1400.0 + 1100.0 * smooth_field(seed, 10, (ii, jj, kk), L)
     p = 101325.0 * (1.0 + 0.1 * smooth_field(seed, 11, (ii, jj, kk), L))
     amp = 200.0 / math.sqrt(3.0)
@@ -268,12 +271,114 @@ def c1(mx=256, my=128, mz=128, n_spec=9, seed=11, dx=1e-3, perturb=0.0, cfl=0.5,
         Y = None
         r_mix = np.full(ii.shape, R_U / 28.9647)
     rho = p / (r_mix * T)
-    bv = np.stack([rho, u, v, w, p, T])
+    return np.stack([rho, u, v, w, p, T]), Y
+
+
+def c1(mx=256, my=128, mz=128, n_spec=9, seed=11, dx=1e-3, perturb=0.0, cfl=0.5, rk_stages=3, limiter=1,
+       bc="outflow", species_seed=2024, viscous=2, quirks=abi.QUIRKS_REFERENCE):
+    """3-D block with n_spec synthetic species and smooth seeded fields:
+    T = 1400 + 1100 S_T, p = 101325 (1 + 0.1 S_p), u_d = 200/sqrt(3) S_ud,
+    Y smoothed Dirichlet(1); outflow on all faces."""
+    sp = synthetic_species(n_spec, species_seed) if n_spec > 0 else None
+    x, y, z = cartesian_grid(mx, my, mz, dx, perturb=perturb, seed=seed)
+    L = (max(mx - 1, 1), max(my - 1, 1), max(mz - 1, 1))
+    bv, Y = _smooth_fields(seed, n_spec, sp, padded_index_coords(mx, my, mz), L)
     blk = BlockSpec(mx, my, mz, x, y, z, bc=outflow_all_faces(mx, my, mz))
     params = dict(inviscid_scheme=3, reconstruction=2, limiter=limiter, viscous_order=viscous,
                   rk_stages=rk_stages, cfl=cfl, prandtl=0.72, schmidt=0.9, quirks=quirks)
     return Case("c1", params, sp, [blk], [], [], [np.ascontiguousarray(bv)],
                 [None if Y is None else np.ascontiguousarray(Y)])
+
+
+# ---------------------------------------------------------------------- C3 ---
+def lattice_sizes(nb, lo, hi, seed):
+    """Per lattice axis a, nb[a] block extents drawn uniformly from the
+    integers [lo, hi]; block (a, b, c) has extents (X[a], Y[b], Z[c]), so
+    neighbours share whole faces."""
+    out = []
+    for ax in range(3):
+        u = uniform(seed, 40 + ax, nb[ax])
+        out.append([lo + int((hi - lo + 1) * x) for x in u])
+    return out
+
+
+def lattice(sizes, n_spec=9, seed=11, dx=1e-3, cfl=0.5, rk_stages=3, limiter=1, species_seed=2024, viscous=2,
+            quirks=abi.QUIRKS_REFERENCE, name="lattice"):
+    """Blocks on a face-matched lattice: sizes[a] lists the vertex extents
+    along axis a; neighbouring blocks share their interface vertex layer and
+    are joined by inner faces (both directions); the outer faces of the
+    lattice are outflow.  Grid, fields and species come from the c1
+    generator evaluated on the global vertex grid, so the blocks together
+    hold exactly the fields of one block of the global extent."""
+    nbx, nby, nbz = (len(s) for s in sizes)
+    G = [sum(s) - (len(s) - 1) for s in sizes]   # global vertex counts
+    org = [np.concatenate([[0], np.cumsum(np.array(s) - 1)[:-1]]).astype(int) for s in sizes]
+    sp = synthetic_species(n_spec, species_seed) if n_spec > 0 else None
+    gx, gy, gz = cartesian_grid(G[0], G[1], G[2], dx)
+    gshape = (G[2], G[1], G[0])
+    gxyz = [a.reshape(gshape) for a in (gx, gy, gz)]
+    idx = padded_index_coords(G[0], G[1], G[2])
+    L = (max(G[0] - 1, 1), max(G[1] - 1, 1), max(G[2] - 1, 1))
+    bv_g, y_g = _smooth_fields(seed, n_spec, sp, idx, L)
+    del idx
+
+    def bid(a, b, c):
+        return a + nbx * (b + nby * c)
+
+    blocks, bvs, yks = [], [], []
+    for c in range(nbz):
+        for b in range(nby):
+            for a in range(nbx):
+                mx, my, mz = sizes[0][a], sizes[1][b], sizes[2][c]
+                ox, oy, oz = org[0][a], org[1][b], org[2][c]
+                xs = [np.ascontiguousarray(g[oz:oz + mz, oy:oy + my, ox:ox + mx]).ravel() for g in gxyz]
+                X, Y, Z = mx - 1, my - 1, mz - 1
+                bc, inner = [], []
+                # (axis, side, neighbour lattice index or None)
+                nbr = [(0, 0, (a - 1, b, c) if a > 0 else None), (0, 1, (a + 1, b, c) if a < nbx - 1 else None),
+                       (1, 0, (a, b - 1, c) if b > 0 else None), (1, 1, (a, b + 1, c) if b < nby - 1 else None),
+                       (2, 0, (a, b, c - 1) if c > 0 else None), (2, 1, (a, b, c + 1) if c < nbz - 1 else None)]
+                ext = (mx, my, mz)
+                for ax, side, nb_ in nbr:
+                    if nb_ is None:
+                        rs = [0, 0, 0]
+                        re = [X, Y, Z]
+                        if side == 0:
+                            re[ax] = 0
+                        else:
+                            rs[ax] = ext[ax] - 1
+                        bc.append((tuple(rs), tuple(re), abi.BC_OUTFLOW, 0))
+                        continue
+                    text = (sizes[0][nb_[0]], sizes[1][nb_[1]], sizes[2][nb_[2]])
+                    src_r, tar_r = [], []
+                    first = True
+                    for q in range(3):
+                        if q == ax:
+                            s_v = 1 if side == 0 else ext[q]
+                            t_v = text[q] if side == 0 else 1
+                            src_r += [s_v, s_v]
+                            tar_r += [t_v, t_v]
+                        elif first:  # first in-plane axis flagged negative
+                            src_r += [-1, -ext[q]]
+                            tar_r += [-1, -text[q]]
+                            first = False
+                        else:
+                            src_r += [1, ext[q]]
+                            tar_r += [1, text[q]]
+                    inner.append((src_r, tar_r, bid(*nb_) + 1))
+                blocks.append(BlockSpec(mx, my, mz, *xs, bc=bc, inner=inner))
+                sl = np.s_[oz:oz + mz + 2 * NGG, oy:oy + my + 2 * NGG, ox:ox + mx + 2 * NGG]
+                bvs.append(np.ascontiguousarray(bv_g[(slice(None),) + sl]))
+                yks.append(None if y_g is None else np.ascontiguousarray(y_g[(slice(None),) + sl]))
+    params = dict(inviscid_scheme=3, reconstruction=2, limiter=limiter, viscous_order=viscous,
+                  rk_stages=rk_stages, cfl=cfl, prandtl=0.72, schmidt=0.9, quirks=quirks)
+    return Case(name, params, sp, blocks, [], [], bvs, yks)
+
+
+def c3(nb=(4, 4, 4), lo=16, hi=160, seed=5, n_spec=9, **kw):
+    """64 blocks (4 x 4 x 4 lattice) with extents drawn from [16, 160]: about
+    40M points, the c1 species and field generator."""
+    return lattice(lattice_sizes(nb, lo, hi, seed), n_spec=n_spec, name="c3", **kw)
 
 
 def cell_stages(case: Case, steps: int) -> int:

@@ -29,6 +29,22 @@ struct TimedLaunch {
   int cls;
 };
 
+// A workgroup prefix over the blocks (nb+1 entries), device and host copies.
+struct Table {
+  int *dev = nullptr;
+  std::vector<int> host;
+};
+
+// Ghost-fill ops grouped into levels: the ops of one level touch disjoint
+// data and run as one launch; lvl[L]..lvl[L+1] are the ops of level L.
+struct Schedule {
+  FaceOp *ops = nullptr;
+  int *off = nullptr;
+  std::vector<int> off_host;
+  std::vector<int> lvl;
+  int levels() const { return lvl.empty() ? 0 : (int)lvl.size() - 1; }
+};
+
 }  // namespace
 
 struct cfx_ctx {
@@ -51,10 +67,20 @@ struct cfx_ctx {
   unsigned long long *d_newton = nullptr;
   unsigned long long *d_minscratch = nullptr;  // min target of non-global dt passes
   double *d_partial = nullptr;
-  std::vector<int> partial_off;
   int n_partial = 0;
   long long n_grid_total = 0;
-  int rhs_tile = 0;
+  BlockView *views_dev = nullptr;
+  Table t_pad, t_int, t_dt, t_last, t_res;     // point-kernel workgroup prefixes
+  Table t_rhs[3][kNVar3];                      // pencil tiles per direction and variant
+  int occ[2][3][kNVar3] = {};                  // resident workgroups per CU [fp32][dir][variant]
+  int n_cu = 256;
+  int rhs_var[3] = {0, 0, 0}, rhs_persist[3] = {0, 0, 0};
+  std::vector<FaceOp> ops;                     // BC ops in application order, then exchange ops
+  int n_bc_ops = 0;
+  Schedule sched_seq, sched_bc, sched_ex, sched_all;
+  bool batched = true, use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
   bool mixed = false;
   bool profiling = false;
   std::vector<TimedLaunch> timed;
@@ -78,6 +104,14 @@ struct cfx_ctx {
     }
     allocs.push_back(p);
     return (T *)p;
+  }
+  Batch all(const Table &t) const { return Batch{views_dev, t.dev, (int)views.size(), t.host.back() - t.host[0]}; }
+  Batch one(const Table &t, int b) const { return Batch{views_dev + b, t.dev + b, 1, t.host[b + 1] - t.host[b]}; }
+  void drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
   }
   hipEvent_t get_event() {
     if (!event_pool.empty()) {
@@ -247,55 +281,275 @@ int build_inflow(const cfx_inflow &in, const SpeciesT<double> &s, const SchemeDe
     if (rc_ != 0) return ctx->fail(rc_, #expr); \
   } while (0)
 
+// ---------------------------------------------------------------- tables ---
+int upload_table(cfx_ctx *ctx, Table &t, const std::vector<int> &counts) {
+  t.host.assign(counts.size() + 1, 0);
+  for (size_t b = 0; b < counts.size(); ++b) t.host[b + 1] = t.host[b] + counts[b];
+  int rc = 0;
+  t.dev = ctx->dalloc<int>(t.host.size(), rc);
+  if (rc) return rc;
+  return (int)hipMemcpy(t.dev, t.host.data(), sizeof(int) * t.host.size(), hipMemcpyHostToDevice);
+}
+
+int chunks(long long n, int per) { return (int)((n + per - 1) / per); }
+
+int build_tables(cfx_ctx *ctx) {
+  const int nb = (int)ctx->hb.size();
+  std::vector<int> pad(nb), in1(nb), dtc(nb), last(nb), res(nb);
+  for (int b = 0; b < nb; ++b) {
+    const HostBlock &H = ctx->hb[b];
+    pad[b] = chunks(H.np, kTpb);
+    in1[b] = chunks(H.ni, kTpb);
+    dtc[b] = chunks(H.ni, kTpb * kDtIter);
+    last[b] = chunks(H.ni, kTpb * kLastIter);
+    res[b] = chunks(H.ni, kTpb * kResIter);
+  }
+  int rc;
+  if ((rc = upload_table(ctx, ctx->t_pad, pad)) || (rc = upload_table(ctx, ctx->t_int, in1)) ||
+      (rc = upload_table(ctx, ctx->t_dt, dtc)) || (rc = upload_table(ctx, ctx->t_last, last)) ||
+      (rc = upload_table(ctx, ctx->t_res, res)))
+    return rc;
+  for (int d = 0; d < ctx->dim; ++d)
+    for (int v = 0; v < rhs_variants(ctx->dim); ++v) {
+      std::vector<int> tiles(nb);
+      for (int b = 0; b < nb; ++b) {
+        const int m[3] = {ctx->hb[b].mx, ctx->hb[b].my, ctx->hb[b].mz};
+        tiles[b] = pencil_tiles(m, d, rhs_tile_shape(ctx->dim, d, v));
+      }
+      if ((rc = upload_table(ctx, ctx->t_rhs[d][v], tiles))) return rc;
+    }
+  return 0;
+}
+
+// ------------------------------------------------------- ghost-fill ops ---
+// Boxes of points an op reads or writes, per block (unpadded indices, ghosts
+// negative or beyond the extent).
+struct Touch {
+  int blk, lo[3], hi[3];
+  bool write;
+};
+
+Touch box_of(int blk, const int a[3], const int b[3], bool write) {
+  Touch t{blk, {0, 0, 0}, {0, 0, 0}, write};
+  for (int q = 0; q < 3; ++q) {
+    t.lo[q] = std::min(a[q], b[q]);
+    t.hi[q] = std::max(a[q], b[q]);
+  }
+  return t;
+}
+
+std::vector<Touch> touches(const FaceOp &op) {
+  std::vector<Touch> out;
+  if (op.kind == kOpExchange) {
+    const InnerDev &f = op.f;
+    out.push_back(box_of(op.blk, f.rs, f.re, true));
+    int lo[3], hi[3], flo[3], fhi[3];
+    const int face_layer = f.direction > 0 ? f.rs[f.face] : f.re[f.face];
+    const int nface = (face_layer - f.rs[f.face]) * f.loop_dir[f.face];
+    for (int q = 0; q < 3; ++q) {
+      const int s = f.src_tar[q];
+      const int step = f.target_loop_dir[q] * f.loop_dir[s];
+      lo[q] = f.ts[q];
+      hi[q] = f.ts[q] + step * (f.n_point[s] - 1);
+      flo[q] = lo[q];
+      fhi[q] = hi[q];
+      if (s == f.face) flo[q] = fhi[q] = f.ts[q] + step * nface;
+    }
+    out.push_back(box_of(op.tgt, lo, hi, false));
+    out.push_back(box_of(op.tgt, flo, fhi, true));
+    return out;
+  }
+  const FaceRange &r = op.r;
+  const int a = r.face, s = r.direction;
+  int lo[3], hi[3];
+  for (int q = 0; q < 3; ++q) lo[q] = r.rs[q], hi[q] = r.re[q];
+  auto layer = [&](int from, int to, bool write) {
+    int l2[3] = {lo[0], lo[1], lo[2]}, h2[3] = {hi[0], hi[1], hi[2]};
+    l2[a] = r.rs[a] + s * from;
+    h2[a] = r.rs[a] + s * to;
+    out.push_back(box_of(op.blk, l2, h2, write));
+  };
+  switch (op.kind) {
+    case CFX_BC_OUTFLOW: layer(0, 0, false); layer(1, kNgg, true); break;
+    case CFX_BC_INFLOW: layer(1, kNgg, true); break;
+    default: layer(-kNgg, -1, false); layer(0, 0, true); layer(1, kNgg, true); break;  // wall, symmetry
+  }
+  return out;
+}
+
+bool conflict(const std::vector<Touch> &x, const std::vector<Touch> &y) {
+  for (const Touch &a : x)
+    for (const Touch &b : y) {
+      if (a.blk != b.blk || !(a.write || b.write)) continue;
+      bool meet = true;
+      for (int q = 0; q < 3; ++q) meet = meet && a.lo[q] <= b.hi[q] && b.lo[q] <= a.hi[q];
+      if (meet) return true;
+    }
+  return false;
+}
+
+// Levels for ops[first..last): an op goes one level after the latest earlier
+// op it conflicts with, so every conflicting pair keeps the application
+// order and the result equals the one-op-at-a-time loop.  levelize=false
+// gives each op its own level (that loop itself).
+int build_schedule(cfx_ctx *ctx, Schedule &sc, int first, int last, bool levelize) {
+  const int n = last - first;
+  std::vector<int> level(n, 0);
+  if (levelize) {
+    std::vector<std::vector<Touch>> tc(n);
+    for (int i = 0; i < n; ++i) tc[i] = touches(ctx->ops[first + i]);
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < j; ++i)
+        if (level[i] + 1 > level[j] && conflict(tc[i], tc[j])) level[j] = level[i] + 1;
+  } else {
+    for (int i = 0; i < n; ++i) level[i] = i;
+  }
+  const int nlev = n ? *std::max_element(level.begin(), level.end()) + 1 : 0;
+  std::vector<FaceOp> ordered;
+  sc.lvl.assign(1, 0);
+  for (int L = 0; L < nlev; ++L) {
+    for (int i = 0; i < n; ++i)
+      if (level[i] == L) ordered.push_back(ctx->ops[first + i]);
+    sc.lvl.push_back((int)ordered.size());
+  }
+  sc.off_host.assign(ordered.size() + 1, 0);
+  for (size_t i = 0; i < ordered.size(); ++i) sc.off_host[i + 1] = sc.off_host[i] + chunks(ordered[i].npts, kTpb);
+  int rc = 0;
+  sc.ops = ctx->dalloc<FaceOp>(ordered.size(), rc);
+  sc.off = ctx->dalloc<int>(sc.off_host.size(), rc);
+  if (rc) return rc;
+  if (!ordered.empty())
+    if ((rc = (int)hipMemcpy(sc.ops, ordered.data(), sizeof(FaceOp) * ordered.size(), hipMemcpyHostToDevice)))
+      return rc;
+  return (int)hipMemcpy(sc.off, sc.off_host.data(), sizeof(int) * sc.off_host.size(), hipMemcpyHostToDevice);
+}
+
+// BC ops in the application order (BoundCond.cu:251-313), then the inner-face
+// exchanges block by block, face by face (Driver.cu:194-217).
+int build_ops(cfx_ctx *ctx) {
+  ctx->ops.clear();
+  for (const BcOp &b : ctx->bc_order) {
+    const HostBlock &H = ctx->hb[b.blk];
+    FaceOp op{};
+    op.kind = b.type;
+    op.blk = b.blk;
+    op.tgt = b.blk;
+    op.cond = H.bc_cond[b.face];
+    op.r = H.bc_range[b.face];
+    if (b.type == CFX_BC_WALL) {
+      op.wl = ctx->walls[op.cond];
+      for (int q = 0; q < 3; ++q) {
+        if (q == op.r.face) continue;
+        op.r.rs[q] += kNgg;
+        op.r.re[q] -= kNgg;
+      }
+      if (op.r.rs[0] > op.r.re[0] || op.r.rs[1] > op.r.re[1] || op.r.rs[2] > op.r.re[2]) continue;
+    }
+    for (int q = 0; q < 3; ++q) op.ext[q] = op.r.re[q] - op.r.rs[q] + 1;
+    op.npts = (long long)op.ext[0] * op.ext[1] * op.ext[2];
+    ctx->ops.push_back(op);
+  }
+  ctx->n_bc_ops = (int)ctx->ops.size();
+  for (size_t b = 0; b < ctx->hb.size(); ++b)
+    for (const InnerDev &f : ctx->hb[b].inner) {
+      FaceOp op{};
+      op.kind = kOpExchange;
+      op.blk = (int)b;
+      op.tgt = f.target_block;
+      op.f = f;
+      for (int q = 0; q < 3; ++q) op.ext[q] = f.n_point[q];
+      op.npts = (long long)op.ext[0] * op.ext[1] * op.ext[2];
+      ctx->ops.push_back(op);
+    }
+  const int n = (int)ctx->ops.size(), nbc = ctx->n_bc_ops;
+  int rc;
+  if ((rc = build_schedule(ctx, ctx->sched_seq, 0, n, false)) || (rc = build_schedule(ctx, ctx->sched_bc, 0, nbc, true)) ||
+      (rc = build_schedule(ctx, ctx->sched_ex, nbc, n, true)) || (rc = build_schedule(ctx, ctx->sched_all, 0, n, true)))
+    return rc;
+  return 0;
+}
+
+// levels [l0, l1) of a schedule, one launch each
+int run_levels(cfx_ctx *ctx, const Schedule &sc, int l0, int l1) {
+  const KernelSet64 &k = g_kernels64[ctx->ns];
+  for (int L = l0; L < l1; ++L) {
+    const int a = sc.lvl[L], b = sc.lvl[L + 1];
+    const int total = sc.off_host[b] - sc.off_host[a];
+    CFX_TRY(ctx->timed_launch(K_BC, [&] {
+      return k.face_ops(ctx->views_dev, sc.ops + a, sc.off + a, b - a, total, ctx->sp_dev, ctx->S, ctx->inflow_dev,
+                        ctx->stream);
+    }));
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------ step parts ---
+int rhs_dir(cfx_ctx *ctx, int d, const Batch &b, int accumulate) {
+  const int v = ctx->rhs_var[d];
+  const RhsEntry &e = ctx->mixed ? (ctx->dim == 3 ? g_kernels32[ctx->ns].rhs3[d][v] : g_kernels32[ctx->ns].rhs2[d][v])
+                                 : (ctx->dim == 3 ? g_kernels64[ctx->ns].rhs3[d][v] : g_kernels64[ctx->ns].rhs2[d][v]);
+  const int padded = (b.total + 7) / 8 * 8;
+  int grid = padded;
+  const int occ = ctx->occ[ctx->mixed ? 1 : 0][d][v];
+  if (ctx->rhs_persist[d] > 0 && occ > 0) grid = std::min(grid, (ctx->rhs_persist[d] * occ * ctx->n_cu + 7) / 8 * 8);
+  return ctx->timed_launch(K_RHS, [&] { return e.launch(b, grid, ctx->sp_dev, ctx->S, accumulate, ctx->stream); });
+}
+
 int do_rhs(cfx_ctx *ctx, int blk) {
+  if (blk < 0 && ctx->batched) {
+    for (int d = 0; d < ctx->dim; ++d) CFX_TRY(rhs_dir(ctx, d, ctx->all(ctx->t_rhs[d][ctx->rhs_var[d]]), d > 0));
+    return 0;
+  }
   const int b0 = blk < 0 ? 0 : blk, b1 = blk < 0 ? (int)ctx->views.size() : blk + 1;
   for (int b = b0; b < b1; ++b)
-    CFX_TRY(ctx->timed_launch(K_RHS, [&] {
-      return launch_rhs(ctx->views[b], ctx->sp_dev, ctx->S, ctx->ns, ctx->rhs_tile, ctx->stream);
-    }));
+    for (int d = 0; d < ctx->dim; ++d) CFX_TRY(rhs_dir(ctx, d, ctx->one(ctx->t_rhs[d][ctx->rhs_var[d]], b), d > 0));
+  return 0;
+}
+
+// runs f(batch) over all blocks at once, or block by block
+template <class F>
+int per_batch(cfx_ctx *ctx, const Table &t, F &&f) {
+  if (ctx->batched) return f(ctx->all(t));
+  for (size_t b = 0; b < ctx->views.size(); ++b)
+    if (int rc = f(ctx->one(t, (int)b))) return rc;
   return 0;
 }
 
 int do_dt(cfx_ctx *ctx, int global, int store_local) {
   auto *bits = reinterpret_cast<unsigned long long *>(ctx->d_dt);
   if (global) CFX_TRY(ctx->timed_launch(K_DT, [&] { return launch_dt_reset(bits, ctx->stream); }));
-  for (auto &B : ctx->views)
-    CFX_TRY(ctx->timed_launch(K_DT, [&] {
-      return launch_dt(B, ctx->S, ctx->ns, store_local, global ? bits : ctx->d_minscratch, ctx->stream);
-    }));
+  CFX_TRY(per_batch(ctx, ctx->t_dt, [&](const Batch &b) {
+    return ctx->timed_launch(K_DT, [&] {
+      return launch_dt(b, ctx->S, ctx->ns, store_local, global ? bits : ctx->d_minscratch, ctx->stream);
+    });
+  }));
   return 0;
 }
 
 int do_update(cfx_ctx *ctx, int stage) {
-  for (auto &B : ctx->views)
-    CFX_TRY(ctx->timed_launch(K_UPDATE, [&] {
-      return launch_update(B, ctx->sp_dev, ctx->S, ctx->ns, stage, ctx->d_dt, ctx->d_flags, ctx->d_newton,
-                           ctx->stream);
-    }));
+  const KernelSet64 &k = g_kernels64[ctx->ns];
+  CFX_TRY(per_batch(ctx, ctx->t_int, [&](const Batch &b) {
+    return ctx->timed_launch(K_UPDATE, [&] {
+      return k.update(b, ctx->sp_dev, ctx->S, stage, ctx->d_dt, ctx->d_flags, ctx->d_newton, ctx->stream);
+    });
+  }));
   return 0;
 }
 
 int do_bc(cfx_ctx *ctx) {
-  for (const BcOp &op : ctx->bc_order) {
-    const HostBlock &H = ctx->hb[op.blk];
-    const int cond = H.bc_cond[op.face];
-    const InflowDev *in = op.type == CFX_BC_INFLOW ? ctx->inflow_dev + cond : nullptr;
-    const WallDev wl = op.type == CFX_BC_WALL ? ctx->walls[cond] : WallDev{0, 300.0};
-    CFX_TRY(ctx->timed_launch(K_BC, [&] {
-      return launch_bc(op.type, ctx->views[op.blk], H.bc_range[op.face], ctx->sp_dev, ctx->S, ctx->ns, in, wl,
-                       ctx->stream);
-    }));
-  }
-  return 0;
+  if (ctx->batched) return run_levels(ctx, ctx->sched_bc, 0, ctx->sched_bc.levels());
+  return run_levels(ctx, ctx->sched_seq, 0, ctx->n_bc_ops);
 }
 
 int do_exchange(cfx_ctx *ctx) {
-  for (size_t b = 0; b < ctx->hb.size(); ++b)
-    for (const InnerDev &f : ctx->hb[b].inner)
-      CFX_TRY(ctx->timed_launch(K_OTHER, [&] {
-        return launch_exchange(ctx->views[b], ctx->views[f.target_block], f, ctx->ns, ctx->stream);
-      }));
-  return 0;
+  if (ctx->batched) return run_levels(ctx, ctx->sched_ex, 0, ctx->sched_ex.levels());
+  return run_levels(ctx, ctx->sched_seq, ctx->n_bc_ops, ctx->sched_seq.levels());
+}
+
+// boundary conditions followed by the inner-face exchange
+int do_ghosts(cfx_ctx *ctx) {
+  if (ctx->batched) return run_levels(ctx, ctx->sched_all, 0, ctx->sched_all.levels());
+  return run_levels(ctx, ctx->sched_seq, 0, ctx->sched_seq.levels());
 }
 
 int do_elim(cfx_ctx *ctx) {
@@ -306,14 +560,17 @@ int do_elim(cfx_ctx *ctx) {
 }
 
 int do_props(cfx_ctx *ctx) {
-  for (auto &B : ctx->views)
-    CFX_TRY(ctx->timed_launch(K_PROPS, [&] { return launch_props(B, ctx->sp_dev, ctx->S, ctx->ns, ctx->stream); }));
+  auto fn = ctx->mixed ? g_kernels32[ctx->ns].props : g_kernels64[ctx->ns].props;
+  CFX_TRY(per_batch(ctx, ctx->t_pad, [&](const Batch &b) {
+    return ctx->timed_launch(K_PROPS, [&] { return fn(b, ctx->sp_dev, ctx->S, ctx->stream); });
+  }));
   return 0;
 }
 
 int do_store_last(cfx_ctx *ctx) {
-  for (auto &B : ctx->views)
-    CFX_TRY(ctx->timed_launch(K_OTHER, [&] { return launch_store_last(B, ctx->stream); }));
+  CFX_TRY(per_batch(ctx, ctx->t_last, [&](const Batch &b) {
+    return ctx->timed_launch(K_OTHER, [&] { return launch_store_last(b, ctx->stream); });
+  }));
   return 0;
 }
 
@@ -332,11 +589,39 @@ int do_step(cfx_ctx *ctx) {
       if ((rc = do_dt(ctx, 0, 1))) return rc;
     }
     if ((rc = do_update(ctx, s))) return rc;
-    if ((rc = do_bc(ctx))) return rc;
-    if ((rc = do_exchange(ctx))) return rc;
+    if ((rc = do_ghosts(ctx))) return rc;
     if ((rc = do_elim(ctx))) return rc;
     if ((rc = do_props(ctx))) return rc;
   }
+  return 0;
+}
+
+// One step captured as a graph (kernel nodes only: the dt reset is a kernel,
+// nothing is copied or set by the runtime inside the step).
+int ensure_graph(cfx_ctx *ctx) {
+  if (ctx->graph_exec) return 0;
+  CFX_TRY((int)hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = do_step(ctx);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return ctx->fail((int)e, "hipStreamEndCapture");
+  ctx->graph = g;
+  CFX_TRY((int)hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0));
+  return 0;
+}
+
+int run_steps(cfx_ctx *ctx, int n_steps) {
+  if (ctx->use_graph && !ctx->profiling) {
+    if (n_steps > 0) CFX_TRY(ensure_graph(ctx));
+    for (int s = 0; s < n_steps; ++s) CFX_TRY((int)hipGraphLaunch(ctx->graph_exec, ctx->stream));
+    return 0;
+  }
+  for (int s = 0; s < n_steps; ++s)
+    if (int rc = do_step(ctx)) return rc;
   return 0;
 }
 
@@ -359,11 +644,18 @@ int cfx_create(const cfx_params *params, const cfx_species *species, const cfx_b
   if (rc) return bail(rc);
   ctx->ns = ctx->sp_host.d.ns;
   ctx->nv = 5 + ctx->ns;
-  if (!ns_supported(ctx->ns)) return bail(CFX_E_UNSUPPORTED);
+  if (!g_kernels64[ctx->ns].present) {
+    ctx->err = "no kernels built for " + std::to_string(ctx->ns) + " species";
+    return bail(CFX_E_UNSUPPORTED);
+  }
   if (params->inviscid_scheme != 3 && params->inviscid_scheme != 0) return bail(CFX_E_UNSUPPORTED);
   if ((params->precision != CFX_FP64 && params->precision != CFX_MIXED) || params->transport != CFX_TRANSPORT_WILKE)
     return bail(CFX_E_UNSUPPORTED);
   ctx->mixed = params->precision == CFX_MIXED;
+  if (ctx->mixed && !g_kernels32[ctx->ns].present) {
+    ctx->err = "no FP32 kernels built for " + std::to_string(ctx->ns) + " species";
+    return bail(CFX_E_UNSUPPORTED);
+  }
   rc = build_mesh(blocks, n_block, ctx->hb, ctx->dim, ctx->err);
   if (rc) return bail(rc);
   SchemeDev &S = ctx->S;
@@ -477,12 +769,31 @@ int cfx_create(const cfx_params *params, const cfx_species *species, const cfx_b
     (void)hipMemset(V.props, 0, sizeof(double) * (4 + ctx->ns) * np);
     (void)hipMemset(V.dq, 0, sizeof(double) * ctx->nv * ni);
     ctx->views.push_back(V);
-    ctx->partial_off.push_back(ctx->n_partial);
-    ctx->n_partial += residual_partials(V);
     ctx->n_grid_total += H.ni;
   }
+  ctx->views_dev = ctx->dalloc<BlockView>(ctx->views.size(), rc);
+  if (rc) return bail(ctx->fail(rc, "hipMalloc block table"));
+  (void)hipMemcpy(ctx->views_dev, ctx->views.data(), sizeof(BlockView) * ctx->views.size(), hipMemcpyHostToDevice);
+  if ((rc = build_tables(ctx))) return bail(ctx->fail(rc, "work tables"));
+  if ((rc = build_ops(ctx))) return bail(ctx->fail(rc, "ghost-fill schedule"));
+  ctx->n_partial = ctx->t_res.host.back();
   ctx->d_partial = ctx->dalloc<double>(4 * (size_t)ctx->n_partial, rc);
   if (rc) return bail(ctx->fail(rc, "hipMalloc partials"));
+  {
+    int dev = 0, ncu = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+      ctx->n_cu = ncu;
+    for (int f32 = 0; f32 < 2; ++f32) {
+      if (f32 && !ctx->mixed) continue;
+      for (int d = 0; d < ctx->dim; ++d)
+        for (int v = 0; v < rhs_variants(ctx->dim); ++v) {
+          const RhsEntry &e = f32 ? (ctx->dim == 3 ? g_kernels32[ctx->ns].rhs3[d][v] : g_kernels32[ctx->ns].rhs2[d][v])
+                                  : (ctx->dim == 3 ? g_kernels64[ctx->ns].rhs3[d][v] : g_kernels64[ctx->ns].rhs2[d][v]);
+          ctx->occ[f32][d][v] = e.occupancy();
+        }
+    }
+  }
   e = hipDeviceSynchronize();
   if (e != hipSuccess) return bail(ctx->fail((int)e, "device setup"));
   *out = ctx;
@@ -492,6 +803,7 @@ int cfx_create(const cfx_params *params, const cfx_species *species, const cfx_b
 void cfx_destroy(cfx_ctx *ctx) {
   if (!ctx) return;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  ctx->drop_graph();
   for (auto &t : ctx->timed) {
     (void)hipEventDestroy(t.a);
     (void)hipEventDestroy(t.b);
@@ -563,9 +875,9 @@ int cfx_download(cfx_ctx *ctx, int blk, int what, double *dst) {
 
 int cfx_init_conservative(cfx_ctx *ctx) {
   if (!ctx) return CFX_E_ARG;
-  for (auto &B : ctx->views) CFX_TRY(launch_cv_from_bv(B, ctx->sp_dev, ctx->S, ctx->ns, ctx->stream));
-  if (int rc = do_bc(ctx)) return rc;
-  if (int rc = do_exchange(ctx)) return rc;
+  const KernelSet64 &k = g_kernels64[ctx->ns];
+  CFX_TRY(per_batch(ctx, ctx->t_pad, [&](const Batch &b) { return k.cv_from_bv(b, ctx->sp_dev, ctx->S, ctx->stream); }));
+  if (int rc = do_ghosts(ctx)) return rc;
   if (int rc = do_props(ctx)) return rc;
   return 0;
 }
@@ -598,8 +910,7 @@ int cfx_update_properties(cfx_ctx *ctx) { return ctx ? do_props(ctx) : CFX_E_ARG
 
 int cfx_residual(cfx_ctx *ctx, double res[4]) {
   if (!ctx || !res) return CFX_E_ARG;
-  for (size_t b = 0; b < ctx->views.size(); ++b)
-    CFX_TRY(launch_residual_partial(ctx->views[b], ctx->d_partial + 4 * (size_t)ctx->partial_off[b], ctx->stream));
+  CFX_TRY(per_batch(ctx, ctx->t_res, [&](const Batch &b) { return launch_residual_partial(b, ctx->d_partial, ctx->stream); }));
   CFX_TRY(launch_residual_final(ctx->d_partial, ctx->n_partial, ctx->d_res, ctx->stream));
   double h[4];
   CFX_TRY((int)hipMemcpyAsync(h, ctx->d_res, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
@@ -610,9 +921,7 @@ int cfx_residual(cfx_ctx *ctx, double res[4]) {
 
 int cfx_step(cfx_ctx *ctx, int n_steps) {
   if (!ctx || n_steps < 0) return CFX_E_ARG;
-  for (int s = 0; s < n_steps; ++s)
-    if (int rc = do_step(ctx)) return rc;
-  return 0;
+  return run_steps(ctx, n_steps);
 }
 
 int cfx_get_dt(cfx_ctx *ctx, double *dt) {
@@ -679,12 +988,93 @@ int cfx_time_steps(cfx_ctx *ctx, int n_steps, double *ms_total) {
 
 int cfx_set_rhs_variant(cfx_ctx *ctx, int variant) {
   if (!ctx) return CFX_E_ARG;
-  if (variant < 0 || variant >= n_rhs_tiles(ctx->dim)) return ctx->fail(CFX_E_ARG, "rhs variant out of range");
-  ctx->rhs_tile = variant;
+  if (variant < 0 || variant >= rhs_variants(ctx->dim)) return ctx->fail(CFX_E_ARG, "rhs variant out of range");
+  CFX_TRY((int)hipStreamSynchronize(ctx->stream));
+  for (int d = 0; d < 3; ++d) ctx->rhs_var[d] = variant, ctx->rhs_persist[d] = 0;
+  ctx->drop_graph();
   return 0;
 }
 
-int cfx_rhs_variants(cfx_ctx *ctx) { return ctx ? n_rhs_tiles(ctx->dim) : CFX_E_ARG; }
+int cfx_rhs_variants(cfx_ctx *ctx) { return ctx ? rhs_variants(ctx->dim) : CFX_E_ARG; }
+
+int cfx_set_rhs_tiles(cfx_ctx *ctx, const int variant[3], const int persist[3]) {
+  if (!ctx || !variant || !persist) return CFX_E_ARG;
+  for (int d = 0; d < ctx->dim; ++d)
+    if (variant[d] < 0 || variant[d] >= rhs_variants(ctx->dim) || persist[d] < 0 || persist[d] > 64)
+      return ctx->fail(CFX_E_ARG, "rhs tile choice out of range");
+  CFX_TRY((int)hipStreamSynchronize(ctx->stream));
+  for (int d = 0; d < ctx->dim; ++d) ctx->rhs_var[d] = variant[d], ctx->rhs_persist[d] = persist[d];
+  ctx->drop_graph();
+  return 0;
+}
+
+int cfx_get_rhs_tiles(cfx_ctx *ctx, int variant[3], int persist[3]) {
+  if (!ctx || !variant || !persist) return CFX_E_ARG;
+  for (int d = 0; d < 3; ++d) variant[d] = ctx->rhs_var[d], persist[d] = ctx->rhs_persist[d];
+  return 0;
+}
+
+// Times every tile shape and persistence factor of each face direction on
+// this context's blocks (reps launches each, after one warm-up launch) and
+// keeps the fastest.  dq holds no result afterwards; the next RHS rewrites it.
+int cfx_autotune(cfx_ctx *ctx, int reps, double *best_ms) {
+  if (!ctx || reps <= 0) return CFX_E_ARG;
+  const bool prof = ctx->profiling;
+  ctx->profiling = false;
+  hipEvent_t a, b;
+  (void)hipEventCreate(&a);
+  (void)hipEventCreate(&b);
+  static const int kPersist[] = {0, 1, 2, 4};
+  int rc = 0;
+  for (int d = 0; d < ctx->dim && !rc; ++d) {
+    double best = 1e30;
+    int bv = 0, bp = 0;
+    for (int v = 0; v < rhs_variants(ctx->dim) && !rc; ++v)
+      for (int p : kPersist) {
+        ctx->rhs_var[d] = v;
+        ctx->rhs_persist[d] = p;
+        auto run = [&] {
+          if (ctx->batched) return rhs_dir(ctx, d, ctx->all(ctx->t_rhs[d][v]), d > 0);
+          for (size_t k = 0; k < ctx->views.size(); ++k)
+            if (int r = rhs_dir(ctx, d, ctx->one(ctx->t_rhs[d][v], (int)k), d > 0)) return r;
+          return 0;
+        };
+        if ((rc = run())) break;
+        (void)hipEventRecord(a, ctx->stream);
+        for (int r = 0; r < reps && !rc; ++r) rc = run();
+        (void)hipEventRecord(b, ctx->stream);
+        (void)hipEventSynchronize(b);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, a, b);
+        if (ms / reps < best) best = ms / reps, bv = v, bp = p;
+      }
+    ctx->rhs_var[d] = bv;
+    ctx->rhs_persist[d] = bp;
+    if (best_ms) best_ms[d] = best;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  ctx->profiling = prof;
+  ctx->drop_graph();
+  if (rc) return ctx->fail(rc, "autotune launch");
+  return (int)hipStreamSynchronize(ctx->stream);
+}
+
+int cfx_set_execution(cfx_ctx *ctx, int batched, int graph) {
+  if (!ctx) return CFX_E_ARG;
+  CFX_TRY((int)hipStreamSynchronize(ctx->stream));
+  ctx->batched = batched != 0;
+  ctx->use_graph = graph != 0;
+  ctx->drop_graph();
+  return 0;
+}
+
+int cfx_schedule_info(cfx_ctx *ctx, int *n_ops, int *n_levels) {
+  if (!ctx) return CFX_E_ARG;
+  if (n_ops) *n_ops = (int)ctx->ops.size();
+  if (n_levels) *n_levels = ctx->sched_all.levels();
+  return 0;
+}
 
 int cfx_kernel_times(cfx_ctx *ctx, int cls, double *ms, long long *launches) {
   if (!ctx || cls < 0 || cls >= K_NCLASS) return CFX_E_ARG;

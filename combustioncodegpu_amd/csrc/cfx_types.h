@@ -112,4 +112,31 @@ struct InnerDev {
   int n_point[3];
 };
 
+// One ghost-fill operation over a box of points: a boundary condition on a
+// face of block blk (kind = CFX_BC_*), or an inner-face exchange from
+// target block tgt into blk (kind = kOpExchange).  Point n of the op is
+// (n % ext[0], n / ext[0] % ext[1], n / (ext[0] ext[1])) relative to the
+// range start.
+constexpr int kOpExchange = 100;
+struct FaceOp {
+  int kind, blk, tgt, cond;
+  int ext[3];
+  long long npts;
+  FaceRange r;  // boundary range (wall: shrunk by ngg tangentially)
+  InnerDev f;   // exchange mapping
+  WallDev wl;
+};
+
+// Work segments: workgroup w (counted from off[0]) belongs to segment s with
+// off[s] <= w < off[s+1], s in [0, n).  The search is uniform per workgroup.
+__device__ __forceinline__ int find_seg(const int *__restrict__ off, int n, int w) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= w) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 }  // namespace cfx

@@ -188,127 +188,4 @@ __device__ __forceinline__ void ausm_from_states(SpeciesPtrT<R> sp, const Scheme
   for (int l = 5; l < NV; ++l) F[l] = coeff * (left ? pl[l] : pr[l]);
 }
 
-// ------------------------------------------------------- inviscid flux ----
-// AUSM+ flux of the face between padded points p and p+s in direction d
-// (InviscidScheme.cu:220-304) with the face states from MUSCL
-// (Reconstruction.cu:79-122).  F has 5+NS entries.
-template <int NS, typename R>
-__device__ __forceinline__ void inviscid_flux(const BlockView &B, SpeciesPtrT<R> sp,
-                                              const SchemeDev &S, int d, long long p, long long s, R *F) {
-  constexpr int NV = 5 + NS;
-  const long long np = B.np;
-  R pl[NV], pr[NV];
-#pragma unroll
-  for (int l = 0; l < NV; ++l) {
-    const double *q = l < 5 ? B.bv + l * np : B.yk + (l - 5) * np;
-    const R q0 = R(q[p - s]), q1 = R(q[p]), q2 = R(q[p + s]), q3 = R(q[p + 2 * s]);
-    muscl<R>(S, q0, q1, q2, q3, pl[l], pr[l]);
-  }
-  const double *M = B.metric + 3 * d * np;
-  const R jl = R(B.jac[p]), jr = R(B.jac[p + s]);
-  const R k1 = R(0.5) * (jl * R(M[p]) + jr * R(M[p + s]));
-  const R k2 = R(0.5) * (jl * R(M[np + p]) + jr * R(M[np + p + s]));
-  const R k3 = R(0.5) * (jl * R(M[2 * np + p]) + jr * R(M[2 * np + p + s]));
-  ausm_from_states<NS, R>(sp, S, pl, pr, k1, k2, k3, F);
-}
-
-// -------------------------------------------------------- viscous flux ----
-// Central 2nd-order viscous flux of the face between p and p+s_d
-// (SecOrdViscScheme::compute_fv/gv/hv, ViscousScheme.cu:11-342), generic in
-// the face direction d.  Quirk bits reproduce the reference's face-metric
-// averaging (fv, hv) and the (i,j+1,k) rho*D neighbour of hv.
-template <int NS, typename R>
-__device__ __forceinline__ void viscous_flux(const BlockView &B, SpeciesPtrT<R> sp,
-                                             const SchemeDev &S, int d, long long p, const long long st[3],
-                                             R *G) {
-  const long long np = B.np;
-  const long long pd = p + st[d];
-  const double *M = B.metric;
-  R mf[9];
-  const bool q5 = (S.quirks & CFX_Q_METRIC_AVG) && d != 1;
-#pragma unroll
-  for (int rc = 0; rc < 9; ++rc) {
-    const R a = R(M[rc * np + p]);
-    mf[rc] = (q5 && rc != 0) ? a : R(0.5) * (a + R(M[rc * np + pd]));
-  }
-  // derivatives along (xi, eta, zeta): normal one-sided, tangential 4-point
-  auto grad = [&](const double *q, R g[3]) {
-    const R a = R(q[p]), b = R(q[pd]);
-#pragma unroll
-    for (int t = 0; t < 3; ++t) {
-      if (t == d) {
-        g[t] = b - a;
-      } else {
-        const long long o = st[t];
-        g[t] = R(0.25) * (R(q[p + o]) - R(q[p - o]) + R(q[pd + o]) - R(q[pd - o]));
-      }
-    }
-  };
-  auto chain = [&](const R g[3], R out[3]) {
-    out[0] = g[0] * mf[0] + g[1] * mf[3] + g[2] * mf[6];
-    out[1] = g[0] * mf[1] + g[1] * mf[4] + g[2] * mf[7];
-    out[2] = g[0] * mf[2] + g[1] * mf[5] + g[2] * mf[8];
-  };
-  R g[3], du[3], dv[3], dw[3], dT[3];
-  grad(B.bv + np, g);
-  chain(g, du);
-  grad(B.bv + 2 * np, g);
-  chain(g, dv);
-  grad(B.bv + 3 * np, g);
-  chain(g, dw);
-  grad(B.bv + 5 * np, g);
-  chain(g, dT);
-  const double *mu = B.props + 2 * np;
-  const double *kap = B.props + 3 * np;
-  const R vis = R(0.5) * (R(mu[p]) + R(mu[pd]));
-  const R third = R(1.0 / 3.0);
-  const R txx = vis * (R(4) * du[0] - R(2) * dv[1] - R(2) * dw[2]) * third;
-  const R tyy = vis * (R(4) * dv[1] - R(2) * du[0] - R(2) * dw[2]) * third;
-  const R tzz = vis * (R(4) * dw[2] - R(2) * du[0] - R(2) * dv[1]) * third;
-  const R txy = vis * (du[1] + dv[0]);
-  const R txz = vis * (du[2] + dw[0]);
-  const R tyz = vis * (dv[2] + dw[1]);
-  const double *Md = M + 3 * d * np;
-  const R j0 = R(B.jac[p]), j1 = R(B.jac[pd]);
-  const R S0 = R(0.5) * (R(Md[p]) * j0 + R(Md[pd]) * j1);
-  const R S1 = R(0.5) * (R(Md[np + p]) * j0 + R(Md[np + pd]) * j1);
-  const R S2 = R(0.5) * (R(Md[2 * np + p]) * j0 + R(Md[2 * np + pd]) * j1);
-  G[0] = R(0);
-  G[1] = S0 * txx + S1 * txy + S2 * txz;
-  G[2] = S0 * txy + S1 * tyy + S2 * tyz;
-  G[3] = S0 * txz + S1 * tyz + S2 * tzz;
-  const R um = R(0.5) * (R(B.bv[np + p]) + R(B.bv[np + pd]));
-  const R vm = R(0.5) * (R(B.bv[2 * np + p]) + R(B.bv[2 * np + pd]));
-  const R wm = R(0.5) * (R(B.bv[3 * np + p]) + R(B.bv[3 * np + pd]));
-  const R cond = R(0.5) * (R(kap[p]) + R(kap[pd]));
-  G[4] = um * G[1] + vm * G[2] + wm * G[3] + cond * (S0 * dT[0] + S1 * dT[1] + S2 * dT[2]);
-  if constexpr (NS > 0) {
-    const long long pD = (d == 2 && (S.quirks & CFX_Q_HV_RHOD_J)) ? p + st[1] : pd;
-    const R tm = R(0.5) * (R(B.bv[5 * np + p]) + R(B.bv[5 * np + pd]));
-    const double *rd = B.props + 4 * np;
-    R proj[NS], dif[NS];
-    R ucx = R(0), ucy = R(0), ucz = R(0);
-#pragma unroll
-    for (int l = 0; l < NS; ++l) {
-      R h, cp;
-      species_h_cp<R>(sp, l, tm, h, cp);
-      dif[l] = R(0.5) * (R(rd[l * np + p]) + R(rd[l * np + pD]));
-      R gy[3], dy[3];
-      grad(B.yk + l * np, gy);
-      chain(gy, dy);
-      proj[l] = S0 * dy[0] + S1 * dy[1] + S2 * dy[2];
-      G[4] += dif[l] * h * proj[l];
-      ucx += dif[l] * dy[0];
-      ucy += dif[l] * dy[1];
-      ucz += dif[l] * dy[2];
-    }
-    const R uc = S0 * ucx + S1 * ucy + S2 * ucz;
-#pragma unroll
-    for (int l = 0; l < NS; ++l) {
-      const double *Y = B.yk + l * np;
-      G[5 + l] = dif[l] * proj[l] - R(0.5) * (R(Y[p]) + R(Y[pd])) * uc;
-    }
-  }
-}
-
 }  // namespace cfx

@@ -1,5 +1,11 @@
-// kernels.h — host launchers of the step kernels (kernels.hip).  All return 0
-// or a hipError_t / CFX_E_* code; all are asynchronous on the given stream.
+// kernels.h — host launchers of the step kernels.  Every kernel works on a
+// batch of blocks: a device table of BlockViews and a prefix table off[] of
+// workgroups per block, so one launch covers any subset of the blocks (a
+// single block is a batch of one).  The species-count templated launchers
+// are compiled once per supported species count and precision
+// (kernels_ns.hip) and register themselves in g_kernels64 / g_kernels32.
+// All launchers return 0 or a hipError_t / CFX_E_* code and are asynchronous
+// on the given stream.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -8,25 +14,88 @@
 
 namespace cfx {
 
-bool ns_supported(int ns);
-int rhs_tile(int dim, int tile, int &tx, int &ty, int &tz);
-int n_rhs_tiles(int dim);
+// ------------------------------------------------------------ work units ---
+constexpr int kTpb = 256;        // threads per workgroup of the point kernels
+constexpr int kDtIter = 8;       // points per thread of the time-step reduction
+constexpr int kLastIter = 4;     // points per thread of the last-step snapshot
+constexpr int kResIter = 8;      // points per thread of the residual partials
 
-int launch_props(const BlockView &B, const SpeciesDev *sp, const SchemeDev &S, int ns, hipStream_t st);
-int launch_rhs(const BlockView &B, const SpeciesDev *sp, const SchemeDev &S, int ns, int tile, hipStream_t st);
+// A batch of blocks: views[0..nb) with workgroup prefix off[0..nb] (device
+// pointers, off[0] need not be 0) and total = off[nb] - off[0] workgroups.
+struct Batch {
+  const BlockView *views;
+  const int *off;
+  int nb;
+  int total;
+};
+
+// --------------------------------------------------------- RHS tile shapes ---
+// Per face direction, the pencil tiles (faces along the direction x two
+// tangential extents) and the tile-group width G that the autotuner chooses
+// among.  Index v of each list is "variant v" of that direction.
+struct PencilTile {
+  int ta, tb, tc, g;
+};
+constexpr int kNVar3 = 4, kNVar2 = 2;
+constexpr PencilTile kTiles3[3][kNVar3] = {
+    {{32, 4, 2, 1}, {32, 4, 4, 1}, {64, 2, 2, 1}, {32, 8, 1, 1}},
+    {{16, 8, 2, 1}, {16, 4, 4, 1}, {32, 4, 2, 1}, {16, 16, 1, 1}},
+    {{16, 8, 2, 1}, {16, 4, 4, 1}, {32, 4, 2, 1}, {16, 16, 1, 1}}};
+constexpr PencilTile kTiles2[2][kNVar2] = {{{32, 8, 1, 1}, {64, 4, 1, 1}}, {{16, 16, 1, 1}, {32, 8, 1, 1}}};
+
+inline int rhs_variants(int dim) { return dim == 3 ? kNVar3 : kNVar2; }
+inline const PencilTile &rhs_tile_shape(int dim, int dir, int v) { return dim == 3 ? kTiles3[dir][v] : kTiles2[dir][v]; }
+
+// Tiles one pencil launch of shape s covers on a block of m[3] points
+// (including the padding tiles of partial G x G groups).
+inline int pencil_tiles(const int m[3], int dir, const PencilTile &s) {
+  const int a = dir, b = dir == 0 ? 1 : 0, c = dir == 2 ? 1 : 2;
+  const int nta = (m[a] + s.ta - 2) / (s.ta - 1);
+  const int ntb = (m[b] + s.tb - 1) / s.tb, ntc = (m[c] + s.tc - 1) / s.tc;
+  return nta * s.g * s.g * ((ntb + s.g - 1) / s.g) * ((ntc + s.g - 1) / s.g);
+}
+
+// grid: workgroups launched (a multiple of 8, at most the padded tile count)
+using RhsFn = int (*)(const Batch &b, int grid, const SpeciesDev *sp, const SchemeDev &S, int accumulate,
+                      hipStream_t st);
+using OccFn = int (*)();  // resident workgroups per CU of the kernel
+struct RhsEntry {
+  RhsFn launch;
+  OccFn occupancy;
+};
+
+struct KernelSet64 {
+  int present;
+  int (*props)(const Batch &b, const SpeciesDev *sp, const SchemeDev &S, hipStream_t st);
+  int (*cv_from_bv)(const Batch &b, const SpeciesDev *sp, const SchemeDev &S, hipStream_t st);
+  int (*update)(const Batch &b, const SpeciesDev *sp, const SchemeDev &S, int stage, const double *dt, int *flags,
+                unsigned long long *newton_fail, hipStream_t st);
+  // ops[0..nops) with workgroup prefix off[0..nops], total workgroups
+  int (*face_ops)(const BlockView *views, const FaceOp *ops, const int *off, int nops, int total,
+                  const SpeciesDev *sp, const SchemeDev &S, const InflowDev *inflow, hipStream_t st);
+  RhsEntry rhs3[3][kNVar3];
+  RhsEntry rhs2[2][kNVar2];
+};
+
+struct KernelSet32 {
+  int present;
+  int (*props)(const Batch &b, const SpeciesDev *sp, const SchemeDev &S, hipStream_t st);
+  RhsEntry rhs3[3][kNVar3];
+  RhsEntry rhs2[2][kNVar2];
+};
+
+extern KernelSet64 g_kernels64[kMaxSp + 1];
+extern KernelSet32 g_kernels32[kMaxSp + 1];
+
+// ------------------------------------------------ species-independent ones ---
 int launch_dt_reset(unsigned long long *dt_bits, hipStream_t st);
-int launch_dt(const BlockView &B, const SchemeDev &S, int ns, int store_local, unsigned long long *dt_bits,
+int launch_dt(const Batch &b, const SchemeDev &S, int ns, int store_local, unsigned long long *dt_bits,
               hipStream_t st);
-int launch_update(const BlockView &B, const SpeciesDev *sp, const SchemeDev &S, int ns, int stage, const double *dt,
-                  int *flags, unsigned long long *newton_fail, hipStream_t st);
-int launch_cv_from_bv(const BlockView &B, const SpeciesDev *sp, const SchemeDev &S, int ns, hipStream_t st);
-int launch_store_last(const BlockView &B, hipStream_t st);
-int residual_partials(const BlockView &B);
-int launch_residual_partial(const BlockView &B, double *partial, hipStream_t st);
+int launch_store_last(const Batch &b, hipStream_t st);
+// partial sums of workgroup w go to partial[4 * (w)] with w counted from 0 of
+// the full prefix, so a per-block launch and a batched one fill the same slots
+int launch_residual_partial(const Batch &b, double *partial, hipStream_t st);
 int launch_residual_final(const double *partial, int n, double *out, hipStream_t st);
-int launch_bc(int type, const BlockView &B, const FaceRange &f, const SpeciesDev *sp, const SchemeDev &S, int ns,
-              const InflowDev *inflow, const WallDev &wall, hipStream_t st);
-int launch_exchange(const BlockView &B, const BlockView &T, const InnerDev &f, int ns, hipStream_t st);
 int launch_eliminate_k(const BlockView &B, int ns, hipStream_t st);
 
 }  // namespace cfx

@@ -1,0 +1,604 @@
+// kernels_ns.hip — the kernels templated on the species count NS, compiled
+// once per supported count (-DCFX_NS=n) and precision of the flux work
+// (-DCFX_PREC=64: the FP64 set with the update, conversions and ghost fills;
+// -DCFX_PREC=32: the FP32 properties and RHS of the mixed-precision path).
+// Each build registers its launchers in g_kernels64[n] / g_kernels32[n].
+#include <hip/hip_runtime.h>
+
+#include "device_physics.h"
+#include "kernels.h"
+#include "rhs_pencil.h"
+
+#ifndef CFX_NS
+#error "CFX_NS (species count) must be defined"
+#endif
+#ifndef CFX_PREC
+#error "CFX_PREC (64 or 32) must be defined"
+#endif
+
+namespace cfx {
+
+namespace {
+
+constexpr int NS = CFX_NS;
+constexpr int NSA = NS > 0 ? NS : 1;  // extent of the per-species arrays
+static_assert(NS >= 0 && NS <= kMaxSp, "species count out of range");
+
+constexpr PencilTile tile_of(bool d3, int dir, int v) {
+  return d3 ? kTiles3[dir][v] : kTiles2[dir < 2 ? dir : 0][v < kNVar2 ? v : 0];
+}
+
+__device__ __forceinline__ long long pidx(const BlockView &B, int i, int j, int k) {
+  return (long long)(k + kNgg) * B.sz + (long long)(j + kNgg) * B.sy + (i + kNgg);
+}
+
+__device__ __forceinline__ long long interior_pidx(const BlockView &B, long long q) {
+  const unsigned u = (unsigned)q, mx = (unsigned)B.mx, my = (unsigned)B.my;
+  const unsigned i = u % mx, r = u / mx;
+  return pidx(B, (int)i, (int)(r % my), (int)(r / my));
+}
+
+// Point kernels: workgroup -> (block, first point of its kTpb chunk).
+struct PointWork {
+  int blk;
+  long long base;
+};
+__device__ __forceinline__ PointWork point_work(const int *__restrict__ off, int nb) {
+  const int w = blockIdx.x + off[0];
+  const int blk = find_seg(off, nb, w);
+  return {blk, (long long)(w - off[blk]) * kTpb};
+}
+
+// ------------------------------------------------------------ properties ---
+// update_physical_properties (Field.cu:270-302) and
+// compute_transport_property (Transport.cu:206-259) over the padded range.
+// The Wilke sums use s_l = sqrt(mu_l) so that sqrt(mu_m/mu_n) = s_m/s_n
+// (NS square roots instead of NS^2) and T*^-0.145 = exp(-0.145 log T*) with
+// one shared log T.  In R = float (mixed precision) the kernel also writes
+// the FP32 copies of the primitives the FP32 RHS reads.
+template <typename R>
+__global__ void __launch_bounds__(kTpb) props_kernel(const BlockView *__restrict__ views, const int *__restrict__ off,
+                                                     int nb, const SpeciesDev *__restrict__ spg, SchemeDev S) {
+  const SpeciesPtrT<R> sp = const_table<R>(spg);
+  const PointWork pw = point_work(off, nb);
+  const BlockView B = views[pw.blk];
+  const long long np = B.np;
+  constexpr bool f32 = sizeof(R) == 4;
+  const long long p = pw.base + threadIdx.x;
+  if (p >= np) return;
+  const R t = R(B.bv[5 * np + p]);
+  R out[4 + (NS > 0 ? NS : 0)];
+  if constexpr (NS == 0) {
+    const R c_temp = R(S.gamma_air * S.r_air);
+    out[0] = R(S.gamma_air);
+    out[1] = sqrt(c_temp * t);
+    const R mu = R(1.716e-5) * pow(t / R(273), R(1.5)) * R(273 + 111) / (t + R(111));
+    out[2] = mu;
+    out[3] = mu * c_temp / R(S.gamma_air - 1) * R(S.inv_pr);
+  } else {
+    R Y[NSA], cp[NSA];
+    R winv = 0, cpt = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      Y[l] = R(B.yk[l * np + p]);
+      cp[l] = species_cp<R>(sp, l, t);
+      winv += Y[l] * R(sp->inv_mw[l]);
+      cpt += Y[l] * cp[l];
+    }
+    const R Ru = R(kRu);
+    const R gam = cpt / (cpt - Ru * winv);
+    out[0] = gam;
+    out[1] = sqrt(gam * Ru * t * winv);
+    // Chapman-Enskog species viscosity (Transport.cu:215-220)
+    const R lt = log(t), st = sqrt(t);
+    R mu_s[NSA], X[NSA], s[NSA], rs[NSA];
+    const R imw = R(1) / winv;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      X[l] = Y[l] * imw * R(sp->inv_mw[l]);
+      const R tdl = t * R(sp->lj_inv[l]);
+      const R a = tdl + R(0.5);
+      const R omega = R(1.147) * exp(R(-0.145) * (lt + R(sp->log_lj_inv[l]))) + R(1) / (a * a);
+      mu_s[l] = R(sp->vis_coeff[l]) * st / omega;
+      s[l] = sqrt(mu_s[l]);
+      rs[l] = R(1) / s[l];
+    }
+    // Wilke mixing and Mason-Saxena conductivity (Transport.cu:221-246)
+    R mu = 0, kappa = 0;
+#pragma unroll
+    for (int m = 0; m < NS; ++m) {
+      R vt = X[m];
+#pragma unroll
+      for (int n = 0; n < NS; ++n) {
+        if (n == m) continue;
+        const R q = R(1) + s[m] * rs[n] * R(sp->wj4[m][n]);
+        vt += q * q * R(sp->sqw8[m][n]) * X[n];
+      }
+      const R lam = mu_s[m] * (cp[m] + R(sp->cond_add[m]));
+      mu += mu_s[m] * X[m] / vt;
+      kappa += lam * X[m] / (R(1.065) * vt - R(0.065) * X[m]);
+    }
+    out[2] = mu;
+    out[3] = kappa;
+    // constant-Schmidt diffusivity (Transport.cu:249-255, X[l] in the test)
+    const R mus = mu * R(S.inv_sc);
+#pragma unroll
+    for (int l = 0; l < NS; ++l)
+      out[4 + l] = fabs(X[l] - R(1)) < R(1e-3) ? mus : (R(1) - Y[l]) * mu / ((R(1) - X[l]) * R(S.sc));
+  }
+  if constexpr (f32) {
+#pragma unroll
+    for (int v = 0; v < 4 + NS; ++v) B.props32[v * np + p] = out[v];
+#pragma unroll
+    for (int v = 0; v < 6; ++v) B.bv32[v * np + p] = float(B.bv[v * np + p]);
+#pragma unroll
+    for (int l = 0; l < NS; ++l) B.yk32[l * np + p] = float(B.yk[l * np + p]);
+  } else {
+#pragma unroll
+    for (int v = 0; v < 4 + NS; ++v) B.props[v * np + p] = out[v];
+  }
+}
+
+template <typename R>
+int launch_props(const Batch &b, const SpeciesDev *sp, const SchemeDev &S, hipStream_t st) {
+  if (b.total <= 0) return 0;
+  props_kernel<R><<<b.total, kTpb, 0, st>>>(b.views, b.off, b.nb, sp, S);
+  return (int)hipGetLastError();
+}
+
+// ------------------------------------------------------------------- RHS ---
+// FP64 pencils at 2 waves per SIMD; FP32 needs half the registers and
+// targets 3.
+template <int DIR, int V, typename R, bool D3>
+int launch_rhs_dir(const Batch &b, int grid, const SpeciesDev *sp, const SchemeDev &S, int accumulate,
+                   hipStream_t st) {
+  constexpr PencilTile s = tile_of(D3, DIR, V);
+  constexpr int WPE = sizeof(R) == 8 ? 2 : 3;
+  if (b.total <= 0) return 0;
+  const int padded = (b.total + 7) / 8 * 8;
+  rhs_pencil_kernel<NS, DIR, s.ta, s.tb, s.tc, WPE, R, s.g>
+      <<<grid, s.ta * s.tb * s.tc, 0, st>>>(b.views, b.off, b.nb, padded, sp, S, accumulate);
+  return (int)hipGetLastError();
+}
+
+template <int DIR, int V, typename R, bool D3>
+int occupancy_rhs_dir() {
+  constexpr PencilTile s = tile_of(D3, DIR, V);
+  constexpr int WPE = sizeof(R) == 8 ? 2 : 3;
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, rhs_pencil_kernel<NS, DIR, s.ta, s.tb, s.tc, WPE, R, s.g>,
+                                                   s.ta * s.tb * s.tc, 0) != hipSuccess)
+    return 0;
+  return n;
+}
+
+template <typename R, typename Set>
+void register_rhs(Set &k) {
+  k.rhs3[0][0] = {launch_rhs_dir<0, 0, R, true>, occupancy_rhs_dir<0, 0, R, true>};
+  k.rhs3[0][1] = {launch_rhs_dir<0, 1, R, true>, occupancy_rhs_dir<0, 1, R, true>};
+  k.rhs3[0][2] = {launch_rhs_dir<0, 2, R, true>, occupancy_rhs_dir<0, 2, R, true>};
+  k.rhs3[0][3] = {launch_rhs_dir<0, 3, R, true>, occupancy_rhs_dir<0, 3, R, true>};
+  k.rhs3[1][0] = {launch_rhs_dir<1, 0, R, true>, occupancy_rhs_dir<1, 0, R, true>};
+  k.rhs3[1][1] = {launch_rhs_dir<1, 1, R, true>, occupancy_rhs_dir<1, 1, R, true>};
+  k.rhs3[1][2] = {launch_rhs_dir<1, 2, R, true>, occupancy_rhs_dir<1, 2, R, true>};
+  k.rhs3[1][3] = {launch_rhs_dir<1, 3, R, true>, occupancy_rhs_dir<1, 3, R, true>};
+  k.rhs3[2][0] = {launch_rhs_dir<2, 0, R, true>, occupancy_rhs_dir<2, 0, R, true>};
+  k.rhs3[2][1] = {launch_rhs_dir<2, 1, R, true>, occupancy_rhs_dir<2, 1, R, true>};
+  k.rhs3[2][2] = {launch_rhs_dir<2, 2, R, true>, occupancy_rhs_dir<2, 2, R, true>};
+  k.rhs3[2][3] = {launch_rhs_dir<2, 3, R, true>, occupancy_rhs_dir<2, 3, R, true>};
+  k.rhs2[0][0] = {launch_rhs_dir<0, 0, R, false>, occupancy_rhs_dir<0, 0, R, false>};
+  k.rhs2[0][1] = {launch_rhs_dir<0, 1, R, false>, occupancy_rhs_dir<0, 1, R, false>};
+  k.rhs2[1][0] = {launch_rhs_dir<1, 0, R, false>, occupancy_rhs_dir<1, 0, R, false>};
+  k.rhs2[1][1] = {launch_rhs_dir<1, 1, R, false>, occupancy_rhs_dir<1, 1, R, false>};
+}
+
+#if CFX_PREC == 64
+// ---------------------------------------------------------- total energy ---
+// compute_total_energy (Thermo.cu:85-105) from bv/yk/cv at padded point p.
+__device__ __forceinline__ void total_energy(const BlockView &B, SpeciesPtrT<double> sp, const SchemeDev &S,
+                                             long long p) {
+  const long long np = B.np;
+  const double u = B.bv[np + p], v = B.bv[2 * np + p], w = B.bv[3 * np + p];
+  double e = 0.5 * B.bv[p] * (u * u + v * v + w * w);
+  if constexpr (NS > 0) {
+    const double t = B.bv[5 * np + p];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      double h, cp;
+      species_h_cp<double>(sp, l, t, h, cp);
+      e += h * B.cv[(5 + l) * np + p];
+    }
+    e -= B.bv[4 * np + p];
+  } else {
+    e += B.bv[4 * np + p] / (S.gamma_air - 1);
+  }
+  B.cv[4 * np + p] = e;
+}
+
+// --------------------------------------------------------------- update ---
+// update_cv_and_bv (TimeAdvanceFunc.cu:301-336) with the RK stage
+// combination q = a*qn + b*(q + dt*dq/J) and the Newton inversion of T
+// (Thermo.cu:108-144, stop |dT| <= 1e-10 K unless CFX_Q_NEWTON_REL).
+__global__ void __launch_bounds__(kTpb) update_kernel(const BlockView *__restrict__ views, const int *__restrict__ off,
+                                                      int nb, const SpeciesDev *__restrict__ spg, SchemeDev S,
+                                                      int stage, const double *__restrict__ dt_dev, int *flags,
+                                                      unsigned long long *newton_fail) {
+  const SpeciesPtrT<double> sp = const_table<double>(spg);
+  constexpr int NV = 5 + NS;
+  const PointWork pw = point_work(off, nb);
+  const BlockView B = views[pw.blk];
+  const long long np = B.np, ni = B.ni;
+  const long long q = pw.base + threadIdx.x;
+  if (q >= ni) return;
+  const double A = stage == 1 ? 0.75 : (stage == 2 ? 1.0 / 3.0 : 0.0);
+  const double Bc = stage == 1 ? 0.25 : (stage == 2 ? 2.0 / 3.0 : 1.0);
+  const double dt = *dt_dev;
+  const bool rel = (S.quirks & CFX_Q_NEWTON_REL) != 0;
+  const long long p = interior_pidx(B, q);
+  const double dtj = (S.rk == 3 ? dt : B.dt_local[q]) / B.jac[p];
+  double c[NV];
+#pragma unroll
+  for (int l = 0; l < NV; ++l) {
+    const double cur = B.cv[l * np + p];
+    const double dq = B.mixed ? (double)B.dq32[l * ni + q] : B.dq[l * ni + q];
+    if (S.rk == 3) {
+      double qn;
+      if (stage == 0) {
+        qn = cur;
+        B.qn[l * ni + q] = cur;
+      } else {
+        qn = B.qn[l * ni + q];
+      }
+      c[l] = A * qn + Bc * (cur + dq * dtj);
+    } else {
+      c[l] = cur + dq * dtj;
+    }
+  }
+  if (B.mz == 1) c[3] = 0.0;
+#pragma unroll
+  for (int l = 0; l < NV; ++l) B.cv[l * np + p] = c[l];
+  const double rho = c[0];
+  const double ir = 1.0 / rho;
+  const double u = c[1] * ir, v = c[2] * ir, w = c[3] * ir;
+  const double v2 = u * u + v * v + w * w;
+  double T, pr;
+  if constexpr (NS > 0) {
+    double Y[NSA];
+    double winv = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      Y[l] = c[5 + l] * ir;
+      B.yk[l * np + p] = Y[l];
+      winv += Y[l] * sp->inv_mw[l];
+    }
+    const double R = kRu * winv;
+    const double e = c[4] / rho - 0.5 * v2;
+    T = B.bv[5 * np + p];
+    double err = 1e30;
+    int it = 0;
+    const double eps = rel ? 1e-3 : 1e-10;
+    while (err > eps && it < 1000) {
+      ++it;
+      double h = 0, cp = 0;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        double hl, cpl;
+        species_h_cp<double>(sp, l, T, hl, cpl);
+        h += hl * Y[l];
+        cp += cpl * Y[l];
+      }
+      const double T1 = T - (h - R * T - e) / (cp - R);
+      err = rel ? fabs(1 - T1 / T) : fabs(T1 - T);
+      T = T1;
+    }
+    if (err > eps) atomicAdd(newton_fail, 1ull);
+    pr = rho * T * R;
+  } else {
+    pr = (S.gamma_air - 1) * (c[4] - 0.5 * rho * v2);
+    T = pr * ir / S.r_air;
+  }
+  B.bv[p] = rho;
+  B.bv[np + p] = u;
+  B.bv[2 * np + p] = v;
+  B.bv[3 * np + p] = w;
+  B.bv[4 * np + p] = pr;
+  B.bv[5 * np + p] = T;
+  if (!(rho > 0.0) || !(T > 0.0) || !(pr > 0.0) || !isfinite(rho) || !isfinite(T)) atomicOr(flags, 1);
+}
+
+// compute_cv_from_bv (Field.cu:242-268) over the padded range
+__global__ void __launch_bounds__(kTpb) cv_from_bv_kernel(const BlockView *__restrict__ views,
+                                                          const int *__restrict__ off, int nb,
+                                                          const SpeciesDev *__restrict__ spg, SchemeDev S) {
+  const SpeciesPtrT<double> sp = const_table<double>(spg);
+  const PointWork pw = point_work(off, nb);
+  const BlockView B = views[pw.blk];
+  const long long np = B.np;
+  const long long p = pw.base + threadIdx.x;
+  if (p >= np) return;
+  const double rho = B.bv[p];
+  B.cv[p] = rho;
+  B.cv[np + p] = rho * B.bv[np + p];
+  B.cv[2 * np + p] = rho * B.bv[2 * np + p];
+  B.cv[3 * np + p] = rho * B.bv[3 * np + p];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) B.cv[(5 + l) * np + p] = rho * B.yk[l * np + p];
+  total_energy(B, sp, S, p);
+}
+
+// ------------------------------------------------------------ ghost fill ---
+// apply_outflow (BoundCond.cu:455-487) at face point (i, j, k)
+__device__ __forceinline__ void bc_outflow_point(const BlockView &B, const FaceRange &f, int i, int j, int k) {
+  int dir[3] = {0, 0, 0};
+  dir[f.face] = f.direction;
+  const long long np = B.np;
+  const long long p = pidx(B, i, j, k);
+  for (int g = 1; g <= kNgg; ++g) {
+    const long long q = pidx(B, i + g * dir[0], j + g * dir[1], k + g * dir[2]);
+#pragma unroll
+    for (int l = 0; l < 6; ++l) B.bv[l * np + q] = B.bv[l * np + p];
+#pragma unroll
+    for (int l = 0; l < 5 + NS; ++l) B.cv[l * np + q] = B.cv[l * np + p];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) B.yk[l * np + q] = B.yk[l * np + p];
+  }
+}
+
+// apply_inflow (BoundCond.cu:489-527); the state's total energy is
+// precomputed on the host with the same formula as compute_total_energy.
+__device__ __forceinline__ void bc_inflow_point(const BlockView &B, const FaceRange &f, int i, int j, int k,
+                                                const InflowDev *__restrict__ in) {
+  int dir[3] = {0, 0, 0};
+  dir[f.face] = f.direction;
+  const long long np = B.np;
+  const double rho = in->st[0];
+  for (int g = 1; g <= kNgg; ++g) {
+    const long long q = pidx(B, i + g * dir[0], j + g * dir[1], k + g * dir[2]);
+#pragma unroll
+    for (int l = 0; l < 6; ++l) B.bv[l * np + q] = in->st[l];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      B.yk[l * np + q] = in->y[l];
+      B.cv[(5 + l) * np + q] = rho * in->y[l];
+    }
+    B.cv[q] = rho;
+    B.cv[np + q] = rho * in->st[1];
+    B.cv[2 * np + q] = rho * in->st[2];
+    B.cv[3 * np + q] = rho * in->st[3];
+    B.cv[4 * np + q] = in->st[6];
+  }
+}
+
+__device__ __forceinline__ double mix_mw(const BlockView &B, SpeciesPtrT<double> sp, const SchemeDev &S,
+                                         long long p) {
+  if constexpr (NS == 0) {
+    return kRu / S.r_air;
+  } else {
+    double w = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) w += B.yk[l * B.np + p] * sp->inv_mw[l];
+    return 1.0 / w;
+  }
+}
+
+// apply_wall (BoundCond.cu:529-629) over the face's own vertex range (the
+// registered range shrunk back by ngg, BoundCond.cu:229-239).
+__device__ __forceinline__ void bc_wall_point(const BlockView &B, const FaceRange &f, int i, int j, int k,
+                                              SpeciesPtrT<double> sp, const SchemeDev &S, const WallDev &wl) {
+  int dir[3] = {0, 0, 0};
+  dir[f.face] = f.direction;
+  const long long np = B.np;
+  const long long p = pidx(B, i, j, k);
+  const long long pin = pidx(B, i - dir[0], j - dir[1], k - dir[2]);
+  double t_wall = wl.temperature;
+  if (wl.adiabatic) t_wall = B.bv[5 * np + pin];
+  const double pw = B.bv[4 * np + pin];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) B.yk[l * np + p] = B.yk[l * np + pin];
+  double mw = mix_mw(B, sp, S, p);
+  B.bv[p] = pw * mw / (t_wall * kRu);
+  B.bv[np + p] = 0;
+  B.bv[2 * np + p] = 0;
+  B.bv[3 * np + p] = 0;
+  B.bv[4 * np + p] = pw;
+  B.bv[5 * np + p] = t_wall;
+  B.cv[p] = B.bv[p];
+  B.cv[np + p] = 0;
+  B.cv[2 * np + p] = 0;
+  B.cv[3 * np + p] = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) B.cv[(l + 5) * np + p] = B.yk[l * np + p] * B.bv[p];
+  total_energy(B, sp, S, p);
+  for (int g = 1; g <= kNgg; ++g) {
+    const long long qi = pidx(B, i - g * dir[0], j - g * dir[1], k - g * dir[2]);
+    const long long qg = pidx(B, i + g * dir[0], j + g * dir[1], k + g * dir[2]);
+#pragma unroll
+    for (int l = 0; l < NS; ++l) B.yk[l * np + qg] = B.yk[l * np + qi];
+    mw = mix_mw(B, sp, S, qg);
+    const double u_i = B.bv[np + qi], v_i = B.bv[2 * np + qi], w_i = B.bv[3 * np + qi];
+    const double p_i = B.bv[4 * np + qi], t_i = B.bv[5 * np + qi];
+    double t_g = t_i;
+    if (!wl.adiabatic) {
+      t_g = 2 * t_wall - t_i;
+      if (t_g <= 0.1 * t_wall) t_g = t_wall;
+    }
+    const double rho_g = p_i * mw / (t_g * kRu);
+    B.bv[qg] = rho_g;
+    B.bv[np + qg] = -u_i;
+    B.bv[2 * np + qg] = -v_i;
+    B.bv[3 * np + qg] = -w_i;
+    B.bv[4 * np + qg] = p_i;
+    B.bv[5 * np + qg] = t_g;
+    B.cv[qg] = rho_g;
+    B.cv[np + qg] = -rho_g * u_i;
+    B.cv[2 * np + qg] = -rho_g * v_i;
+    B.cv[3 * np + qg] = -rho_g * w_i;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) B.cv[(l + 5) * np + qg] = B.yk[l * np + qg] * rho_g;
+    total_energy(B, sp, S, qg);
+  }
+}
+
+// Slip wall: the boundary point loses its velocity along the face normal
+// n = grad(xi_face)/|grad(xi_face)|; ghost g mirrors interior point g with
+// the normal velocity negated and rho, p, T, Y copied.
+__device__ __forceinline__ void bc_symmetry_point(const BlockView &B, const FaceRange &f, int i, int j, int k,
+                                                  SpeciesPtrT<double> sp, const SchemeDev &S) {
+  int dir[3] = {0, 0, 0};
+  dir[f.face] = f.direction;
+  const long long np = B.np;
+  const long long p = pidx(B, i, j, k);
+  double n[3];
+  for (int c = 0; c < 3; ++c) n[c] = B.metric[(f.face * 3 + c) * np + p];
+  const double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  n[0] /= nn;
+  n[1] /= nn;
+  n[2] /= nn;
+  {
+    const double vn = B.bv[np + p] * n[0] + B.bv[2 * np + p] * n[1] + B.bv[3 * np + p] * n[2];
+    for (int c = 0; c < 3; ++c) B.bv[(1 + c) * np + p] -= vn * n[c];
+    for (int c = 0; c < 3; ++c) B.cv[(1 + c) * np + p] = B.bv[p] * B.bv[(1 + c) * np + p];
+    total_energy(B, sp, S, p);
+  }
+  for (int g = 1; g <= kNgg; ++g) {
+    const long long qi = pidx(B, i - g * dir[0], j - g * dir[1], k - g * dir[2]);
+    const long long qg = pidx(B, i + g * dir[0], j + g * dir[1], k + g * dir[2]);
+    const double vn = B.bv[np + qi] * n[0] + B.bv[2 * np + qi] * n[1] + B.bv[3 * np + qi] * n[2];
+    B.bv[qg] = B.bv[qi];
+    for (int c = 0; c < 3; ++c) B.bv[(1 + c) * np + qg] = B.bv[(1 + c) * np + qi] - 2 * vn * n[c];
+    B.bv[4 * np + qg] = B.bv[4 * np + qi];
+    B.bv[5 * np + qg] = B.bv[5 * np + qi];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      B.yk[l * np + qg] = B.yk[l * np + qi];
+      B.cv[(5 + l) * np + qg] = B.bv[qg] * B.yk[l * np + qg];
+    }
+    B.cv[qg] = B.bv[qg];
+    for (int c = 0; c < 3; ++c) B.cv[(1 + c) * np + qg] = B.bv[qg] * B.bv[(1 + c) * np + qg];
+    total_energy(B, sp, S, qg);
+  }
+}
+
+// inner_communication (Field.cu:305-358) with the face loop of
+// Mesh.cpp:905-917 at loop point n: shared-face points are averaged between
+// the two blocks, ghost layers take the neighbour's values.
+__device__ __forceinline__ void exchange_point(const BlockView &B, const BlockView &T, const InnerDev &f,
+                                               const int n[3]) {
+  int idx[3], d_idx[3], it[3];
+  for (int q = 0; q < 3; ++q) {
+    d_idx[q] = f.loop_dir[q] * n[q];
+    idx[q] = f.rs[q] + d_idx[q];
+  }
+  for (int q = 0; q < 3; ++q) it[q] = f.ts[q] + f.target_loop_dir[q] * d_idx[f.src_tar[q]];
+  const long long p = pidx(B, idx[0], idx[1], idx[2]);
+  const long long t = pidx(T, it[0], it[1], it[2]);
+  const long long np = B.np, tnp = T.np;
+  const int face_layer = f.direction > 0 ? f.rs[f.face] : f.re[f.face];
+  if (idx[f.face] == face_layer) {
+#pragma unroll
+    for (int l = 0; l < 6; ++l) {
+      const double a = 0.5 * (T.bv[l * tnp + t] + B.bv[l * np + p]);
+      B.bv[l * np + p] = a;
+      T.bv[l * tnp + t] = a;
+    }
+#pragma unroll
+    for (int l = 0; l < 5 + NS; ++l) {
+      const double a = 0.5 * (T.cv[l * tnp + t] + B.cv[l * np + p]);
+      B.cv[l * np + p] = a;
+      T.cv[l * tnp + t] = a;
+    }
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      const double a = 0.5 * (T.yk[l * tnp + t] + B.yk[l * np + p]);
+      B.yk[l * np + p] = a;
+      T.yk[l * tnp + t] = a;
+    }
+  } else {
+#pragma unroll
+    for (int l = 0; l < 6; ++l) B.bv[l * np + p] = T.bv[l * tnp + t];
+#pragma unroll
+    for (int l = 0; l < 5 + NS; ++l) B.cv[l * np + p] = T.cv[l * tnp + t];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) B.yk[l * np + p] = T.yk[l * tnp + t];
+  }
+}
+
+// One launch runs a set of ghost-fill ops that touch disjoint data (one
+// level of the op schedule, or a single op): workgroup -> op by the prefix
+// off[], then one point of the op per thread.
+__global__ void __launch_bounds__(kTpb) face_ops_kernel(const BlockView *__restrict__ views,
+                                                        const FaceOp *__restrict__ ops, const int *__restrict__ off,
+                                                        int nops, const SpeciesDev *__restrict__ spg, SchemeDev S,
+                                                        const InflowDev *__restrict__ inflow) {
+  const SpeciesPtrT<double> sp = const_table<double>(spg);
+  const int w = blockIdx.x + off[0];
+  const int s = find_seg(off, nops, w);
+  const FaceOp &op = ops[s];
+  const long long n = (long long)(w - off[s]) * kTpb + threadIdx.x;
+  if (n >= op.npts) return;
+  const int e0 = op.ext[0], e1 = op.ext[1];
+  const int nn[3] = {(int)(n % e0), (int)((n / e0) % e1), (int)(n / ((long long)e0 * e1))};
+  const BlockView B = views[op.blk];
+  const int i = op.r.rs[0] + nn[0], j = op.r.rs[1] + nn[1], k = op.r.rs[2] + nn[2];
+  switch (op.kind) {
+    case CFX_BC_OUTFLOW: bc_outflow_point(B, op.r, i, j, k); break;
+    case CFX_BC_INFLOW: bc_inflow_point(B, op.r, i, j, k, inflow + op.cond); break;
+    case CFX_BC_WALL: bc_wall_point(B, op.r, i, j, k, sp, S, op.wl); break;
+    case CFX_BC_SYMMETRY: bc_symmetry_point(B, op.r, i, j, k, sp, S); break;
+    case kOpExchange: {
+      const BlockView T = views[op.tgt];
+      exchange_point(B, T, op.f, nn);
+      break;
+    }
+    default: break;
+  }
+}
+
+int launch_update(const Batch &b, const SpeciesDev *sp, const SchemeDev &S, int stage, const double *dt, int *flags,
+                  unsigned long long *nf, hipStream_t st) {
+  if (b.total <= 0) return 0;
+  update_kernel<<<b.total, kTpb, 0, st>>>(b.views, b.off, b.nb, sp, S, stage, dt, flags, nf);
+  return (int)hipGetLastError();
+}
+
+int launch_cv_from_bv(const Batch &b, const SpeciesDev *sp, const SchemeDev &S, hipStream_t st) {
+  if (b.total <= 0) return 0;
+  cv_from_bv_kernel<<<b.total, kTpb, 0, st>>>(b.views, b.off, b.nb, sp, S);
+  return (int)hipGetLastError();
+}
+
+int launch_face_ops(const BlockView *views, const FaceOp *ops, const int *off, int nops, int total,
+                    const SpeciesDev *sp, const SchemeDev &S, const InflowDev *inflow, hipStream_t st) {
+  if (total <= 0) return 0;
+  face_ops_kernel<<<total, kTpb, 0, st>>>(views, ops, off, nops, sp, S, inflow);
+  return (int)hipGetLastError();
+}
+
+struct Register64 {
+  Register64() {
+    KernelSet64 &k = g_kernels64[NS];
+    k.props = launch_props<double>;
+    k.cv_from_bv = launch_cv_from_bv;
+    k.update = launch_update;
+    k.face_ops = launch_face_ops;
+    register_rhs<double>(k);
+    k.present = 1;
+  }
+} register64;
+
+#else  // CFX_PREC == 32
+
+struct Register32 {
+  Register32() {
+    KernelSet32 &k = g_kernels32[NS];
+    k.props = launch_props<float>;
+    register_rhs<float>(k);
+    k.present = 1;
+  }
+} register32;
+
+#endif
+
+}  // namespace
+
+}  // namespace cfx

@@ -66,16 +66,12 @@ struct PencilShape {
   static constexpr int STG = KI * BI::V > KV * BV::V ? KI * BI::V : KV * BV::V;
 };
 
-// Tile order: workgroups are dealt to the 8 XCDs round-robin, so each XCD
-// receives a contiguous run of the tile sequence; inside it the tiles of the
-// (b, c) plane are visited in G x G groups with a fastest, so that tiles
-// sharing a tangential halo run close together and reuse it from L2.
-// Returns false for the padding tiles of partial groups.
+// Tile order inside a block: the tiles of the (b, c) plane are visited in
+// G x G groups with a fastest, so that tiles sharing a tangential halo run
+// close together.  Returns false for the padding tiles of partial groups.
 template <int G>
-__device__ __forceinline__ bool pencil_tile(int bid, int nta, int ntb, int ntc, int &ia, int &ib, int &ic) {
-  const int gb = (ntb + G - 1) / G, gc = (ntc + G - 1) / G;
-  const int nt = nta * G * G * gb * gc;
-  const int lid = nt % 8 == 0 ? (bid % 8) * (nt / 8) + bid / 8 : bid;
+__device__ __forceinline__ bool pencil_tile(int lid, int nta, int ntb, int ntc, int &ia, int &ib, int &ic) {
+  const int gb = (ntb + G - 1) / G;
   ia = lid % nta;
   int q = lid / nta;
   const int bin = q % G;
@@ -87,10 +83,17 @@ __device__ __forceinline__ bool pencil_tile(int bid, int nta, int ntb, int ntc, 
   return ib < ntb && ic < ntc;
 }
 
+// The tiles of all blocks of a batch form one sequence (block by block,
+// off[] the per-block prefix).  Workgroups are dealt to the 8 XCDs
+// round-robin, so sequence position t is remapped to lid = (t % 8) * (padded
+// / 8) + t / 8: each XCD works through a contiguous run of tiles (whole
+// blocks where they are small) and keeps their halos in its own L2.  With a
+// grid smaller than the tile count the workgroups stay resident and stride
+// through the sequence (persistent launch).
 template <int NS, int DIR, int TA, int TB, int TC, int WPE, typename R, int G = 1>
 __global__ void __launch_bounds__(TA *TB *TC) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
-    rhs_pencil_kernel(BlockView B, const SpeciesDev *__restrict__ spg, SchemeDev S, int accumulate, int nta, int ntb,
-                      int ntc) {
+    rhs_pencil_kernel(const BlockView *__restrict__ views, const int *__restrict__ off, int nb, int padded,
+                      const SpeciesDev *__restrict__ spg, SchemeDev S, int accumulate) {
   const SpeciesPtrT<R> sp = const_table<R>(spg);
   using P = PencilShape<NS, DIR, TA, TB, TC>;
   using BI = typename P::BI;
@@ -100,221 +103,232 @@ __global__ void __launch_bounds__(TA *TB *TC) __attribute__((amdgpu_waves_per_eu
   __shared__ R stg[P::STG];    // a batch of staged variables
   __shared__ R fbuf[NV * NT];  // each thread's face flux F_inv - F_visc
   using Fd = Fields<R>;
-  const R *__restrict__ BV_ = Fd::bv(B);
-  const R *__restrict__ YK_ = Fd::yk(B);
-  const R *__restrict__ PR_ = Fd::props(B);
-  const R *__restrict__ MT_ = Fd::metric(B);
-  const R *__restrict__ JC_ = Fd::jac(B);
+  const int total = off[nb] - off[0];
+  for (int tseq = blockIdx.x; tseq < padded; tseq += gridDim.x) {
+    const int lid = (tseq & 7) * (padded >> 3) + (tseq >> 3);
+    if (lid >= total) continue;  // uniform per workgroup
+    const int wgi = lid + off[0];
+    const int blk = find_seg(off, nb, wgi);
+    const BlockView B = views[blk];
+    const R *__restrict__ BV_ = Fd::bv(B);
+    const R *__restrict__ YK_ = Fd::yk(B);
+    const R *__restrict__ PR_ = Fd::props(B);
+    const R *__restrict__ MT_ = Fd::metric(B);
+    const R *__restrict__ JC_ = Fd::jac(B);
 
-  const long long np = B.np;
-  const int m[3] = {B.mx, B.my, B.mz};
-  const long long st[3] = {1, B.sy, B.sz};
-  int ia, ib, ic;
-  if (!pencil_tile<G>(blockIdx.x, nta, ntb, ntc, ia, ib, ic)) return;  // uniform per workgroup
-  int o[3];
-  o[A] = ia * (TA - 1);
-  o[Bx] = ib * TB;
-  o[Cx] = ic * TC;
-  const int tid = threadIdx.x;
-  int t[3];
-  t[0] = tid % P::T0;
-  t[1] = (tid / P::T0) % P::T1;
-  t[2] = tid / (P::T0 * P::T1);
-  int g[3];
-  g[A] = o[A] - 1 + t[A];
-  g[Bx] = o[Bx] + t[Bx];
-  g[Cx] = o[Cx] + t[Cx];
-  const bool face_ok = g[A] + 1 <= m[A] && g[Bx] < m[Bx] && g[Cx] < m[Cx];
-  const bool cell_ok = face_ok && t[A] <= TA - 2 && g[A] + 1 < m[A];
-  const long long pl_i = face_ok ? (long long)(g[2] + kNgg) * B.sz + (long long)(g[1] + kNgg) * B.sy + (g[0] + kNgg) : 0;
-  const long long pr_i = pl_i + st[A];
+    const long long np = B.np;
+    const int m[3] = {B.mx, B.my, B.mz};
+    const long long st[3] = {1, B.sy, B.sz};
+    const int nta = (m[A] + TA - 2) / (TA - 1);  // TA faces cover TA-1 cells
+    const int ntb = (m[Bx] + TB - 1) / TB;
+    const int ntc = (m[Cx] + TC - 1) / TC;
+    int ia, ib, ic;
+    if (!pencil_tile<G>(wgi - off[blk], nta, ntb, ntc, ia, ib, ic)) continue;  // uniform per workgroup
+    int o[3];
+    o[A] = ia * (TA - 1);
+    o[Bx] = ib * TB;
+    o[Cx] = ic * TC;
+    const int tid = threadIdx.x;
+    int t[3];
+    t[0] = tid % P::T0;
+    t[1] = (tid / P::T0) % P::T1;
+    t[2] = tid / (P::T0 * P::T1);
+    int g[3];
+    g[A] = o[A] - 1 + t[A];
+    g[Bx] = o[Bx] + t[Bx];
+    g[Cx] = o[Cx] + t[Cx];
+    const bool face_ok = g[A] + 1 <= m[A] && g[Bx] < m[Bx] && g[Cx] < m[Cx];
+    const bool cell_ok = face_ok && t[A] <= TA - 2 && g[A] + 1 < m[A];
+    const long long pl_i = face_ok ? (long long)(g[2] + kNgg) * B.sz + (long long)(g[1] + kNgg) * B.sy + (g[0] + kNgg) : 0;
+    const long long pr_i = pl_i + st[A];
 
-  // stage K variables (source pointers) over box BX with global origin bo;
-  // points beyond the padded range are left unwritten (no valid face reads them)
-  auto stage = [&](auto box, const R *const *src, int K, const int bo[3]) {
-    using BX = decltype(box);
-    __syncthreads();
-    for (int e = tid; e < BX::V; e += NT) {
-      const int x = bo[0] + e % BX::L0;
-      const int y = bo[1] + (e / BX::L0) % BX::L1;
-      const int z = bo[2] + e / (BX::L0 * BX::L1);
-      if (x <= m[0] + 1 && y <= m[1] + 1 && z <= m[2] + 1) {
-        const long long gi = (long long)(z + kNgg) * B.sz + (long long)(y + kNgg) * B.sy + (x + kNgg);
-        for (int k = 0; k < K; ++k) stg[k * BX::V + e] = src[k][gi];
-      }
-    }
-    __syncthreads();
-  };
-
-  R k1 = 0, k2 = 0, k3 = 0;
-
-  // ---- inviscid phase: F_inv into this thread's face slot
-  {
-    int bo[3];
-    bo[A] = o[A] - 2;
-    bo[Bx] = o[Bx];
-    bo[Cx] = o[Cx];
-    int l[3];
-    l[A] = t[A] + 1;
-    l[Bx] = t[Bx];
-    l[Cx] = t[Cx];
-    const int li = l[0] + BI::L0 * (l[1] + BI::L1 * l[2]);
-    constexpr int sa = BI::sa;
-    R pl[NV], pr[NV];
-#pragma unroll
-    for (int v0 = 0; v0 < NV; v0 += P::KI) {
-      const R *src[P::KI];
-      const int K = NV - v0 < P::KI ? NV - v0 : P::KI;
-#pragma unroll
-      for (int k = 0; k < P::KI; ++k) {
-        const int v = v0 + k < NV ? v0 + k : NV - 1;
-        src[k] = v < 5 ? BV_ + v * np : YK_ + (v - 5) * np;
-      }
-      stage(BI{}, src, K, bo);
-#pragma unroll
-      for (int k = 0; k < P::KI; ++k) {
-        if (v0 + k >= NV) break;
-        const R *q = stg + k * BI::V + li;
-        muscl<R>(S, q[-sa], q[0], q[sa], q[2 * sa], pl[v0 + k], pr[v0 + k]);
-      }
-    }
-    R F[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) F[v] = R(0);
-    if (face_ok) {
-      const R *M = MT_ + 3 * A * np;
-      const R jl = JC_[pl_i], jr = JC_[pr_i];
-      k1 = R(0.5) * (jl * M[pl_i] + jr * M[pr_i]);
-      k2 = R(0.5) * (jl * M[np + pl_i] + jr * M[np + pr_i]);
-      k3 = R(0.5) * (jl * M[2 * np + pl_i] + jr * M[2 * np + pr_i]);
-      ausm_from_states<NS, R>(sp, S, pl, pr, k1, k2, k3, F);
-    }
-#pragma unroll
-    for (int v = 0; v < NV; ++v) fbuf[v * NT + tid] = F[v];
-  }
-
-  // ---- viscous phase: F_visc subtracted from the slot as it is formed
-  if (S.viscous == 2) {
-    int bo[3];
-    bo[A] = o[A] - 1;
-    bo[Bx] = o[Bx] - 1;
-    bo[Cx] = o[Cx] - 1;
-    int l[3];
-    l[A] = t[A];
-    l[Bx] = t[Bx] + 1;
-    l[Cx] = t[Cx] + 1;
-    const int li = l[0] + BV::L0 * (l[1] + BV::L1 * l[2]);
-    constexpr int sa = BV::sa, sb = BV::sb, sc = BV::sc;
-    R mf[9];
-    if (face_ok) {
-      const bool q5 = (S.quirks & CFX_Q_METRIC_AVG) && DIR != 1;
-#pragma unroll
-      for (int rc = 0; rc < 9; ++rc) {
-        const R a = MT_[rc * np + pl_i];
-        mf[rc] = (q5 && rc != 0) ? R(0.5) * (a + a) : R(0.5) * (a + MT_[rc * np + pr_i]);
-      }
-    } else {
-#pragma unroll
-      for (int rc = 0; rc < 9; ++rc) mf[rc] = R(0);
-    }
-    // computational-space derivatives of staged variable k at the face
-    auto grad = [&](int k, R gc[3]) {
-      const R *q = stg + k * BV::V + li;
-      gc[A] = q[sa] - q[0];
-      gc[Bx] = R(0.25) * (q[sb] - q[-sb] + q[sa + sb] - q[sa - sb]);
-      gc[Cx] = R(0.25) * (q[sc] - q[-sc] + q[sa + sc] - q[sa - sc]);
-    };
-    auto chain = [&](const R gc[3], R out[3]) {
-      out[0] = gc[0] * mf[0] + gc[1] * mf[3] + gc[2] * mf[6];
-      out[1] = gc[0] * mf[1] + gc[1] * mf[4] + gc[2] * mf[7];
-      out[2] = gc[0] * mf[2] + gc[1] * mf[5] + gc[2] * mf[8];
-    };
-    R gcv[3], du[3], dv[3], dw[3], dT[3], avg[4];
-    {
-      const R *src[P::KV] = {BV_ + np, BV_ + 2 * np, BV_ + 3 * np, BV_ + 5 * np};
-      stage(BV{}, src, 4, bo);
-      grad(0, gcv);
-      chain(gcv, du);
-      grad(1, gcv);
-      chain(gcv, dv);
-      grad(2, gcv);
-      chain(gcv, dw);
-      grad(3, gcv);
-      chain(gcv, dT);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) avg[q] = R(0.5) * (stg[q * BV::V + li] + stg[q * BV::V + li + sa]);
-    }
-    {
-      const R vis = face_ok ? R(0.5) * (PR_[2 * np + pl_i] + PR_[2 * np + pr_i]) : R(0);
-      const R cond = face_ok ? R(0.5) * (PR_[3 * np + pl_i] + PR_[3 * np + pr_i]) : R(0);
-      const R third = R(1.0 / 3.0);
-      const R txx = vis * (R(4) * du[0] - R(2) * dv[1] - R(2) * dw[2]) * third;
-      const R tyy = vis * (R(4) * dv[1] - R(2) * du[0] - R(2) * dw[2]) * third;
-      const R tzz = vis * (R(4) * dw[2] - R(2) * du[0] - R(2) * dv[1]) * third;
-      const R txy = vis * (du[1] + dv[0]);
-      const R txz = vis * (du[2] + dw[0]);
-      const R tyz = vis * (dv[2] + dw[1]);
-      const R G1 = k1 * txx + k2 * txy + k3 * txz;
-      const R G2 = k1 * txy + k2 * tyy + k3 * tyz;
-      const R G3 = k1 * txz + k2 * tyz + k3 * tzz;
-      const R G4 = avg[0] * G1 + avg[1] * G2 + avg[2] * G3 + cond * (k1 * dT[0] + k2 * dT[1] + k3 * dT[2]);
-      fbuf[1 * NT + tid] -= G1;
-      fbuf[2 * NT + tid] -= G2;
-      fbuf[3 * NT + tid] -= G3;
-      fbuf[4 * NT + tid] -= G4;
-    }
-    if constexpr (NS > 0) {
-      const R tm = avg[3];
-      const long long pD = (DIR == 2 && (S.quirks & CFX_Q_HV_RHOD_J)) ? pl_i + B.sy : pr_i;
-      R dp[NS], yb[NS];
-      R ucx = 0, ucy = 0, ucz = 0, e4 = 0;
-#pragma unroll
-      for (int s0 = 0; s0 < NS; s0 += P::KV) {
-        const R *src[P::KV];
-        const int K = NS - s0 < P::KV ? NS - s0 : P::KV;
-#pragma unroll
-        for (int k = 0; k < P::KV; ++k) src[k] = YK_ + (s0 + k < NS ? s0 + k : NS - 1) * np;
-        stage(BV{}, src, K, bo);
-#pragma unroll
-        for (int k = 0; k < P::KV; ++k) {
-          const int s = s0 + k;
-          if (s >= NS) break;
-          grad(k, gcv);
-          R dy[3];
-          chain(gcv, dy);
-          yb[s] = R(0.5) * (stg[k * BV::V + li] + stg[k * BV::V + li + sa]);
-          const R dif = face_ok ? R(0.5) * (PR_[(4 + s) * np + pl_i] + PR_[(4 + s) * np + pD]) : R(0);
-          R h, cp;
-          species_h_cp<R>(sp, s, tm, h, cp);
-          const R proj = k1 * dy[0] + k2 * dy[1] + k3 * dy[2];
-          e4 += dif * h * proj;
-          ucx += dif * dy[0];
-          ucy += dif * dy[1];
-          ucz += dif * dy[2];
-          dp[s] = dif * proj;
+    // stage K variables (source pointers) over box BX with global origin bo;
+    // points beyond the padded range are left unwritten (no valid face reads them)
+    auto stage = [&](auto box, const R *const *src, int K, const int bo[3]) {
+      using BX = decltype(box);
+      __syncthreads();
+      for (int e = tid; e < BX::V; e += NT) {
+        const int x = bo[0] + e % BX::L0;
+        const int y = bo[1] + (e / BX::L0) % BX::L1;
+        const int z = bo[2] + e / (BX::L0 * BX::L1);
+        if (x <= m[0] + 1 && y <= m[1] + 1 && z <= m[2] + 1) {
+          const long long gi = (long long)(z + kNgg) * B.sz + (long long)(y + kNgg) * B.sy + (x + kNgg);
+          for (int k = 0; k < K; ++k) stg[k * BX::V + e] = src[k][gi];
         }
       }
-      const R uc = k1 * ucx + k2 * ucy + k3 * ucz;
-      fbuf[4 * NT + tid] -= e4;
-#pragma unroll
-      for (int s = 0; s < NS; ++s) fbuf[(5 + s) * NT + tid] -= dp[s] - yb[s] * uc;
-    }
-  }
+      __syncthreads();
+    };
 
-  // ---- cell difference of the face slots, dq written once per direction
-  __syncthreads();
-  if (cell_ok) {
-    int c[3] = {g[0], g[1], g[2]};
-    c[A] += 1;
-    const long long q = ((long long)c[2] * B.my + c[1]) * B.mx + c[0];
-    R *__restrict__ DQ = Fd::dq(B);
+    R k1 = 0, k2 = 0, k3 = 0;
+
+    // ---- inviscid phase: F_inv into this thread's face slot
+    {
+      int bo[3];
+      bo[A] = o[A] - 2;
+      bo[Bx] = o[Bx];
+      bo[Cx] = o[Cx];
+      int l[3];
+      l[A] = t[A] + 1;
+      l[Bx] = t[Bx];
+      l[Cx] = t[Cx];
+      const int li = l[0] + BI::L0 * (l[1] + BI::L1 * l[2]);
+      constexpr int sa = BI::sa;
+      R pl[NV], pr[NV];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      const R d = -(fbuf[v * NT + tid + ta_stride] - fbuf[v * NT + tid]);
-      if (accumulate) DQ[v * B.ni + q] += d;
-      else DQ[v * B.ni + q] = d;
+      for (int v0 = 0; v0 < NV; v0 += P::KI) {
+        const R *src[P::KI];
+        const int K = NV - v0 < P::KI ? NV - v0 : P::KI;
+#pragma unroll
+        for (int k = 0; k < P::KI; ++k) {
+          const int v = v0 + k < NV ? v0 + k : NV - 1;
+          src[k] = v < 5 ? BV_ + v * np : YK_ + (v - 5) * np;
+        }
+        stage(BI{}, src, K, bo);
+#pragma unroll
+        for (int k = 0; k < P::KI; ++k) {
+          if (v0 + k >= NV) break;
+          const R *q = stg + k * BI::V + li;
+          muscl<R>(S, q[-sa], q[0], q[sa], q[2 * sa], pl[v0 + k], pr[v0 + k]);
+        }
+      }
+      R F[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) F[v] = R(0);
+      if (face_ok) {
+        const R *M = MT_ + 3 * A * np;
+        const R jl = JC_[pl_i], jr = JC_[pr_i];
+        k1 = R(0.5) * (jl * M[pl_i] + jr * M[pr_i]);
+        k2 = R(0.5) * (jl * M[np + pl_i] + jr * M[np + pr_i]);
+        k3 = R(0.5) * (jl * M[2 * np + pl_i] + jr * M[2 * np + pr_i]);
+        ausm_from_states<NS, R>(sp, S, pl, pr, k1, k2, k3, F);
+      }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) fbuf[v * NT + tid] = F[v];
     }
-  }
+
+    // ---- viscous phase: F_visc subtracted from the slot as it is formed
+    if (S.viscous == 2) {
+      int bo[3];
+      bo[A] = o[A] - 1;
+      bo[Bx] = o[Bx] - 1;
+      bo[Cx] = o[Cx] - 1;
+      int l[3];
+      l[A] = t[A];
+      l[Bx] = t[Bx] + 1;
+      l[Cx] = t[Cx] + 1;
+      const int li = l[0] + BV::L0 * (l[1] + BV::L1 * l[2]);
+      constexpr int sa = BV::sa, sb = BV::sb, sc = BV::sc;
+      R mf[9];
+      if (face_ok) {
+        const bool q5 = (S.quirks & CFX_Q_METRIC_AVG) && DIR != 1;
+#pragma unroll
+        for (int rc = 0; rc < 9; ++rc) {
+          const R a = MT_[rc * np + pl_i];
+          mf[rc] = (q5 && rc != 0) ? R(0.5) * (a + a) : R(0.5) * (a + MT_[rc * np + pr_i]);
+        }
+      } else {
+#pragma unroll
+        for (int rc = 0; rc < 9; ++rc) mf[rc] = R(0);
+      }
+      // computational-space derivatives of staged variable k at the face
+      auto grad = [&](int k, R gc[3]) {
+        const R *q = stg + k * BV::V + li;
+        gc[A] = q[sa] - q[0];
+        gc[Bx] = R(0.25) * (q[sb] - q[-sb] + q[sa + sb] - q[sa - sb]);
+        gc[Cx] = R(0.25) * (q[sc] - q[-sc] + q[sa + sc] - q[sa - sc]);
+      };
+      auto chain = [&](const R gc[3], R out[3]) {
+        out[0] = gc[0] * mf[0] + gc[1] * mf[3] + gc[2] * mf[6];
+        out[1] = gc[0] * mf[1] + gc[1] * mf[4] + gc[2] * mf[7];
+        out[2] = gc[0] * mf[2] + gc[1] * mf[5] + gc[2] * mf[8];
+      };
+      R gcv[3], du[3], dv[3], dw[3], dT[3], avg[4];
+      {
+        const R *src[P::KV] = {BV_ + np, BV_ + 2 * np, BV_ + 3 * np, BV_ + 5 * np};
+        stage(BV{}, src, 4, bo);
+        grad(0, gcv);
+        chain(gcv, du);
+        grad(1, gcv);
+        chain(gcv, dv);
+        grad(2, gcv);
+        chain(gcv, dw);
+        grad(3, gcv);
+        chain(gcv, dT);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) avg[q] = R(0.5) * (stg[q * BV::V + li] + stg[q * BV::V + li + sa]);
+      }
+      {
+        const R vis = face_ok ? R(0.5) * (PR_[2 * np + pl_i] + PR_[2 * np + pr_i]) : R(0);
+        const R cond = face_ok ? R(0.5) * (PR_[3 * np + pl_i] + PR_[3 * np + pr_i]) : R(0);
+        const R third = R(1.0 / 3.0);
+        const R txx = vis * (R(4) * du[0] - R(2) * dv[1] - R(2) * dw[2]) * third;
+        const R tyy = vis * (R(4) * dv[1] - R(2) * du[0] - R(2) * dw[2]) * third;
+        const R tzz = vis * (R(4) * dw[2] - R(2) * du[0] - R(2) * dv[1]) * third;
+        const R txy = vis * (du[1] + dv[0]);
+        const R txz = vis * (du[2] + dw[0]);
+        const R tyz = vis * (dv[2] + dw[1]);
+        const R G1 = k1 * txx + k2 * txy + k3 * txz;
+        const R G2 = k1 * txy + k2 * tyy + k3 * tyz;
+        const R G3 = k1 * txz + k2 * tyz + k3 * tzz;
+        const R G4 = avg[0] * G1 + avg[1] * G2 + avg[2] * G3 + cond * (k1 * dT[0] + k2 * dT[1] + k3 * dT[2]);
+        fbuf[1 * NT + tid] -= G1;
+        fbuf[2 * NT + tid] -= G2;
+        fbuf[3 * NT + tid] -= G3;
+        fbuf[4 * NT + tid] -= G4;
+      }
+      if constexpr (NS > 0) {
+        const R tm = avg[3];
+        const long long pD = (DIR == 2 && (S.quirks & CFX_Q_HV_RHOD_J)) ? pl_i + B.sy : pr_i;
+        R dp[NS], yb[NS];
+        R ucx = 0, ucy = 0, ucz = 0, e4 = 0;
+#pragma unroll
+        for (int s0 = 0; s0 < NS; s0 += P::KV) {
+          const R *src[P::KV];
+          const int K = NS - s0 < P::KV ? NS - s0 : P::KV;
+#pragma unroll
+          for (int k = 0; k < P::KV; ++k) src[k] = YK_ + (s0 + k < NS ? s0 + k : NS - 1) * np;
+          stage(BV{}, src, K, bo);
+#pragma unroll
+          for (int k = 0; k < P::KV; ++k) {
+            const int s = s0 + k;
+            if (s >= NS) break;
+            grad(k, gcv);
+            R dy[3];
+            chain(gcv, dy);
+            yb[s] = R(0.5) * (stg[k * BV::V + li] + stg[k * BV::V + li + sa]);
+            const R dif = face_ok ? R(0.5) * (PR_[(4 + s) * np + pl_i] + PR_[(4 + s) * np + pD]) : R(0);
+            R h, cp;
+            species_h_cp<R>(sp, s, tm, h, cp);
+            const R proj = k1 * dy[0] + k2 * dy[1] + k3 * dy[2];
+            e4 += dif * h * proj;
+            ucx += dif * dy[0];
+            ucy += dif * dy[1];
+            ucz += dif * dy[2];
+            dp[s] = dif * proj;
+          }
+        }
+        const R uc = k1 * ucx + k2 * ucy + k3 * ucz;
+        fbuf[4 * NT + tid] -= e4;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) fbuf[(5 + s) * NT + tid] -= dp[s] - yb[s] * uc;
+      }
+    }
+
+    // ---- cell difference of the face slots, dq written once per direction
+    __syncthreads();
+    if (cell_ok) {
+      int c[3] = {g[0], g[1], g[2]};
+      c[A] += 1;
+      const long long q = ((long long)c[2] * B.my + c[1]) * B.mx + c[0];
+      R *__restrict__ DQ = Fd::dq(B);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const R d = -(fbuf[v * NT + tid + ta_stride] - fbuf[v * NT + tid]);
+        if (accumulate) DQ[v * B.ni + q] += d;
+        else DQ[v * B.ni + q] = d;
+      }
+    }
+  }  // tile loop
 }
 
 }  // namespace cfx

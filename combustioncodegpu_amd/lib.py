@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libcfx.so")
 _lib = None
 
 K_RHS, K_UPDATE, K_BC, K_PROPS, K_DT, K_OTHER = range(6)
-KERNEL_CLASSES = ("rhs", "update", "bc", "props", "dt", "other")
+KERNEL_CLASSES = ("rhs", "update", "ghost", "props", "dt", "other")
 
 
 def load():
@@ -45,6 +45,17 @@ def load():
     lib.cfx_rhs_variants.restype = C.c_int
     lib.cfx_set_rhs_variant.argtypes = [vp, C.c_int]
     lib.cfx_set_rhs_variant.restype = C.c_int
+    ip = C.POINTER(C.c_int)
+    lib.cfx_set_rhs_tiles.argtypes = [vp, ip, ip]
+    lib.cfx_set_rhs_tiles.restype = C.c_int
+    lib.cfx_get_rhs_tiles.argtypes = [vp, ip, ip]
+    lib.cfx_get_rhs_tiles.restype = C.c_int
+    lib.cfx_autotune.argtypes = [vp, C.c_int, dp]
+    lib.cfx_autotune.restype = C.c_int
+    lib.cfx_set_execution.argtypes = [vp, C.c_int, C.c_int]
+    lib.cfx_set_execution.restype = C.c_int
+    lib.cfx_schedule_info.argtypes = [vp, ip, ip]
+    lib.cfx_schedule_info.restype = C.c_int
     _lib = lib
     return lib
 
@@ -76,6 +87,34 @@ class Solver(StepContext):
 
     def set_rhs_variant(self, v):
         self._call("set_rhs_variant", int(v))
+
+    def set_rhs_tiles(self, variant, persist):
+        v = (C.c_int * 3)(*variant)
+        p = (C.c_int * 3)(*persist)
+        self._call("set_rhs_tiles", v, p)
+
+    def rhs_tiles(self):
+        v = (C.c_int * 3)()
+        p = (C.c_int * 3)()
+        self._call("get_rhs_tiles", v, p)
+        return list(v), list(p)
+
+    def autotune(self, reps=3):
+        """Pick the fastest RHS tile shape and persistence per direction on
+        this context's blocks; returns (variants, persistence, ms per direction)."""
+        ms = (C.c_double * 3)()
+        self._call("autotune", int(reps), ms)
+        v, p = self.rhs_tiles()
+        return v, p, list(ms)
+
+    def set_execution(self, batched=True, graph=False):
+        self._call("set_execution", 1 if batched else 0, 1 if graph else 0)
+
+    def schedule_info(self):
+        n_ops = C.c_int(0)
+        n_lev = C.c_int(0)
+        self._call("schedule_info", C.byref(n_ops), C.byref(n_lev))
+        return n_ops.value, n_lev.value
 
     def kernel_times(self):
         out = {}

@@ -224,16 +224,43 @@ int cfx_sync(cfx_ctx *ctx);
 int cfx_time_steps(cfx_ctx *ctx, int n_steps, double *ms_total);
 
 /* Per-kernel timing of the last cfx_time_steps call: for kernel class c
- * (0 rhs, 1 update, 2 bc, 3 props, 4 dt, 5 other) the summed milliseconds
- * and launch count, measured with events on the context's stream. */
+ * (0 rhs, 1 update, 2 ghost fill = boundary conditions and inner-face
+ * exchange, 3 props, 4 dt, 5 other) the summed milliseconds and launch
+ * count, measured with events on the context's stream (eager launches only:
+ * a context with profiling on does not replay the step graph). */
 int cfx_kernel_times(cfx_ctx *ctx, int cls, double *ms, long long *launches);
 int cfx_set_profiling(cfx_ctx *ctx, int on);
 
-/* Kernel-variant selection used by the autotuner (no counterpart in the
- * reference): the number of compiled variants of the RHS kernel (tile shape
- * x target occupancy) for the context's dimension, and the selection. */
+/* Execution of the step (no counterpart in the reference, whose Driver
+ * launches every kernel block by block, Driver.cu:189-244).
+ *   batched = 0: one launch per block and kernel, one launch per boundary
+ *                face and per inner face, in the reference's order;
+ *   batched = 1 (default): one launch per kernel over all blocks, and the
+ *                boundary/exchange face operations grouped into levels of
+ *                operations that touch disjoint points (order kept between
+ *                any two that overlap), one launch per level;
+ *   graph = 1:   cfx_step replays one captured step graph (batched or not).
+ * Results are identical in every mode. */
+int cfx_set_execution(cfx_ctx *ctx, int batched, int graph);
+/* Ghost-fill operations (boundary faces + inner faces) of one stage and the
+ * number of levels (launches) they take when batched. */
+int cfx_schedule_info(cfx_ctx *ctx, int *n_ops, int *n_levels);
+
+/* RHS tile selection and autotuner (no counterpart in the reference, whose
+ * inviscid kernels use fixed 64-point lines, TimeAdvanceFunc.cu:38-154).
+ * The RHS runs one kernel per face direction; each direction has
+ * cfx_rhs_variants() tile shapes, and a persistence factor p (0: one
+ * workgroup per tile; p > 0: p x resident-workgroups-per-CU x CUs
+ * workgroups looping over the tiles). */
 int cfx_rhs_variants(cfx_ctx *ctx);
+/* Same tile variant in every direction, persistence 0. */
 int cfx_set_rhs_variant(cfx_ctx *ctx, int variant);
+int cfx_set_rhs_tiles(cfx_ctx *ctx, const int variant[3], const int persist[3]);
+int cfx_get_rhs_tiles(cfx_ctx *ctx, int variant[3], int persist[3]);
+/* Times every (variant, persistence) of every direction on the context's
+ * blocks, reps launches each, and keeps the fastest; best_ms[3] (may be
+ * NULL) receives the per-direction time of the choice.  Leaves dq undefined. */
+int cfx_autotune(cfx_ctx *ctx, int reps, double *best_ms);
 
 #ifdef __cplusplus
 }
