@@ -7,8 +7,11 @@ A step is one RK3 step (3 stages) over the whole block; inputs are resident
 in HBM before the timed region.
 
 Prints one JSON line: metric/value in Mcell-stage updates/s (interior cells x
-stages / wall time), the roofline object of the dominant kernel (the fused
-RHS), and the CPU baseline (the oracle on the host cores, bounded sample).
+stages / wall time) of the plain FP64 step, the roofline object of the
+dominant kernel (the RHS), the CPU baseline (the oracle on the host cores,
+bounded sample), and beside them the changed forms: mixed precision,
+graph-captured autotuned execution, and the 64-block lattice (about 40M
+points) run block by block and batched.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
   python bench.py --copy-calibration   # plain copy kernel of known size, for
@@ -67,71 +70,113 @@ def cpu_baseline(case, threads):
                       f"{cs} cell-stages in {dt:.2f} s, OpenMP on {used} host threads"}
 
 
-def measure(case, args, name):
+def measure(case, args, name, precision="fp64", batched=True, graph=False, autotune=False, roofline=True,
+            mlp=False):
+    """Throughput of one execution form on a case: warmup, then exactly
+    args.steps steps between synchronisations, wall clock; per-kernel times
+    from HIP events in a separate profiled run of 3 eager steps.  With mlp
+    the surrogate is trained on the GPU first (outside the timed region) and
+    selected as the transport."""
+    from combustioncodegpu_amd import abi
     from combustioncodegpu_amd.lib import Solver
+    case.params["precision"] = abi.MIXED if precision == "mixed" else abi.FP64
     g = Solver(case)
+    case.params["precision"] = abi.FP64
+    mlp_report = None
+    if mlp:
+        log(f"[bench] {name}: training the transport surrogate")
+        t0 = time.perf_counter()
+        mlp_report = g.mlp_train()
+        mlp_report["wall_s"] = round(time.perf_counter() - t0, 2)
+        g.set_transport(abi.TRANSPORT_MLP)
+    g.set_execution(batched=batched, graph=graph)
     g.upload_initial()
     g.init_conservative()
     g.sync()
+    tiles = None
+    if autotune:
+        log(f"[bench] {name}: autotune")
+        var, per, ms = g.autotune(reps=3)
+        g.upload_initial()
+        g.init_conservative()
+        tiles = {"variant": var, "persist": per, "rhs_ms_per_direction": [round(x, 4) for x in ms]}
     log(f"[bench] {name}: warmup")
     g.step(args.warmup)
     g.sync()
     rc, nfail = g.check()
     if rc != 0:
         log(f"[bench] {name}: health check failed rc={rc} newton_fail={nfail}")
+        g.close()
         return None
-    log(f"[bench] {name}: timed region")
+    log(f"[bench] {name}: timed region ({args.steps} steps)")
     g.sync()
     t0 = time.perf_counter()
     ev_ms = g.time_steps(args.steps)
     g.sync()
     wall = time.perf_counter() - t0
     value = case.n_cells * 3 * args.steps / wall / 1e6
-    # per-kernel durations from HIP events on the context's stream
+    # per-kernel durations from HIP events on the context's stream (eager)
     g.set_profiling(True)
     g.time_steps(3)
     kt = g.kernel_times()
     g.set_profiling(False)
-    # one timed RHS event spans the three direction launches of one stage
-    rhs_ms, rhs_n = kt["rhs"]
-    rhs_avg_s = rhs_ms / max(rhs_n, 1) / 1e3
-    bpc = BYTES_PER_CELL_STAGE_FP64 if name == "fp64" else BYTES_PER_CELL_STAGE_MIXED
-    achieved = bpc * case.n_cells / rhs_avg_s / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"rhs_c1_{name}_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_stage")
-    res = g.residual()
-    dt = g.get_dt()
-    g.close()
-    return {
+    n_ops, n_lev = g.schedule_info()
+    out = {
         "name": name, "value": round(value, 3), "ms_per_step": round(1e3 * wall / args.steps, 4),
         "event_ms_per_step": round(ev_ms / args.steps, 4),
         "kernel_ms_per_step": {k: round(v[0] / 3, 4) for k, v in kt.items()},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                     "kernel": "rhs_pencil_kernel (LDS-staged RHS, one launch per face direction)",
-                     "kernel_avg_ms_per_stage": round(rhs_avg_s * 1e3, 4), "launches_per_stage": 3,
-                     "bytes_per_cell_stage": round(bpc, 2)},
-        "dt": dt, "residual": [float(x) for x in res],
+        "launches_per_step": {k: round(v[1] / 3, 1) for k, v in kt.items()},
+        "execution": {"batched": batched, "graph": graph, "blocks": len(case.blocks),
+                      "ghost_fill_ops_per_stage": n_ops, "ghost_fill_levels_per_stage": n_lev},
     }
+    if tiles:
+        out["rhs_tiles"] = tiles
+    if mlp:
+        out["mlp"] = {"training": mlp_report, "counters": g.mlp_stats()}
+    # one timed RHS event per direction launch (batched: all blocks at once)
+    rhs_ms, rhs_n = kt["rhs"]
+    stages = 3 * 3
+    rhs_stage_s = rhs_ms / stages / 1e3
+    if roofline:
+        bpc = BYTES_PER_CELL_STAGE_FP64 if precision == "fp64" else BYTES_PER_CELL_STAGE_MIXED
+        achieved = bpc * case.n_cells / rhs_stage_s / 1e9
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", f"rhs_c1_{precision}_traffic.json")
+        if os.path.exists(tpath):
+            with open(tpath) as f:
+                traffic = json.load(f).get("hbm_bytes_per_stage")
+        out["roofline"] = {
+            "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+            "kernel": "rhs_pencil_kernel (LDS-staged RHS, one launch per face direction)",
+            "kernel_avg_ms_per_stage": round(rhs_stage_s * 1e3, 4),
+            "launches_per_stage": round(rhs_n / stages, 2), "bytes_per_cell_stage": round(bpc, 2)}
+    out["dt"] = g.get_dt()
+    out["residual"] = [float(x) for x in g.residual()]
+    g.close()
+    return out
 
 
-def mixed_errors(case, checkpoints=(1, 10, 100)):
-    """Per-field error of the mixed path against the FP64 path after 1, 10
-    and 100 RK3 steps (max over cells; rho, p, T relative to their max,
-    velocity as a vector relative to max |V|, Y absolute)."""
+def mixed_errors(case, checkpoints=(1, 10, 100), mlp=True):
+    """Per-field error of the mixed path (and of the mixed path with the
+    surrogate transport) against the FP64 path after 1, 10 and 100 RK3 steps
+    (max over cells; rho, p, T relative to their max, velocity as a vector
+    relative to max |V|, Y absolute; L2 relative for rho and T)."""
     import numpy as np
     from combustioncodegpu_amd import abi
     from combustioncodegpu_amd.lib import Solver
     ctx = {}
-    for prec in ("fp64", "mixed"):
-        case.params["precision"] = abi.MIXED if prec == "mixed" else abi.FP64
+    forms = ("fp64", "mixed", "mixed_mlp") if mlp else ("fp64", "mixed")
+    for prec in forms:
+        case.params["precision"] = abi.FP64 if prec == "fp64" else abi.MIXED
         ctx[prec] = Solver(case)
+        if prec == "mixed_mlp":
+            ctx[prec].mlp_train()
+            ctx[prec].set_transport(abi.TRANSPORT_MLP)
         ctx[prec].upload_initial()
         ctx[prec].init_conservative()
-    out = {}
+    case.params["precision"] = abi.FP64
+    out = {f: {} for f in forms if f != "fp64"}
     done = 0
     for n in checkpoints:
         for c in ctx.values():
@@ -139,20 +184,23 @@ def mixed_errors(case, checkpoints=(1, 10, 100)):
         done = n
         b = {k: c.download(0, abi.BV)[:, 2:-2, 2:-2, 2:-2] for k, c in ctx.items()}
         y = {k: c.download(0, abi.YK)[:, 2:-2, 2:-2, 2:-2] for k, c in ctx.items()}
-        r, m = b["fp64"], b["mixed"]
+        r = b["fp64"]
         vmax = np.sqrt((r[1:4] ** 2).sum(axis=0)).max()
-        out[f"after_{n}_steps"] = {
-            "rho": float(np.abs(m[0] - r[0]).max() / np.abs(r[0]).max()),
-            "velocity": float(np.sqrt(((m[1:4] - r[1:4]) ** 2).sum(axis=0)).max() / vmax),
-            "p": float(np.abs(m[4] - r[4]).max() / np.abs(r[4]).max()),
-            "T": float(np.abs(m[5] - r[5]).max() / np.abs(r[5]).max()),
-            "Y": float(np.abs(y["mixed"] - y["fp64"]).max()),
-            "rho_l2": float(np.sqrt(np.mean((m[0] - r[0]) ** 2)) / np.sqrt(np.mean(r[0] ** 2))),
-            "T_l2": float(np.sqrt(np.mean((m[5] - r[5]) ** 2)) / np.sqrt(np.mean(r[5] ** 2))),
-        }
+        for f in out:
+            m = b[f]
+            out[f][f"after_{n}_steps"] = {
+                "rho": float(np.abs(m[0] - r[0]).max() / np.abs(r[0]).max()),
+                "velocity": float(np.sqrt(((m[1:4] - r[1:4]) ** 2).sum(axis=0)).max() / vmax),
+                "p": float(np.abs(m[4] - r[4]).max() / np.abs(r[4]).max()),
+                "T": float(np.abs(m[5] - r[5]).max() / np.abs(r[5]).max()),
+                "Y": float(np.abs(y[f] - y["fp64"]).max()),
+                "rho_l2": float(np.sqrt(np.mean((m[0] - r[0]) ** 2)) / np.sqrt(np.mean(r[0] ** 2))),
+                "T_l2": float(np.sqrt(np.mean((m[5] - r[5]) ** 2)) / np.sqrt(np.mean(r[5] ** 2))),
+            }
+    if "mixed_mlp" in ctx:
+        out["mixed_mlp"]["surrogate_counters"] = ctx["mixed_mlp"].mlp_stats()
     for c in ctx.values():
         c.close()
-    case.params["precision"] = abi.FP64
     return out
 
 
@@ -162,14 +210,19 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c3", action="store_true", help="skip the 64-block batched workload")
+    ap.add_argument("--no-c4", action="store_true", help="skip the 384x256x256 surrogate workload")
+    ap.add_argument("--no-mlp", action="store_true", help="skip every surrogate-transport form")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--size", type=str, default="256,128,128")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--rhs-only", type=int, default=0,
                     help="profiling mode: run this many RHS launches on the workload and exit")
+    ap.add_argument("--steps-only", type=int, default=0,
+                    help="profiling mode: run this many steps of the workload's changed form and exit")
+    ap.add_argument("--workload", choices=["c1", "c3"], default="c1")
     ap.add_argument("--rhs-variant", type=int, default=-1)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
-                    help="precision of the headline line; the other variant is measured beside it")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -182,34 +235,104 @@ def main():
     from combustioncodegpu_amd.lib import Solver
 
     mx, my, mz = (int(v) for v in args.size.split(","))
-    log(f"[bench] generating workload {mx}x{my}x{mz}, 9 species")
-    t0 = time.perf_counter()
-    case = cases.c1(mx, my, mz)
-    log(f"[bench] generated in {time.perf_counter() - t0:.1f} s")
-    if args.rhs_only:
-        if args.precision == "mixed":
-            case.params["precision"] = abi.MIXED
+    if args.rhs_only or args.steps_only:
+        log(f"[bench] generating workload {args.workload}")
+        case = cases.c1(mx, my, mz) if args.workload == "c1" else cases.c3()
+        case.params["precision"] = abi.MIXED if args.precision == "mixed" else abi.FP64
         g = Solver(case)
         g.upload_initial()
         g.init_conservative()
         if args.rhs_variant >= 0:
             g.set_rhs_variant(args.rhs_variant)
-        for _ in range(args.rhs_only):
-            g.rhs(-1)
+        if args.rhs_only:
+            for _ in range(args.rhs_only):
+                g.rhs(-1)
+        else:
+            g.set_execution(batched=True, graph=True)
+            g.autotune(reps=2)
+            g.step(args.steps_only)
         g.sync()
-        log(f"[bench] {args.rhs_only} RHS launches done ({args.precision})")
+        log(f"[bench] {args.workload} {args.precision}: profiling run done")
         g.close()
         return 0
-    variants = {}
-    for prec in ("fp64", "mixed"):
-        case.params["precision"] = abi.MIXED if prec == "mixed" else abi.FP64
-        variants[prec] = measure(case, args, prec)
-        if variants[prec] is None:
-            return 3
+
+    log(f"[bench] generating workload {mx}x{my}x{mz}, 9 species")
+    t0 = time.perf_counter()
+    case = cases.c1(mx, my, mz)
+    log(f"[bench] generated in {time.perf_counter() - t0:.1f} s")
+    forms = {}
+    # the plain FP64 drop-in: eager launches as the reference's Driver issues them
+    forms["fp64"] = measure(case, args, "fp64", "fp64", batched=False, graph=False)
+    forms["mixed"] = measure(case, args, "mixed", "mixed", batched=False, graph=False)
+    forms["mixed_graph_tuned"] = measure(case, args, "mixed_graph_tuned", "mixed", batched=True, graph=True,
+                                         autotune=True)
+    forms["fp64_graph_tuned"] = measure(case, args, "fp64_graph_tuned", "fp64", batched=True, graph=True,
+                                        autotune=True)
+    if not args.no_mlp:
+        forms["mixed_graph_tuned_mlp"] = measure(case, args, "mixed_graph_tuned_mlp", "mixed", batched=True,
+                                                 graph=True, autotune=True, mlp=True)
+    if any(v is None for v in forms.values()):
+        return 3
     log("[bench] mixed-precision error against the FP64 path")
-    errors = mixed_errors(case)
-    head = variants[args.precision]
-    other = variants["mixed" if args.precision == "fp64" else "fp64"]
+    errors = mixed_errors(case, mlp=not args.no_mlp)
+    c4_out = None
+    if not args.no_mlp and not args.no_c4:
+        log("[bench] generating the 384x256x256 surrogate workload")
+        c4 = cases.c1(384, 256, 256)
+        c4_forms = {
+            "fp64": measure(c4, args, "c4_fp64", "fp64", batched=False, graph=False, roofline=False),
+            "mixed_graph_tuned": measure(c4, args, "c4_mixed", "mixed", batched=True, graph=True, autotune=True,
+                                         roofline=False),
+            "mixed_graph_tuned_mlp": measure(c4, args, "c4_mixed_mlp", "mixed", batched=True, graph=True,
+                                             autotune=True, roofline=False, mlp=True),
+        }
+        if any(v is None for v in c4_forms.values()):
+            return 3
+        c4_out = {
+            "workload": "3-D block 384x256x256, 9 species, mixed precision, graph-captured, autotuned; "
+                        "Wilke vs surrogate transport",
+            "cells": c4.n_cells,
+            "forms": {k: {kk: v[kk] for kk in ("value", "ms_per_step", "kernel_ms_per_step") + tuple(
+                x for x in ("rhs_tiles", "mlp") if x in v)} for k, v in c4_forms.items()},
+            "surrogate_speedup_vs_wilke": round(c4_forms["mixed_graph_tuned_mlp"]["value"]
+                                                / c4_forms["mixed_graph_tuned"]["value"], 3),
+            "surrogate_props_ms_vs_wilke": [c4_forms["mixed_graph_tuned_mlp"]["kernel_ms_per_step"]["props"],
+                                            c4_forms["mixed_graph_tuned"]["kernel_ms_per_step"]["props"]],
+            "speedup_vs_fp64": round(c4_forms["mixed_graph_tuned_mlp"]["value"] / c4_forms["fp64"]["value"], 3),
+        }
+        del c4
+    c3_out = None
+    if not args.no_c3:
+        log("[bench] generating the 64-block lattice workload")
+        t0 = time.perf_counter()
+        c3 = cases.c3()
+        log(f"[bench] generated {c3.n_cells} points in {len(c3.blocks)} blocks in {time.perf_counter() - t0:.1f} s")
+        c3_forms = {
+            "fp64_per_block": measure(c3, args, "c3_fp64_per_block", "fp64", batched=False, graph=False,
+                                      roofline=False),
+            "fp64_batched_graph_tuned": measure(c3, args, "c3_fp64_batched", "fp64", batched=True, graph=True,
+                                                autotune=True, roofline=False),
+            "mixed_batched_graph_tuned": measure(c3, args, "c3_mixed_batched", "mixed", batched=True, graph=True,
+                                                 autotune=True, roofline=False),
+        }
+        if any(v is None for v in c3_forms.values()):
+            return 3
+        sizes = sorted({(b.mx, b.my, b.mz) for b in c3.blocks})
+        c3_out = {
+            "workload": f"{len(c3.blocks)} blocks on a 4x4x4 face-matched lattice, extents drawn from [16,160] "
+                        f"(smallest {sizes[0]}, largest {sizes[-1]}), {c3.n_cells} points, 9 species",
+            "cells": c3.n_cells,
+            "forms": {k: {kk: v[kk] for kk in ("value", "ms_per_step", "kernel_ms_per_step", "launches_per_step",
+                                               "execution") + (("rhs_tiles",) if "rhs_tiles" in v else ())}
+                      for k, v in c3_forms.items()},
+            "batched_speedup_vs_per_block_fp64": round(c3_forms["fp64_batched_graph_tuned"]["value"]
+                                                       / c3_forms["fp64_per_block"]["value"], 3),
+            "mixed_batched_speedup_vs_per_block_fp64": round(c3_forms["mixed_batched_graph_tuned"]["value"]
+                                                             / c3_forms["fp64_per_block"]["value"], 3),
+        }
+        del c3
+    head = forms["fp64"]
+    best = max((k for k in forms if k.startswith("mixed")), key=lambda k: forms[k]["value"])
     out = {
         "metric": "Mcell-stage updates/s, FP64 plain vs mixed+batched+surrogate; HBM GB/s",
         "value": head["value"],
@@ -221,18 +344,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "dtype": "f64",
         "data": "synthetic (seeded smooth T/p/u fields, smoothed Dirichlet mass fractions, 9 synthetic species)",
-        "config": {"workload": "explicit step, 3-D block 256x128x128, 9 species, Van Albada MUSCL, "
+        "config": {"workload": "plain FP64 explicit step, 3-D block 256x128x128, 9 species, Van Albada MUSCL, "
                                "AUSM+, viscous + Wilke, SSP-RK3 global CFL 0.5",
-                   "cells": case.n_cells, "variant": args.precision,
-                   "event_ms_per_step": head["event_ms_per_step"]},
+                   "cells": case.n_cells, "event_ms_per_step": head["event_ms_per_step"]},
         "roofline": head["roofline"],
         "kernel_ms_per_step": head["kernel_ms_per_step"],
         "variants": {
-            other["name"]: {k: other[k] for k in ("value", "ms_per_step", "kernel_ms_per_step")},
-            "mixed_speedup_vs_fp64": round(variants["mixed"]["value"] / variants["fp64"]["value"], 3),
+            **{k: {kk: v[kk] for kk in ("value", "ms_per_step", "kernel_ms_per_step") + tuple(
+                x for x in ("rhs_tiles", "mlp") if x in v)} for k, v in forms.items() if k != "fp64"},
+            "changed_form_speedup_vs_fp64": {k: round(forms[k]["value"] / head["value"], 3)
+                                             for k in forms if k != "fp64"},
+            "best_changed_form": best,
             "mixed_error_vs_fp64": errors,
+            "c3_batched": c3_out,
+            "c4_surrogate": c4_out,
         },
         "dt": head["dt"],
         "residual": head["residual"],

@@ -20,7 +20,9 @@ All accept reduced sizes for tests.
 """
 from __future__ import annotations
 
+import concurrent.futures as cf
 import math
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -230,40 +232,53 @@ def smooth_field(seed, stream, shape_idx, lengths):
     return 0.25 * s
 
 
+def _box_filter(a, passes):
+    for _ in range(passes):
+        for ax in (0, 1, 2):
+            ap = np.concatenate([np.take(a, [0], axis=ax), a, np.take(a, [-1], axis=ax)], axis=ax)
+            sl = [slice(None)] * 3
+            parts = []
+            for o in range(3):
+                sl[ax] = slice(o, o + a.shape[ax])
+                parts.append(ap[tuple(sl)])
+            a = (parts[0] + parts[1] + parts[2]) / 3.0
+    return a
+
+
+def _threads(n):
+    return max(1, min(n, os.cpu_count() or 1, 8))
+
+
 def smoothed_dirichlet(seed, stream, n_spec, shape, passes=4):
     """Per point Dirichlet(1) mass fractions (normalised exponentials), then
     `passes` separable 3-point box filters (edge-replicated), renormalised.
-    Species are drawn and filtered one at a time (the filter is linear per
-    species), which bounds the temporaries on large grids."""
+    Species are drawn and filtered independently (the filter is linear per
+    species), in parallel threads on large grids."""
     n = int(np.prod(shape))
     y = np.empty((n_spec,) + tuple(shape))
-    for l in range(n_spec):
+
+    def draw(l):
         y[l] = -np.log1p(-uniform(seed, stream, n, offset=l * n)).reshape(shape)
-    y /= y.sum(axis=0, keepdims=True)
-    for l in range(n_spec):
-        a = y[l]
-        for _ in range(passes):
-            for ax in (0, 1, 2):
-                ap = np.concatenate([np.take(a, [0], axis=ax), a, np.take(a, [-1], axis=ax)], axis=ax)
-                sl = [slice(None)] * 3
-                parts = []
-                for o in range(3):
-                    sl[ax] = slice(o, o + a.shape[ax])
-                    parts.append(ap[tuple(sl)])
-                a = (parts[0] + parts[1] + parts[2]) / 3.0
-        y[l] = a
+
+    def smooth(l):
+        y[l] = _box_filter(y[l], passes)
+
+    with cf.ThreadPoolExecutor(_threads(n_spec)) as ex:
+        list(ex.map(draw, range(n_spec)))
+        y /= y.sum(axis=0, keepdims=True)
+        list(ex.map(smooth, range(n_spec)))
     return y / y.sum(axis=0, keepdims=True)
 
 
 def _smooth_fields(seed, n_spec, sp, idx, L):
     """The c1 field generator on index coordinates idx = (i, j, k) arrays."""
     ii, jj, kk = idx
-    T = 1400.0 + 1100.0 * smooth_field(seed, 10, (ii, jj, kk), L)
-    p = 101325.0 * (1.0 + 0.1 * smooth_field(seed, 11, (ii, jj, kk), L))
+    with cf.ThreadPoolExecutor(_threads(5)) as ex:
+        s = list(ex.map(lambda st: smooth_field(seed, st, (ii, jj, kk), L), (10, 11, 12, 13, 14)))
+    T = 1400.0 + 1100.0 * s[0]
+    p = 101325.0 * (1.0 + 0.1 * s[1])
     amp = 200.0 / math.sqrt(3.0)
-    u = amp * smooth_field(seed, 12, (ii, jj, kk), L)
-    v = amp * smooth_field(seed, 13, (ii, jj, kk), L)
-    w = amp * smooth_field(seed, 14, (ii, jj, kk), L)
+    u, v, w = amp * s[2], amp * s[3], amp * s[4]
     if sp is not None:
         Y = smoothed_dirichlet(seed, 20, n_spec, ii.shape)
         r_mix = R_U * np.einsum("l...,l->...", Y, 1.0 / sp["mw"])
@@ -375,7 +390,7 @@ def lattice(sizes, n_spec=9, seed=11, dx=1e-3, cfl=0.5, rk_stages=3, limiter=1, 
     return Case(name, params, sp, blocks, [], [], bvs, yks)
 
 
-def c3(nb=(4, 4, 4), lo=16, hi=160, seed=5, n_spec=9, **kw):
+def c3(nb=(4, 4, 4), lo=16, hi=160, seed=2, n_spec=9, **kw):
     """64 blocks (4 x 4 x 4 lattice) with extents drawn from [16, 160]: about
     40M points, the c1 species and field generator."""
     return lattice(lattice_sizes(nb, lo, hi, seed), n_spec=n_spec, name="c3", **kw)

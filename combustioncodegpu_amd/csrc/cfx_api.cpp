@@ -10,131 +10,11 @@
 #include <vector>
 
 #include "../../include/cfx.h"
-#include "cfx_types.h"
+#include "context.h"
 #include "kernels.h"
 #include "mesh.h"
 
 using namespace cfx;
-
-namespace {
-
-enum KClass { K_RHS = 0, K_UPDATE, K_BC, K_PROPS, K_DT, K_OTHER, K_NCLASS };
-
-struct BcOp {
-  int type, blk, face;
-};
-
-struct TimedLaunch {
-  hipEvent_t a, b;
-  int cls;
-};
-
-// A workgroup prefix over the blocks (nb+1 entries), device and host copies.
-struct Table {
-  int *dev = nullptr;
-  std::vector<int> host;
-};
-
-// Ghost-fill ops grouped into levels: the ops of one level touch disjoint
-// data and run as one launch; lvl[L]..lvl[L+1] are the ops of level L.
-struct Schedule {
-  FaceOp *ops = nullptr;
-  int *off = nullptr;
-  std::vector<int> off_host;
-  std::vector<int> lvl;
-  int levels() const { return lvl.empty() ? 0 : (int)lvl.size() - 1; }
-};
-
-}  // namespace
-
-struct cfx_ctx {
-  SchemeDev S{};
-  int ns = 0, nv = 5, dim = 3;
-  int rk = 3;
-  std::vector<HostBlock> hb;
-  std::vector<BlockView> views;
-  std::vector<void *> allocs;
-  SpeciesDev sp_host{};
-  SpeciesDev *sp_dev = nullptr;
-  std::vector<InflowDev> inflow_host;
-  InflowDev *inflow_dev = nullptr;
-  std::vector<WallDev> walls;
-  std::vector<BcOp> bc_order;
-  hipStream_t stream = nullptr;
-  double *d_dt = nullptr;  // device dt; its bits are reduced with atomicMin
-  double *d_res = nullptr;
-  int *d_flags = nullptr;
-  unsigned long long *d_newton = nullptr;
-  unsigned long long *d_minscratch = nullptr;  // min target of non-global dt passes
-  double *d_partial = nullptr;
-  int n_partial = 0;
-  long long n_grid_total = 0;
-  BlockView *views_dev = nullptr;
-  Table t_pad, t_int, t_dt, t_last, t_res;     // point-kernel workgroup prefixes
-  Table t_rhs[3][kNVar3];                      // pencil tiles per direction and variant
-  int occ[2][3][kNVar3] = {};                  // resident workgroups per CU [fp32][dir][variant]
-  int n_cu = 256;
-  int rhs_var[3] = {0, 0, 0}, rhs_persist[3] = {0, 0, 0};
-  std::vector<FaceOp> ops;                     // BC ops in application order, then exchange ops
-  int n_bc_ops = 0;
-  Schedule sched_seq, sched_bc, sched_ex, sched_all;
-  bool batched = true, use_graph = false;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
-  bool mixed = false;
-  bool profiling = false;
-  std::vector<TimedLaunch> timed;
-  std::vector<hipEvent_t> event_pool;
-  double class_ms[K_NCLASS] = {0};
-  long long class_n[K_NCLASS] = {0};
-  std::string err;
-
-  int fail(int rc, const std::string &msg) {
-    err = msg;
-    if (rc > 0) err += std::string(": ") + hipGetErrorString((hipError_t)rc);
-    return rc;
-  }
-  template <class T>
-  T *dalloc(size_t n, int &rc) {
-    void *p = nullptr;
-    const hipError_t e = hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T));
-    if (e != hipSuccess) {
-      rc = (int)e;
-      return nullptr;
-    }
-    allocs.push_back(p);
-    return (T *)p;
-  }
-  Batch all(const Table &t) const { return Batch{views_dev, t.dev, (int)views.size(), t.host.back() - t.host[0]}; }
-  Batch one(const Table &t, int b) const { return Batch{views_dev + b, t.dev + b, 1, t.host[b + 1] - t.host[b]}; }
-  void drop_graph() {
-    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    graph_exec = nullptr;
-    graph = nullptr;
-  }
-  hipEvent_t get_event() {
-    if (!event_pool.empty()) {
-      hipEvent_t e = event_pool.back();
-      event_pool.pop_back();
-      return e;
-    }
-    hipEvent_t e;
-    (void)hipEventCreate(&e);
-    return e;
-  }
-  // brackets a launch with events when profiling
-  template <class F>
-  int timed_launch(int cls, F &&f) {
-    if (!profiling) return f();
-    TimedLaunch t{get_event(), get_event(), cls};
-    (void)hipEventRecord(t.a, stream);
-    const int rc = f();
-    (void)hipEventRecord(t.b, stream);
-    timed.push_back(t);
-    return rc;
-  }
-};
 
 namespace {
 
@@ -275,11 +155,6 @@ int build_inflow(const cfx_inflow &in, const SpeciesT<double> &s, const SchemeDe
   return 0;
 }
 
-#define CFX_TRY(expr)                  \
-  do {                                 \
-    const int rc_ = (expr);            \
-    if (rc_ != 0) return ctx->fail(rc_, #expr); \
-  } while (0)
 
 // ---------------------------------------------------------------- tables ---
 int upload_table(cfx_ctx *ctx, Table &t, const std::vector<int> &counts) {
@@ -560,6 +435,16 @@ int do_elim(cfx_ctx *ctx) {
 }
 
 int do_props(cfx_ctx *ctx) {
+  if (ctx->transport == CFX_TRANSPORT_MLP) {
+    if (!ctx->mlp_ready) return ctx->fail(CFX_E_SURROGATE, "surrogate transport selected but not trained or loaded");
+    auto fn = g_kernels32[ctx->ns].props_mlp;
+    CFX_TRY(per_batch(ctx, ctx->t_pad, [&](const Batch &b) {
+      return ctx->timed_launch(K_PROPS, [&] {
+        return fn(b, ctx->sp_dev, ctx->S, ctx->mlp_dev, ctx->mlp_ops, ctx->mlp_bias, ctx->mlp_counters, ctx->stream);
+      });
+    }));
+    return 0;
+  }
   auto fn = ctx->mixed ? g_kernels32[ctx->ns].props : g_kernels64[ctx->ns].props;
   CFX_TRY(per_batch(ctx, ctx->t_pad, [&](const Batch &b) {
     return ctx->timed_launch(K_PROPS, [&] { return fn(b, ctx->sp_dev, ctx->S, ctx->stream); });
@@ -649,9 +534,16 @@ int cfx_create(const cfx_params *params, const cfx_species *species, const cfx_b
     return bail(CFX_E_UNSUPPORTED);
   }
   if (params->inviscid_scheme != 3 && params->inviscid_scheme != 0) return bail(CFX_E_UNSUPPORTED);
-  if ((params->precision != CFX_FP64 && params->precision != CFX_MIXED) || params->transport != CFX_TRANSPORT_WILKE)
+  if ((params->precision != CFX_FP64 && params->precision != CFX_MIXED) ||
+      (params->transport != CFX_TRANSPORT_WILKE && params->transport != CFX_TRANSPORT_MLP))
     return bail(CFX_E_UNSUPPORTED);
   ctx->mixed = params->precision == CFX_MIXED;
+  if (params->transport == CFX_TRANSPORT_MLP &&
+      (!ctx->mixed || ctx->ns < 1 || ctx->ns > kMlpMaxSp || !g_kernels32[ctx->ns].props_mlp)) {
+    ctx->err = "surrogate transport needs the mixed-precision path and 1..14 species";
+    return bail(CFX_E_UNSUPPORTED);
+  }
+  ctx->transport = params->transport;
   if (ctx->mixed && !g_kernels32[ctx->ns].present) {
     ctx->err = "no FP32 kernels built for " + std::to_string(ctx->ns) + " species";
     return bail(CFX_E_UNSUPPORTED);

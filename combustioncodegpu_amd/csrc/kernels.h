@@ -75,11 +75,19 @@ struct KernelSet64 {
                   const SpeciesDev *sp, const SchemeDev &S, const InflowDev *inflow, hipStream_t st);
   RhsEntry rhs3[3][kNVar3];
   RhsEntry rhs2[2][kNVar2];
+  // surrogate training set: n samples of (T, log p, Y) and log(mu, kappa,
+  // rho*D) from the FP64 transport; lim = t_lo, t_hi, log p_lo, log p_hi
+  int (*mlp_samples)(const SpeciesDev *sp, const SchemeDev &S, long long n, unsigned long long seed,
+                     const double lim[4], double alpha, double *x, double *y, hipStream_t st);
 };
 
+struct MlpDev;
 struct KernelSet32 {
   int present;
   int (*props)(const Batch &b, const SpeciesDev *sp, const SchemeDev &S, hipStream_t st);
+  // properties with the surrogate transport (mixed precision)
+  int (*props_mlp)(const Batch &b, const SpeciesDev *sp, const SchemeDev &S, const MlpDev *mlp, const void *ops,
+                   const float *bias, unsigned long long *counters, hipStream_t st);
   RhsEntry rhs3[3][kNVar3];
   RhsEntry rhs2[2][kNVar2];
 };

@@ -7,6 +7,7 @@
 
 #include "device_physics.h"
 #include "kernels.h"
+#include "mlp.h"
 #include "rhs_pencil.h"
 
 #ifndef CFX_NS
@@ -50,12 +51,72 @@ __device__ __forceinline__ PointWork point_work(const int *__restrict__ off, int
 }
 
 // ------------------------------------------------------------ properties ---
-// update_physical_properties (Field.cu:270-302) and
-// compute_transport_property (Transport.cu:206-259) over the padded range.
-// The Wilke sums use s_l = sqrt(mu_l) so that sqrt(mu_m/mu_n) = s_m/s_n
-// (NS square roots instead of NS^2) and T*^-0.145 = exp(-0.145 log T*) with
-// one shared log T.  In R = float (mixed precision) the kernel also writes
-// the FP32 copies of the primitives the FP32 RHS reads.
+// update_physical_properties (Field.cu:270-302): mixture cp, gamma and speed
+// of sound at temperature t and mass fractions Y; cp_l and sum Y_l/W_l are
+// returned for the transport.
+template <typename R>
+__device__ __forceinline__ void mixture_gamma_c(SpeciesPtrT<R> sp, R t, const R *Y, R *cp, R &winv, R *out) {
+  R cpt = 0;
+  winv = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    cp[l] = species_cp<R>(sp, l, t);
+    winv += Y[l] * R(sp->inv_mw[l]);
+    cpt += Y[l] * cp[l];
+  }
+  const R Ru = R(kRu);
+  const R gam = cpt / (cpt - Ru * winv);
+  out[0] = gam;
+  out[1] = sqrt(gam * Ru * t * winv);
+}
+
+// compute_transport_property (Transport.cu:206-259): out = mu, kappa,
+// rho*D_1..ns.  The Wilke sums use s_l = sqrt(mu_l) so that sqrt(mu_m/mu_n)
+// = s_m/s_n (NS square roots instead of NS^2) and T*^-0.145 =
+// exp(-0.145 log T*) with one shared log T.
+template <typename R>
+__device__ __forceinline__ void wilke_point(SpeciesPtrT<R> sp, const SchemeDev &S, R t, const R *Y, const R *cp,
+                                            R winv, R *out) {
+  // Chapman-Enskog species viscosity (Transport.cu:215-220)
+  const R lt = log(t), st = sqrt(t);
+  R mu_s[NSA], X[NSA], s[NSA], rs[NSA];
+  const R imw = R(1) / winv;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    X[l] = Y[l] * imw * R(sp->inv_mw[l]);
+    const R tdl = t * R(sp->lj_inv[l]);
+    const R a = tdl + R(0.5);
+    const R omega = R(1.147) * exp(R(-0.145) * (lt + R(sp->log_lj_inv[l]))) + R(1) / (a * a);
+    mu_s[l] = R(sp->vis_coeff[l]) * st / omega;
+    s[l] = sqrt(mu_s[l]);
+    rs[l] = R(1) / s[l];
+  }
+  // Wilke mixing and Mason-Saxena conductivity (Transport.cu:221-246)
+  R mu = 0, kappa = 0;
+#pragma unroll
+  for (int m = 0; m < NS; ++m) {
+    R vt = X[m];
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      if (n == m) continue;
+      const R q = R(1) + s[m] * rs[n] * R(sp->wj4[m][n]);
+      vt += q * q * R(sp->sqw8[m][n]) * X[n];
+    }
+    const R lam = mu_s[m] * (cp[m] + R(sp->cond_add[m]));
+    mu += mu_s[m] * X[m] / vt;
+    kappa += lam * X[m] / (R(1.065) * vt - R(0.065) * X[m]);
+  }
+  out[0] = mu;
+  out[1] = kappa;
+  // constant-Schmidt diffusivity (Transport.cu:249-255, X[l] in the test)
+  const R mus = mu * R(S.inv_sc);
+#pragma unroll
+  for (int l = 0; l < NS; ++l)
+    out[2 + l] = fabs(X[l] - R(1)) < R(1e-3) ? mus : (R(1) - Y[l]) * mu / ((R(1) - X[l]) * R(S.sc));
+}
+
+// Properties over the padded range.  In R = float (mixed precision) the
+// kernel also writes the FP32 copies of the primitives the FP32 RHS reads.
 template <typename R>
 __global__ void __launch_bounds__(kTpb) props_kernel(const BlockView *__restrict__ views, const int *__restrict__ off,
                                                      int nb, const SpeciesDev *__restrict__ spg, SchemeDev S) {
@@ -68,6 +129,7 @@ __global__ void __launch_bounds__(kTpb) props_kernel(const BlockView *__restrict
   if (p >= np) return;
   const R t = R(B.bv[5 * np + p]);
   R out[4 + (NS > 0 ? NS : 0)];
+  R Y[NSA];
   if constexpr (NS == 0) {
     const R c_temp = R(S.gamma_air * S.r_air);
     out[0] = R(S.gamma_air);
@@ -76,63 +138,25 @@ __global__ void __launch_bounds__(kTpb) props_kernel(const BlockView *__restrict
     out[2] = mu;
     out[3] = mu * c_temp / R(S.gamma_air - 1) * R(S.inv_pr);
   } else {
-    R Y[NSA], cp[NSA];
-    R winv = 0, cpt = 0;
+    R cp[NSA], winv;
 #pragma unroll
-    for (int l = 0; l < NS; ++l) {
-      Y[l] = R(B.yk[l * np + p]);
-      cp[l] = species_cp<R>(sp, l, t);
-      winv += Y[l] * R(sp->inv_mw[l]);
-      cpt += Y[l] * cp[l];
-    }
-    const R Ru = R(kRu);
-    const R gam = cpt / (cpt - Ru * winv);
-    out[0] = gam;
-    out[1] = sqrt(gam * Ru * t * winv);
-    // Chapman-Enskog species viscosity (Transport.cu:215-220)
-    const R lt = log(t), st = sqrt(t);
-    R mu_s[NSA], X[NSA], s[NSA], rs[NSA];
-    const R imw = R(1) / winv;
-#pragma unroll
-    for (int l = 0; l < NS; ++l) {
-      X[l] = Y[l] * imw * R(sp->inv_mw[l]);
-      const R tdl = t * R(sp->lj_inv[l]);
-      const R a = tdl + R(0.5);
-      const R omega = R(1.147) * exp(R(-0.145) * (lt + R(sp->log_lj_inv[l]))) + R(1) / (a * a);
-      mu_s[l] = R(sp->vis_coeff[l]) * st / omega;
-      s[l] = sqrt(mu_s[l]);
-      rs[l] = R(1) / s[l];
-    }
-    // Wilke mixing and Mason-Saxena conductivity (Transport.cu:221-246)
-    R mu = 0, kappa = 0;
-#pragma unroll
-    for (int m = 0; m < NS; ++m) {
-      R vt = X[m];
-#pragma unroll
-      for (int n = 0; n < NS; ++n) {
-        if (n == m) continue;
-        const R q = R(1) + s[m] * rs[n] * R(sp->wj4[m][n]);
-        vt += q * q * R(sp->sqw8[m][n]) * X[n];
-      }
-      const R lam = mu_s[m] * (cp[m] + R(sp->cond_add[m]));
-      mu += mu_s[m] * X[m] / vt;
-      kappa += lam * X[m] / (R(1.065) * vt - R(0.065) * X[m]);
-    }
-    out[2] = mu;
-    out[3] = kappa;
-    // constant-Schmidt diffusivity (Transport.cu:249-255, X[l] in the test)
-    const R mus = mu * R(S.inv_sc);
-#pragma unroll
-    for (int l = 0; l < NS; ++l)
-      out[4 + l] = fabs(X[l] - R(1)) < R(1e-3) ? mus : (R(1) - Y[l]) * mu / ((R(1) - X[l]) * R(S.sc));
+    for (int l = 0; l < NS; ++l) Y[l] = R(B.yk[l * np + p]);
+    mixture_gamma_c<R>(sp, t, Y, cp, winv, out);
+    wilke_point<R>(sp, S, t, Y, cp, winv, out + 2);
   }
   if constexpr (f32) {
+    // loads first (T and Y are already held), then the stores
+    float b32[6];
+#pragma unroll
+    for (int v = 0; v < 6; ++v) b32[v] = float(B.bv[v * np + p]);
 #pragma unroll
     for (int v = 0; v < 4 + NS; ++v) B.props32[v * np + p] = out[v];
 #pragma unroll
-    for (int v = 0; v < 6; ++v) B.bv32[v * np + p] = float(B.bv[v * np + p]);
+    for (int v = 0; v < 6; ++v) B.bv32[v * np + p] = b32[v];
+    if constexpr (NS > 0) {
 #pragma unroll
-    for (int l = 0; l < NS; ++l) B.yk32[l * np + p] = float(B.yk[l * np + p]);
+      for (int l = 0; l < NS; ++l) B.yk32[l * np + p] = float(Y[l]);
+    }
   } else {
 #pragma unroll
     for (int v = 0; v < 4 + NS; ++v) B.props[v * np + p] = out[v];
@@ -235,26 +259,36 @@ __global__ void __launch_bounds__(kTpb) update_kernel(const BlockView *__restric
   const double dt = *dt_dev;
   const bool rel = (S.quirks & CFX_Q_NEWTON_REL) != 0;
   const long long p = interior_pidx(B, q);
-  const double dtj = (S.rk == 3 ? dt : B.dt_local[q]) / B.jac[p];
+  // every load is issued before the first store: the stores go through
+  // pointers the compiler must assume alias the loaded arrays
+  double cur[NV], dqv[NV], qnv[NV];
+#pragma unroll
+  for (int l = 0; l < NV; ++l) cur[l] = B.cv[l * np + p];
+  if (B.mixed) {
+#pragma unroll
+    for (int l = 0; l < NV; ++l) dqv[l] = (double)B.dq32[l * ni + q];
+  } else {
+#pragma unroll
+    for (int l = 0; l < NV; ++l) dqv[l] = B.dq[l * ni + q];
+  }
+  const bool rk3 = S.rk == 3;
+  if (rk3 && stage > 0) {
+#pragma unroll
+    for (int l = 0; l < NV; ++l) qnv[l] = B.qn[l * ni + q];
+  } else {
+#pragma unroll
+    for (int l = 0; l < NV; ++l) qnv[l] = cur[l];
+  }
+  const double t_old = B.bv[5 * np + p];
+  const double dtj = (rk3 ? dt : B.dt_local[q]) / B.jac[p];
   double c[NV];
 #pragma unroll
-  for (int l = 0; l < NV; ++l) {
-    const double cur = B.cv[l * np + p];
-    const double dq = B.mixed ? (double)B.dq32[l * ni + q] : B.dq[l * ni + q];
-    if (S.rk == 3) {
-      double qn;
-      if (stage == 0) {
-        qn = cur;
-        B.qn[l * ni + q] = cur;
-      } else {
-        qn = B.qn[l * ni + q];
-      }
-      c[l] = A * qn + Bc * (cur + dq * dtj);
-    } else {
-      c[l] = cur + dq * dtj;
-    }
-  }
+  for (int l = 0; l < NV; ++l) c[l] = rk3 ? A * qnv[l] + Bc * (cur[l] + dqv[l] * dtj) : cur[l] + dqv[l] * dtj;
   if (B.mz == 1) c[3] = 0.0;
+  if (rk3 && stage == 0) {
+#pragma unroll
+    for (int l = 0; l < NV; ++l) B.qn[l * ni + q] = cur[l];
+  }
 #pragma unroll
   for (int l = 0; l < NV; ++l) B.cv[l * np + p] = c[l];
   const double rho = c[0];
@@ -273,7 +307,7 @@ __global__ void __launch_bounds__(kTpb) update_kernel(const BlockView *__restric
     }
     const double R = kRu * winv;
     const double e = c[4] / rho - 0.5 * v2;
-    T = B.bv[5 * np + p];
+    T = t_old;
     double err = 1e30;
     int it = 0;
     const double eps = rel ? 1e-3 : 1e-10;
@@ -574,9 +608,89 @@ int launch_face_ops(const BlockView *views, const FaceOp *ops, const int *off, i
   return (int)hipGetLastError();
 }
 
+// ------------------------------------------------- surrogate training set ---
+// Counter-based uniform in [0, 1): SplitMix64 finaliser of (seed, sample, k).
+__device__ __forceinline__ double rng_uniform(unsigned long long seed, long long s, unsigned k) {
+  unsigned long long z = seed + 0x9E3779B97F4A7C15ull * ((unsigned long long)s * 64ull + k + 1ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// Gamma(a, 1) by Marsaglia & Tsang (2000), "A simple method for generating
+// gamma variables", with the boost G(a) = G(a+1) U^(1/a) for a < 1; normals
+// by Box-Muller.  Bounded: after 32 rejections the mode is returned.
+__device__ __forceinline__ double rng_gamma(unsigned long long seed, long long s, unsigned &k, double a) {
+  const double aa = a < 1.0 ? a + 1.0 : a;
+  const double d = aa - 1.0 / 3.0, c = 1.0 / sqrt(9.0 * d);
+  double g = d;
+  for (int it = 0; it < 32; ++it) {
+    const double u1 = rng_uniform(seed, s, k++), u2 = rng_uniform(seed, s, k++);
+    const double z = sqrt(-2.0 * log(1.0 - u1)) * cos(6.283185307179586 * u2);
+    const double v1 = 1.0 + c * z;
+    if (v1 <= 0.0) continue;
+    const double v = v1 * v1 * v1;
+    const double u = rng_uniform(seed, s, k++);
+    if (log(1.0 - u) < 0.5 * z * z + d - d * v + d * log(v)) {
+      g = d * v;
+      break;
+    }
+  }
+  if (a < 1.0) g *= pow(1.0 - rng_uniform(seed, s, k++), 1.0 / a);
+  return g;
+}
+
+// Sample s: T ~ U[t_lo, t_hi], log p ~ U[lp_lo, lp_hi], Y ~ Dirichlet(alpha);
+// inputs x = (T, log p, Y) and targets y = log(mu, kappa, rho*D_l) of the
+// FP64 transport routine.
+__global__ void __launch_bounds__(256) mlp_samples_kernel(const SpeciesDev *__restrict__ spg, SchemeDev S,
+                                                          long long n, unsigned long long seed, double t_lo,
+                                                          double t_hi, double lp_lo, double lp_hi, double alpha,
+                                                          double *__restrict__ x, double *__restrict__ y) {
+  const SpeciesPtrT<double> sp = const_table<double>(spg);
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  unsigned k = 0;
+  const double t = t_lo + (t_hi - t_lo) * rng_uniform(seed, s, k++);
+  const double lp = lp_lo + (lp_hi - lp_lo) * rng_uniform(seed, s, k++);
+  double Y[NSA], sum = 0.0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    Y[l] = rng_gamma(seed, s, k, alpha);
+    sum += Y[l];
+  }
+  const double inv = sum > 0.0 ? 1.0 / sum : 0.0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) Y[l] = sum > 0.0 ? Y[l] * inv : 1.0 / NS;
+  double cp[NSA], winv, gc[2], out[2 + NSA];
+  mixture_gamma_c<double>(sp, t, Y, cp, winv, gc);
+  wilke_point<double>(sp, S, t, Y, cp, winv, out);
+  double *xr = x + s * (NS + 2), *yr = y + s * (NS + 2);
+  xr[0] = t;
+  xr[1] = lp;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) xr[2 + l] = Y[l];
+#pragma unroll
+  for (int o = 0; o < NS + 2; ++o) yr[o] = log(out[o]);
+}
+
+int launch_mlp_samples(const SpeciesDev *sp, const SchemeDev &S, long long n, unsigned long long seed,
+                       const double lim[4], double alpha, double *x, double *y, hipStream_t st) {
+  if constexpr (NS == 0) {
+    return CFX_E_UNSUPPORTED;
+  } else {
+    if (n <= 0) return 0;
+    mlp_samples_kernel<<<(int)((n + 255) / 256), 256, 0, st>>>(sp, S, n, seed, lim[0], lim[1], lim[2], lim[3],
+                                                                alpha, x, y);
+    return (int)hipGetLastError();
+  }
+}
+
 struct Register64 {
   Register64() {
     KernelSet64 &k = g_kernels64[NS];
+    k.mlp_samples = launch_mlp_samples;
     k.props = launch_props<double>;
     k.cv_from_bv = launch_cv_from_bv;
     k.update = launch_update;
@@ -588,9 +702,105 @@ struct Register64 {
 
 #else  // CFX_PREC == 32
 
+// ----------------------------------------------- properties, MLP transport ---
+// The mixed-precision properties with mu, kappa and rho*D from the surrogate.
+// Each wavefront evaluates its 64 points as four 16-point MFMA tiles (inputs
+// and outputs pass through LDS between the point-per-lane and the tile
+// layouts).  Points outside the training envelope take the Wilke routine;
+// one wavefront in check_every also evaluates Wilke at all its points and
+// keeps Wilke where the surrogate is off by more than check_tol.
+// counters: [0] points outside, [1] points checked, [2] checks failed.
+__global__ void __launch_bounds__(kTpb) props_mlp_kernel(const BlockView *__restrict__ views,
+                                                         const int *__restrict__ off, int nb,
+                                                         const SpeciesDev *__restrict__ spg, SchemeDev S,
+                                                         const MlpDev *__restrict__ mlp,
+                                                         const bf16x8 *__restrict__ ops,
+                                                         const float *__restrict__ bias,
+                                                         unsigned long long *__restrict__ counters) {
+  __shared__ float xl[kTpb / 64][64][kMlpIO + 1];
+  __shared__ float yl[kTpb / 64][64][kMlpIO + 1];
+  const SpeciesPtrT<float> sp = const_table<float>(spg);
+  const PointWork pw = point_work(off, nb);
+  const BlockView B = views[pw.blk];
+  const long long np = B.np;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+  const long long p0 = pw.base + threadIdx.x;
+  const bool valid = p0 < np;
+  const long long p = valid ? p0 : np - 1;  // every lane takes part in the MFMA tiles
+  const float t = float(B.bv[5 * np + p]);
+  const float lp = logf(float(B.bv[4 * np + p]));
+  float Y[NSA], cp[NSA], winv, out[4 + NSA];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) Y[l] = float(B.yk[l * np + p]);
+  mixture_gamma_c<float>(sp, t, Y, cp, winv, out);
+  float x[kMlpIO];
+  const bool outside = mlp_inputs<NS>(mlp, t, lp, Y, x);
+#pragma unroll
+  for (int f = 0; f < kMlpIO; ++f) xl[wv][lane][f] = x[f];
+  __syncthreads();
+#pragma unroll
+  for (int tt = 0; tt < 4; ++tt) {
+    const int q = 16 * tt + c;
+    bf16x8 xo;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) xo[j] = (__bf16)xl[wv][q][4 * g + j], xo[4 + j] = (__bf16)0.0f;
+    const f32x4 yv = mlp_infer_tile(ops, bias, xo);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) yl[wv][q][4 * g + r] = yv[r];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int o = 0; o < NS + 2; ++o) out[2 + o] = __expf(yl[wv][lane][o] * mlp->t_std[o] + mlp->t_mean[o]);
+  const int gw = blockIdx.x * (kTpb / 64) + wv;
+  const bool check = (gw % mlp->check_every) == 0;
+  bool fail = false;
+  if (check || outside) {
+    float w[2 + NSA];
+    wilke_point<float>(sp, S, t, Y, cp, winv, w);
+    if (check) {
+#pragma unroll
+      for (int o = 0; o < NS + 2; ++o) fail = fail || !(fabsf(out[2 + o] - w[o]) <= mlp->check_tol * fabsf(w[o]));
+    }
+    if (outside || fail) {
+#pragma unroll
+      for (int o = 0; o < NS + 2; ++o) out[2 + o] = w[o];
+    }
+  }
+  const unsigned long long n_out = __ballot(valid && outside), n_chk = __ballot(valid && check),
+                           n_fail = __ballot(valid && fail);
+  if (lane == 0) {
+    if (n_out) atomicAdd(&counters[0], (unsigned long long)__popcll(n_out));
+    if (n_chk) atomicAdd(&counters[1], (unsigned long long)__popcll(n_chk));
+    if (n_fail) atomicAdd(&counters[2], (unsigned long long)__popcll(n_fail));
+  }
+  if (!valid) return;
+  float b32[6];
+#pragma unroll
+  for (int v = 0; v < 6; ++v) b32[v] = float(B.bv[v * np + p]);
+#pragma unroll
+  for (int v = 0; v < 4 + NS; ++v) B.props32[v * np + p] = out[v];
+#pragma unroll
+  for (int v = 0; v < 6; ++v) B.bv32[v * np + p] = b32[v];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) B.yk32[l * np + p] = Y[l];
+}
+
+int launch_props_mlp(const Batch &b, const SpeciesDev *sp, const SchemeDev &S, const MlpDev *mlp, const void *ops,
+                     const float *bias, unsigned long long *counters, hipStream_t st) {
+  if constexpr (NS == 0 || NS > kMlpMaxSp) {
+    return CFX_E_UNSUPPORTED;
+  } else {
+    if (b.total <= 0) return 0;
+    props_mlp_kernel<<<b.total, kTpb, 0, st>>>(b.views, b.off, b.nb, sp, S, mlp, (const bf16x8 *)ops, bias,
+                                              counters);
+    return (int)hipGetLastError();
+  }
+}
+
 struct Register32 {
   Register32() {
     KernelSet32 &k = g_kernels32[NS];
+    k.props_mlp = launch_props_mlp;
     k.props = launch_props<float>;
     register_rhs<float>(k);
     k.present = 1;

@@ -9,6 +9,8 @@ from __future__ import annotations
 import ctypes as C
 import os
 
+import numpy as np
+
 from . import abi
 from .driver import StepContext
 
@@ -56,6 +58,27 @@ def load():
     lib.cfx_set_execution.restype = C.c_int
     lib.cfx_schedule_info.argtypes = [vp, ip, ip]
     lib.cfx_schedule_info.restype = C.c_int
+    fp = C.POINTER(C.c_float)
+    lib.cfx_mlp_default_config.argtypes = [C.POINTER(abi.CfxMlpConfig)]
+    lib.cfx_mlp_default_config.restype = None
+    lib.cfx_mlp_train.argtypes = [vp, C.POINTER(abi.CfxMlpConfig), C.POINTER(abi.CfxMlpReport)]
+    lib.cfx_mlp_train.restype = C.c_int
+    lib.cfx_mlp_n_params.argtypes = [vp]
+    lib.cfx_mlp_n_params.restype = C.c_int
+    lib.cfx_mlp_set_params.argtypes = [vp, fp, fp]
+    lib.cfx_mlp_set_params.restype = C.c_int
+    lib.cfx_mlp_get_params.argtypes = [vp, fp, fp]
+    lib.cfx_mlp_get_params.restype = C.c_int
+    lib.cfx_mlp_set_check.argtypes = [vp, C.c_double, C.c_int]
+    lib.cfx_mlp_set_check.restype = C.c_int
+    lib.cfx_mlp_gradient.argtypes = [vp, fp, fp, C.c_int, fp, dp]
+    lib.cfx_mlp_gradient.restype = C.c_int
+    lib.cfx_mlp_predict.argtypes = [vp, fp, C.c_int, fp]
+    lib.cfx_mlp_predict.restype = C.c_int
+    lib.cfx_set_transport.argtypes = [vp, C.c_int]
+    lib.cfx_set_transport.restype = C.c_int
+    lib.cfx_mlp_stats.argtypes = [vp, C.POINTER(C.c_longlong)]
+    lib.cfx_mlp_stats.restype = C.c_int
     _lib = lib
     return lib
 
@@ -115,6 +138,67 @@ class Solver(StepContext):
         n_lev = C.c_int(0)
         self._call("schedule_info", C.byref(n_ops), C.byref(n_lev))
         return n_ops.value, n_lev.value
+
+    # ------------------------------------------------ transport surrogate
+    @staticmethod
+    def mlp_default_config(**overrides):
+        cfg = abi.CfxMlpConfig()
+        load().cfx_mlp_default_config(C.byref(cfg))
+        for k, v in overrides.items():
+            setattr(cfg, k, v)
+        return cfg
+
+    def mlp_train(self, **overrides):
+        """Train the surrogate on the GPU; returns the report as a dict."""
+        cfg = self.mlp_default_config(**overrides)
+        rep = abi.CfxMlpReport()
+        self._call("mlp_train", C.byref(cfg), C.byref(rep))
+        out = {k: getattr(rep, k) for k, _ in abi.CfxMlpReport._fields_}
+        out["val_max_rel"] = list(rep.val_max_rel)
+        out["val_p99_rel"] = list(rep.val_p99_rel)
+        return out
+
+    def mlp_n_params(self):
+        return self.lib.cfx_mlp_n_params(self.h)
+
+    def mlp_set_params(self, params, norm):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        n = np.ascontiguousarray(norm, dtype=np.float32)
+        self._call("mlp_set_params", p.ctypes.data_as(C.POINTER(C.c_float)), n.ctypes.data_as(C.POINTER(C.c_float)))
+
+    def mlp_get_params(self):
+        p = np.zeros(self.mlp_n_params(), dtype=np.float32)
+        n = np.zeros(64, dtype=np.float32)
+        self._call("mlp_get_params", p.ctypes.data_as(C.POINTER(C.c_float)), n.ctypes.data_as(C.POINTER(C.c_float)))
+        return p, n
+
+    def mlp_set_check(self, tol, every):
+        self._call("mlp_set_check", float(tol), int(every))
+
+    def mlp_gradient(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(self.mlp_n_params(), dtype=np.float32)
+        loss = C.c_double(0.0)
+        f = C.POINTER(C.c_float)
+        self._call("mlp_gradient", x.ctypes.data_as(f), t.ctypes.data_as(f), int(x.shape[0]), g.ctypes.data_as(f),
+                   C.byref(loss))
+        return g, loss.value
+
+    def mlp_predict(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros_like(x)
+        f = C.POINTER(C.c_float)
+        self._call("mlp_predict", x.ctypes.data_as(f), int(x.shape[0]), y.ctypes.data_as(f))
+        return y
+
+    def set_transport(self, transport):
+        self._call("set_transport", int(transport))
+
+    def mlp_stats(self):
+        c = (C.c_longlong * 3)()
+        self._call("mlp_stats", c)
+        return {"outside": c[0], "checked": c[1], "check_failed": c[2]}
 
     def kernel_times(self):
         out = {}

@@ -262,6 +262,68 @@ int cfx_get_rhs_tiles(cfx_ctx *ctx, int variant[3], int persist[3]);
  * NULL) receives the per-direction time of the choice.  Leaves dq undefined. */
 int cfx_autotune(cfx_ctx *ctx, int reps, double *best_ms);
 
+/* ------------------------------------------------ transport surrogate ---
+ * No counterpart in the reference (its transport is Transport.cu:206-259,
+ * which generates the training samples and stays the fallback).  A 3 x 64
+ * SiLU MLP maps (T, log p, Y_1..Y_ns) to log(mu, kappa, rho*D_1..ns), BF16
+ * weights on MFMA with FP32 accumulation, trained on the GPU.  Runs on the
+ * mixed-precision path with 1..14 species, selected by
+ * cfx_params.transport = CFX_TRANSPORT_MLP or cfx_set_transport.  Points
+ * outside the training envelope (T / p box, Mahalanobis distance in the
+ * principal axes of the training inputs) take the Wilke routine; one
+ * wavefront in check_every also evaluates Wilke and keeps it where the
+ * surrogate is off by more than check_tol.  Until trained or loaded, a
+ * context with the surrogate selected returns CFX_E_SURROGATE from the calls
+ * that evaluate properties. */
+typedef struct {
+  long long n_samples;    /* training set including the held-out part (2,000,000) */
+  double holdout;         /* held-out fraction for validation (0.02) */
+  int epochs;             /* passes over the training part (40) */
+  int batch;              /* samples per Adam step, multiple of 32 (8192) */
+  double lr, lr_final;    /* Adam step size, cosine-decayed (2e-3 -> 1e-5) */
+  double t_min, t_max;    /* sampled temperature range [K] (250, 3000) */
+  double p_min, p_max;    /* sampled pressure range [Pa], log-uniform (0.5, 2 atm) */
+  double dirichlet_alpha; /* mass fractions ~ Dirichlet(alpha) (0.5) */
+  double check_tol;       /* relative bound of the sampled Wilke check (0.05) */
+  int check_every;        /* one wavefront in check_every is checked (64) */
+  unsigned long long seed;
+} cfx_mlp_config;
+
+typedef struct {
+  double samples_ms, train_ms; /* GPU time: sample generation, training */
+  int epochs, steps;
+  double loss_first, loss_last; /* mean squared error of the normalised log outputs */
+  double val_rmse;              /* the same on the held-out samples */
+  double val_max_rel[3];        /* held-out max relative error: mu, kappa, max_l rho*D_l */
+  double val_p99_rel[3];        /* 99th percentile of the same */
+  long long n_train, n_val;
+  int n_params;
+} cfx_mlp_report;
+
+void cfx_mlp_default_config(cfx_mlp_config *cfg);
+/* Generates the training set on the GPU with the FP64 transport routine,
+ * trains, validates, and makes the surrogate available (cfg NULL: defaults). */
+int cfx_mlp_train(cfx_ctx *ctx, const cfx_mlp_config *cfg, cfx_mlp_report *report);
+/* Number of FP32 parameters: W1[64][ns+2] b1[64] W2[64][64] b2 W3[64][64] b3
+ * W4[ns+2][64] b4[ns+2], in that order. */
+int cfx_mlp_n_params(cfx_ctx *ctx);
+/* norm: 4 x 16 floats: input mean, input 1/std, output mean, output std
+ * (inputs T, log p, Y_l; outputs log mu, log kappa, log rho*D_l).  Loaded
+ * parameters get an envelope that admits every input. */
+int cfx_mlp_set_params(cfx_ctx *ctx, const float *params, const float *norm);
+int cfx_mlp_get_params(cfx_ctx *ctx, float *params, float *norm);
+int cfx_mlp_set_check(cfx_ctx *ctx, double check_tol, int check_every);
+/* Gradient of loss = sum (y - t)^2 / (n (ns+2)) over n (a multiple of 32)
+ * normalised samples x[n][16], t[n][16] (zero padded) with the GPU training
+ * kernels; grad receives cfx_mlp_n_params floats. */
+int cfx_mlp_gradient(cfx_ctx *ctx, const float *x, const float *t, int n, float *grad, double *loss);
+/* Normalised outputs y[n][16] of normalised inputs x[n][16]. */
+int cfx_mlp_predict(cfx_ctx *ctx, const float *x, int n, float *y);
+int cfx_set_transport(cfx_ctx *ctx, int transport);
+/* Counters since training/loading: [0] points outside the envelope,
+ * [1] points checked against Wilke, [2] checks failed. */
+int cfx_mlp_stats(cfx_ctx *ctx, long long counts[3]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,6 +46,11 @@ def lib():
         _lib.orc_sutherland.argtypes = [C.c_double]
         _lib.orc_sutherland.restype = C.c_double
         _lib.orc_inviscid_face.argtypes = [C.POINTER(abi.CfxSpecies), C.POINTER(abi.CfxParams), dp, dp, dp, dp]
+        fp = C.POINTER(C.c_float)
+        _lib.orc_mlp_forward.argtypes = [fp, C.c_int, C.c_int, fp, C.c_int, C.c_int, dp]
+        _lib.orc_mlp_forward.restype = C.c_int
+        _lib.orc_mlp_loss_grad.argtypes = [fp, C.c_int, C.c_int, fp, fp, C.c_int, C.c_int, dp, dp]
+        _lib.orc_mlp_loss_grad.restype = C.c_int
     return _lib
 
 
@@ -122,3 +127,30 @@ def inviscid_face(sp, params, pv, metric, jac):
     f = np.zeros(5 + ns)
     lib().orc_inviscid_face(C.byref(s), C.byref(prm), abi.dptr(pv), abi.dptr(metric), abi.dptr(jac), abi.dptr(f))
     return f
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def mlp_forward(params, nin, nout, x, emulate=True):
+    """Network outputs y[n][16] of inputs x[n][16] (float rows)."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    xx = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.zeros((xx.shape[0], 16))
+    rc = lib().orc_mlp_forward(_fp(p), nin, nout, _fp(xx), xx.shape[0], 1 if emulate else 0, abi.dptr(y))
+    assert rc == 0
+    return y
+
+
+def mlp_loss_grad(params, nin, nout, x, t, emulate=True):
+    """(loss, gradient) of sum (y - t)^2 / (n nout)."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    xx = np.ascontiguousarray(x, dtype=np.float32)
+    tt = np.ascontiguousarray(t, dtype=np.float32)
+    g = np.zeros(p.size)
+    loss = C.c_double(0.0)
+    rc = lib().orc_mlp_loss_grad(_fp(p), nin, nout, _fp(xx), _fp(tt), xx.shape[0], 1 if emulate else 0,
+                                 abi.dptr(g), C.byref(loss))
+    assert rc == 0
+    return loss.value, g

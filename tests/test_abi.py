@@ -57,6 +57,8 @@ int main(void) {
   printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(cfx_bc), sizeof(cfx_inner), sizeof(cfx_block),
          sizeof(cfx_inflow), sizeof(cfx_wall), sizeof(cfx_params), sizeof(cfx_species));
   printf("%zu %zu %zu\n", offsetof(cfx_params, cfl), offsetof(cfx_params, inflow), offsetof(cfx_block, inner));
+  printf("%zu %zu %zu %zu\n", sizeof(cfx_mlp_config), sizeof(cfx_mlp_report), offsetof(cfx_mlp_config, seed),
+         offsetof(cfx_mlp_report, n_params));
   return 0;
 }
 '''
@@ -69,5 +71,7 @@ int main(void) {
     sizes = [int(v) for v in out]
     mine = [C.sizeof(abi.CfxBC), C.sizeof(abi.CfxInner), C.sizeof(abi.CfxBlock), C.sizeof(abi.CfxInflow),
             C.sizeof(abi.CfxWall), C.sizeof(abi.CfxParams), C.sizeof(abi.CfxSpecies),
-            abi.CfxParams.cfl.offset, abi.CfxParams.inflow.offset, abi.CfxBlock.inner.offset]
+            abi.CfxParams.cfl.offset, abi.CfxParams.inflow.offset, abi.CfxBlock.inner.offset,
+            C.sizeof(abi.CfxMlpConfig), C.sizeof(abi.CfxMlpReport), abi.CfxMlpConfig.seed.offset,
+            abi.CfxMlpReport.n_params.offset]
     assert sizes == mine

@@ -1,0 +1,123 @@
+"""The transport surrogate on the GPU.
+
+  - forward pass (BF16 MFMA, FP32 accumulation, the operand lane layout)
+    against the oracle's network with the same BF16 rounding points;
+  - gradients of the GPU training kernels against the oracle's
+    backpropagation (same rounding points), and against exact FP64;
+  - training on samples of the FP64 Wilke routine reaches a stated accuracy
+    on held-out samples;
+  - what the surrogate gains over the method without it: the same
+    mixed-precision step with Wilke transport, timed per kernel class, and
+    the surrogate step's error against the FP64 oracle after 10 steps.
+"""
+import numpy as np
+import pytest
+
+from combustioncodegpu_amd import abi, cases
+from tests.test_oracle_mlp import random_params, samples
+
+pytestmark = pytest.mark.gpu
+
+NS = 9
+NIN = NOUT = NS + 2
+
+
+def identity_norm():
+    n = np.zeros(64, dtype=np.float32)
+    n[16:32] = 1.0  # input 1/std
+    n[48:64] = 1.0  # output std
+    return n
+
+
+def solver(case):
+    from combustioncodegpu_amd.lib import Solver
+    return Solver(case)
+
+
+def small_mixed_case(mlp=False):
+    c = cases.c1(24, 20, 16)
+    c.params["precision"] = abi.MIXED
+    if mlp:
+        c.params["transport"] = abi.TRANSPORT_MLP
+    return c
+
+
+def test_forward_matches_oracle():
+    from oracle import pyoracle
+    g = solver(small_mixed_case())
+    p = random_params(NIN, NOUT, 11)
+    g.mlp_set_params(p, identity_norm())
+    x, _ = samples(1000, NIN, NOUT, 12)
+    y = g.mlp_predict(x)[:, :NOUT]
+    ye = pyoracle.mlp_forward(p, NIN, NOUT, x, emulate=True)[:, :NOUT]
+    yx = pyoracle.mlp_forward(p, NIN, NOUT, x, emulate=False)[:, :NOUT]
+    err_e = np.abs(y - ye).max() / np.abs(ye).max()
+    err_x = np.abs(y - yx).max() / np.abs(yx).max()
+    print(f"[mlp] forward: vs emulated oracle {err_e:.2e}, vs FP64 {err_x:.2e}", flush=True)
+    assert err_e <= 2e-3
+    assert err_x <= 5e-2
+
+
+def test_gradient_matches_oracle():
+    from oracle import pyoracle
+    g = solver(small_mixed_case())
+    p = random_params(NIN, NOUT, 21)
+    g.mlp_set_params(p, identity_norm())
+    x, t = samples(256, NIN, NOUT, 22)
+    grad, loss = g.mlp_gradient(x, t)
+    le, ge = pyoracle.mlp_loss_grad(p, NIN, NOUT, x, t, emulate=True)
+    lx, gx = pyoracle.mlp_loss_grad(p, NIN, NOUT, x, t, emulate=False)
+    err_e = np.abs(grad - ge).max() / np.abs(ge).max()
+    err_x = np.abs(grad - gx).max() / np.abs(gx).max()
+    print(f"[mlp] gradient: vs emulated oracle {err_e:.2e}, vs FP64 {err_x:.2e}; loss {loss:.6e} vs {le:.6e}",
+          flush=True)
+    assert abs(loss - le) <= 1e-3 * le
+    assert err_e <= 5e-3
+    assert err_x <= 5e-2
+
+
+def test_training_reaches_accuracy():
+    g = solver(small_mixed_case())
+    rep = g.mlp_train(n_samples=262144, epochs=12, batch=4096)
+    print(f"[mlp] small training: {rep}", flush=True)
+    assert rep["loss_last"] < rep["loss_first"] / 50
+    assert max(rep["val_p99_rel"]) <= 0.05
+    rep_params, norm = g.mlp_get_params()
+    assert np.all(np.isfinite(rep_params))
+
+
+def test_surrogate_step_gain_and_error():
+    from oracle.pyoracle import Oracle
+    ref = cases.c1(24, 20, 16)
+    o = Oracle(ref)
+    o.upload_initial()
+    o.init_conservative()
+    o.step(10)
+    out = {}
+    for name in ("wilke", "mlp"):
+        g = solver(small_mixed_case())
+        if name == "mlp":
+            rep = g.mlp_train(n_samples=262144, epochs=12, batch=4096)
+            g.set_transport(abi.TRANSPORT_MLP)
+        g.upload_initial()
+        g.init_conservative()
+        g.step(10)
+        bg, bo = g.download(0, abi.BV), o.download(0, abi.BV)
+        yg, yo = g.download(0, abi.YK), o.download(0, abi.YK)
+        err = {"rho": float(np.abs(bg[0] - bo[0]).max() / np.abs(bo[0]).max()),
+               "T": float(np.abs(bg[5] - bo[5]).max() / np.abs(bo[5]).max()),
+               "Y": float(np.abs(yg - yo).max())}
+        vel = np.sqrt((bo[1:4] ** 2).sum(axis=0)).max()
+        err["velocity"] = float(np.sqrt(((bg[1:4] - bo[1:4]) ** 2).sum(axis=0)).max() / vel)
+        g.set_profiling(True)
+        g.time_steps(3)
+        kt = g.kernel_times()
+        out[name] = {"err": err, "props_ms": kt["props"][0] / 3, "stats": g.mlp_stats() if name == "mlp" else None}
+        rc, nfail = g.check()
+        assert rc == 0 and nfail == 0
+        g.close()
+    print(f"[mlp] ablation: {out}", flush=True)
+    for k, v in out["mlp"]["err"].items():
+        assert v <= 1e-4, (k, v)
+    st = out["mlp"]["stats"]
+    assert st["checked"] > 0 and st["check_failed"] <= st["checked"] // 100 + 1
