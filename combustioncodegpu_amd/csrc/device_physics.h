@@ -41,6 +41,23 @@ struct Tiny<float> {
   static constexpr float v = 1e-30f;
 };
 
+// Division, reciprocal and square root of the flux work: exact IEEE
+// operations in FP64 (parity with the oracle); in FP32 (mixed precision) the
+// hardware reciprocal and square root (v_rcp_f32, v_sqrt_f32, 1 ulp), whose
+// error is far below the FP32 rounding already accepted there.
+template <typename R>
+__device__ __forceinline__ R fdiv(R a, R b) { return a / b; }
+template <>
+__device__ __forceinline__ float fdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+template <typename R>
+__device__ __forceinline__ R frcp(R a) { return R(1) / a; }
+template <>
+__device__ __forceinline__ float frcp<float>(float a) { return __builtin_amdgcn_rcpf(a); }
+template <typename R>
+__device__ __forceinline__ R fsqrt(R a) { return sqrt(a); }
+template <>
+__device__ __forceinline__ float fsqrt<float>(float a) { return __builtin_amdgcn_sqrtf(a); }
+
 // ---------------------------------------------------------------- thermo ---
 // h_l(T) and cp_l(T) of one species (Thermo.cu:46-66): NASA-7 polynomial in
 // Horner form with the low/high set chosen at t_mid and the linear enthalpy
@@ -79,7 +96,7 @@ template <typename R>
 __device__ __forceinline__ R limiter(int tag, R a, R b) {
   if (tag == 1) {
     const R ab = a * b;
-    return ab > R(0) ? ab * (a + b) / (a * a + b * b + Tiny<R>::v) : R(0);
+    return ab > R(0) ? fdiv<R>(ab * (a + b), a * a + b * b + Tiny<R>::v) : R(0);
   }
   const R m = fmin(fabs(a), fabs(b));
   return (a > R(0) && b > R(0)) ? m : ((a < R(0) && b < R(0)) ? -m : R(0));
@@ -121,8 +138,8 @@ __device__ __forceinline__ void ausm_from_states(SpeciesPtrT<R> sp, const Scheme
       wr += pr[5 + l] * R(sp->inv_mw[l]);
     }
     const R Ru = R(kRu);
-    const R t_l = pl[4] / (pl[0] * Ru * wl);
-    const R t_r = pr[4] / (pr[0] * Ru * wr);
+    const R t_l = fdiv<R>(pl[4], pl[0] * Ru * wl);
+    const R t_r = fdiv<R>(pr[4], pr[0] * Ru * wr);
     R cpl = R(0), cpr = R(0);
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
@@ -136,26 +153,26 @@ __device__ __forceinline__ void ausm_from_states(SpeciesPtrT<R> sp, const Scheme
     }
     E_l = pl[0] * el - pl[4];
     E_r = pr[0] * er - pr[4];
-    gam_l = cpl / (cpl - Ru * wl);
-    gam_r = cpr / (cpr - Ru * wr);
+    gam_l = fdiv<R>(cpl, cpl - Ru * wl);
+    gam_r = fdiv<R>(cpr, cpr - Ru * wr);
   } else {
     const R g1 = R(1) / R(S.gamma_air - 1);
     E_l = el * pl[0] + pl[4] * g1;
     E_r = er * pr[0] + pr[4] * g1;
     gam_l = gam_r = R(S.gamma_air);
   }
-  const R gk = sqrt(k1 * k1 + k2 * k2 + k3 * k3);
-  const R igk = R(1) / gk;
+  const R gk = fsqrt<R>(k1 * k1 + k2 * k2 + k3 * k3);
+  const R igk = frcp<R>(gk);
   const R ul = (k1 * pl[1] + k2 * pl[2] + k3 * pl[3]) * igk;
   const R ur = (k1 * pr[1] + k2 * pr[2] + k3 * pr[3]) * igk;
-  const R c = R(0.5) * (sqrt(gam_l * pl[4] / pl[0]) + sqrt(gam_r * pr[4] / pr[0]));
-  const R ic = R(1) / c;
+  const R c = R(0.5) * (fsqrt<R>(fdiv<R>(gam_l * pl[4], pl[0])) + fsqrt<R>(fdiv<R>(gam_r * pr[4], pr[0])));
+  const R ic = frcp<R>(c);
   const R ml = ul * ic, mr = ur * ic;
   constexpr double alpha = 3 / 16.0;
   R mlp, plp, mrn, prn;
   if (fabs(ml) > R(1)) {
     mlp = R(0.5) * (ml + fabs(ml));
-    plp = mlp / ml;
+    plp = fdiv<R>(mlp, ml);
   } else {
     const R a = (ml + R(1)) * (ml + R(1)) * R(0.25);
     const R b = (ml * ml - R(1)) * (ml * ml - R(1));
@@ -164,7 +181,7 @@ __device__ __forceinline__ void ausm_from_states(SpeciesPtrT<R> sp, const Scheme
   }
   if (fabs(mr) > R(1)) {
     mrn = R(0.5) * (mr - fabs(mr));
-    prn = mrn / mr;
+    prn = fdiv<R>(mrn, mr);
   } else {
     const R a = (mr - R(1)) * (mr - R(1)) * R(0.25);
     const R b = (mr * mr - R(1)) * (mr * mr - R(1));
@@ -183,7 +200,7 @@ __device__ __forceinline__ void ausm_from_states(SpeciesPtrT<R> sp, const Scheme
   F[1] = coeff * (left ? pl[1] : pr[1]) + p_coeff * k1;
   F[2] = coeff * (left ? pl[2] : pr[2]) + p_coeff * k2;
   F[3] = coeff * (left ? pl[3] : pr[3]) + p_coeff * k3;
-  F[4] = coeff * ((left ? E_l : E_r) + (left ? pl[4] : pr[4])) / rho;
+  F[4] = fdiv<R>(coeff * ((left ? E_l : E_r) + (left ? pl[4] : pr[4])), rho);
 #pragma unroll
   for (int l = 5; l < NV; ++l) F[l] = coeff * (left ? pl[l] : pr[l]);
 }

@@ -609,9 +609,10 @@ int launch_face_ops(const BlockView *views, const FaceOp *ops, const int *off, i
 }
 
 // ------------------------------------------------- surrogate training set ---
-// Counter-based uniform in [0, 1): SplitMix64 finaliser of (seed, sample, k).
+// Counter-based uniform in [0, 1): SplitMix64 finaliser of (seed, sample, k);
+// each sample owns 4096 counters (a sample draws at most 2 + ns (3*32 + 1)).
 __device__ __forceinline__ double rng_uniform(unsigned long long seed, long long s, unsigned k) {
-  unsigned long long z = seed + 0x9E3779B97F4A7C15ull * ((unsigned long long)s * 64ull + k + 1ull);
+  unsigned long long z = seed + 0x9E3779B97F4A7C15ull * ((unsigned long long)s * 4096ull + k + 1ull);
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
   z ^= z >> 31;

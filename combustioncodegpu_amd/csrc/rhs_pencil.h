@@ -50,7 +50,7 @@ struct Box {
   static constexpr int sa = str[A], sb = str[Bx], sc = str[Cx];
 };
 
-template <int NS, int DIR, int TA, int TB, int TC>
+template <int NS, int DIR, int TA, int TB, int TC, typename R>
 struct PencilShape {
   static constexpr int A = Axes<DIR>::a, Bx = Axes<DIR>::b, Cx = Axes<DIR>::c;
   static constexpr int NV = 5 + NS;
@@ -61,8 +61,12 @@ struct PencilShape {
   static constexpr int tx[3] = {1, T0, T0 * T1};
   using BI = Box<DIR, TA + 3, TB, TC>;
   using BV = Box<DIR, TA + 1, TB + 2, TC + 2>;
-  static constexpr int KI = NV < 7 ? NV : 7;  // inviscid variables per batch
-  static constexpr int KV = 4;                // viscous variables per batch
+  // variables staged per batch: FP64 tiles stage 7 (inviscid) / 4 (viscous)
+  // at a time; FP32 tiles have the LDS to stage every inviscid variable and
+  // up to 9 mass fractions at once, which halves the barriers
+  static constexpr bool F32 = sizeof(R) == 4;
+  static constexpr int KI = F32 ? NV : (NV < 7 ? NV : 7);  // inviscid variables per batch
+  static constexpr int KV = F32 ? (NS > 4 ? (NS < 9 ? NS : 9) : 4) : 4;  // viscous variables per batch
   static constexpr int STG = KI * BI::V > KV * BV::V ? KI * BI::V : KV * BV::V;
 };
 
@@ -95,7 +99,7 @@ __global__ void __launch_bounds__(TA *TB *TC) __attribute__((amdgpu_waves_per_eu
     rhs_pencil_kernel(const BlockView *__restrict__ views, const int *__restrict__ off, int nb, int padded,
                       const SpeciesDev *__restrict__ spg, SchemeDev S, int accumulate) {
   const SpeciesPtrT<R> sp = const_table<R>(spg);
-  using P = PencilShape<NS, DIR, TA, TB, TC>;
+  using P = PencilShape<NS, DIR, TA, TB, TC, R>;
   using BI = typename P::BI;
   using BV = typename P::BV;
   constexpr int NV = P::NV, NT = P::NT, A = P::A, Bx = P::Bx, Cx = P::Cx;
