@@ -179,8 +179,9 @@ def mixed_errors(case, checkpoints=(1, 10, 100), mlp=True):
     out = {f: {} for f in forms if f != "fp64"}
     done = 0
     for n in checkpoints:
-        for c in ctx.values():
+        for c in ctx.values():  # one context at a time: no kernels of two contexts overlap
             c.step(n - done)
+            c.sync()
         done = n
         b = {k: c.download(0, abi.BV)[:, 2:-2, 2:-2, 2:-2] for k, c in ctx.items()}
         y = {k: c.download(0, abi.YK)[:, 2:-2, 2:-2, 2:-2] for k, c in ctx.items()}

@@ -66,26 +66,57 @@ struct BlockView {
   float *bv32, *yk32, *props32, *metric32, *jac32, *dq32;
 };
 
+// Pointers in the global address space.  A view loaded from device memory
+// (the block table) hands out generic pointers, for which the compiler emits
+// flat loads and stores; flat accesses also share the LDS wait counter, so
+// they serialise the LDS staging with the HBM traffic.  Kernels therefore use
+// GView, the same view with every array pointer marked global.
+template <class T>
+using gptr = __attribute__((address_space(1))) T *;
+
+struct GView {
+  int mx, my, mz;
+  int px, py, pz;
+  long long np, ni, sy, sz;
+  gptr<double> cv, bv, yk, props, metric, jac, dq, qn, dt_local, bv_last;
+  int mixed;
+  gptr<float> bv32, yk32, props32, metric32, jac32, dq32;
+};
+
+__device__ __forceinline__ GView gview(const BlockView &B) {
+  GView g;
+  g.mx = B.mx, g.my = B.my, g.mz = B.mz, g.px = B.px, g.py = B.py, g.pz = B.pz;
+  g.np = B.np, g.ni = B.ni, g.sy = B.sy, g.sz = B.sz;
+  g.cv = (gptr<double>)B.cv, g.bv = (gptr<double>)B.bv, g.yk = (gptr<double>)B.yk;
+  g.props = (gptr<double>)B.props, g.metric = (gptr<double>)B.metric, g.jac = (gptr<double>)B.jac;
+  g.dq = (gptr<double>)B.dq, g.qn = (gptr<double>)B.qn, g.dt_local = (gptr<double>)B.dt_local;
+  g.bv_last = (gptr<double>)B.bv_last;
+  g.mixed = B.mixed;
+  g.bv32 = (gptr<float>)B.bv32, g.yk32 = (gptr<float>)B.yk32, g.props32 = (gptr<float>)B.props32;
+  g.metric32 = (gptr<float>)B.metric32, g.jac32 = (gptr<float>)B.jac32, g.dq32 = (gptr<float>)B.dq32;
+  return g;
+}
+
 // Field selection by the arithmetic type of the flux work.
 template <typename R>
 struct Fields;
 template <>
 struct Fields<double> {
-  __host__ __device__ static const double *bv(const BlockView &B) { return B.bv; }
-  __host__ __device__ static const double *yk(const BlockView &B) { return B.yk; }
-  __host__ __device__ static const double *props(const BlockView &B) { return B.props; }
-  __host__ __device__ static const double *metric(const BlockView &B) { return B.metric; }
-  __host__ __device__ static const double *jac(const BlockView &B) { return B.jac; }
-  __host__ __device__ static double *dq(const BlockView &B) { return B.dq; }
+  __device__ static gptr<const double> bv(const GView &B) { return B.bv; }
+  __device__ static gptr<const double> yk(const GView &B) { return B.yk; }
+  __device__ static gptr<const double> props(const GView &B) { return B.props; }
+  __device__ static gptr<const double> metric(const GView &B) { return B.metric; }
+  __device__ static gptr<const double> jac(const GView &B) { return B.jac; }
+  __device__ static gptr<double> dq(const GView &B) { return B.dq; }
 };
 template <>
 struct Fields<float> {
-  __host__ __device__ static const float *bv(const BlockView &B) { return B.bv32; }
-  __host__ __device__ static const float *yk(const BlockView &B) { return B.yk32; }
-  __host__ __device__ static const float *props(const BlockView &B) { return B.props32; }
-  __host__ __device__ static const float *metric(const BlockView &B) { return B.metric32; }
-  __host__ __device__ static const float *jac(const BlockView &B) { return B.jac32; }
-  __host__ __device__ static float *dq(const BlockView &B) { return B.dq32; }
+  __device__ static gptr<const float> bv(const GView &B) { return B.bv32; }
+  __device__ static gptr<const float> yk(const GView &B) { return B.yk32; }
+  __device__ static gptr<const float> props(const GView &B) { return B.props32; }
+  __device__ static gptr<const float> metric(const GView &B) { return B.metric32; }
+  __device__ static gptr<const float> jac(const GView &B) { return B.jac32; }
+  __device__ static gptr<float> dq(const GView &B) { return B.dq32; }
 };
 
 // Inflow state resident on device: rho,u,v,w,p,T, E, then Y[ns].

@@ -13,12 +13,12 @@ KernelSet32 g_kernels32[kMaxSp + 1];
 
 namespace {
 
-__device__ __forceinline__ long long pidx(const BlockView &B, int i, int j, int k) {
+__device__ __forceinline__ long long pidx(const GView &B, int i, int j, int k) {
   return (long long)(k + kNgg) * B.sz + (long long)(j + kNgg) * B.sy + (i + kNgg);
 }
 
 // padded index of interior point q (i fastest)
-__device__ __forceinline__ long long interior_pidx(const BlockView &B, long long q) {
+__device__ __forceinline__ long long interior_pidx(const GView &B, long long q) {
   const unsigned u = (unsigned)q, mx = (unsigned)B.mx, my = (unsigned)B.my;
   const unsigned i = u % mx, r = u / mx;
   return pidx(B, (int)i, (int)(r % my), (int)(r / my));
@@ -47,7 +47,7 @@ __global__ void __launch_bounds__(kTpb) dt_kernel(const BlockView *__restrict__ 
   __shared__ double red[kTpb / 64];
   const int w = blockIdx.x + off[0];
   const int blk = find_seg(off, nb, w);
-  const BlockView B = views[blk];
+  const GView B = gview(views[blk]);
   const long long np = B.np;
   const long long base = (long long)(w - off[blk]) * (kTpb * kDtIter);
   double dmin = 1e300;
@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(kTpb) store_last_kernel(const BlockView *__res
                                                           const int *__restrict__ off, int nb) {
   const int w = blockIdx.x + off[0];
   const int blk = find_seg(off, nb, w);
-  const BlockView B = views[blk];
+  const GView B = gview(views[blk]);
   const long long np = B.np, ni = B.ni;
   const long long base = (long long)(w - off[blk]) * (kTpb * kLastIter);
   for (int it = 0; it < kLastIter; ++it) {
@@ -120,7 +120,7 @@ __global__ void __launch_bounds__(kTpb) residual_partial_kernel(const BlockView 
   __shared__ double red[kTpb / 64][4];
   const int w = blockIdx.x + off[0];
   const int blk = find_seg(off, nb, w);
-  const BlockView B = views[blk];
+  const GView B = gview(views[blk]);
   const long long np = B.np, ni = B.ni;
   const long long base = (long long)(w - off[blk]) * (kTpb * kResIter);
   double s[4] = {0, 0, 0, 0};
@@ -171,7 +171,8 @@ __global__ void __launch_bounds__(kTpb) residual_final_kernel(const double *part
 }
 
 // eliminate_k_gradient (Field.cu:360-384), 2-D only
-__global__ void eliminate_k_kernel(BlockView B, int ns) {
+__global__ void eliminate_k_kernel(BlockView view, int ns) {
+  const GView B = gview(view);
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) - kNgg;
   const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) - kNgg;
   if (i >= B.mx + kNgg || j >= B.my + kNgg) return;

@@ -29,11 +29,11 @@ constexpr PencilTile tile_of(bool d3, int dir, int v) {
   return d3 ? kTiles3[dir][v] : kTiles2[dir < 2 ? dir : 0][v < kNVar2 ? v : 0];
 }
 
-__device__ __forceinline__ long long pidx(const BlockView &B, int i, int j, int k) {
+__device__ __forceinline__ long long pidx(const GView &B, int i, int j, int k) {
   return (long long)(k + kNgg) * B.sz + (long long)(j + kNgg) * B.sy + (i + kNgg);
 }
 
-__device__ __forceinline__ long long interior_pidx(const BlockView &B, long long q) {
+__device__ __forceinline__ long long interior_pidx(const GView &B, long long q) {
   const unsigned u = (unsigned)q, mx = (unsigned)B.mx, my = (unsigned)B.my;
   const unsigned i = u % mx, r = u / mx;
   return pidx(B, (int)i, (int)(r % my), (int)(r / my));
@@ -122,7 +122,7 @@ __global__ void __launch_bounds__(kTpb) props_kernel(const BlockView *__restrict
                                                      int nb, const SpeciesDev *__restrict__ spg, SchemeDev S) {
   const SpeciesPtrT<R> sp = const_table<R>(spg);
   const PointWork pw = point_work(off, nb);
-  const BlockView B = views[pw.blk];
+  const GView B = gview(views[pw.blk]);
   const long long np = B.np;
   constexpr bool f32 = sizeof(R) == 4;
   const long long p = pw.base + threadIdx.x;
@@ -173,6 +173,7 @@ int launch_props(const Batch &b, const SpeciesDev *sp, const SchemeDev &S, hipSt
 // ------------------------------------------------------------------- RHS ---
 // FP64 pencils at 2 waves per SIMD; FP32 needs half the registers and
 // targets 3.
+// grid < padded tile count: the persistent kernel strides through the tiles
 template <int DIR, int V, typename R, bool D3>
 int launch_rhs_dir(const Batch &b, int grid, const SpeciesDev *sp, const SchemeDev &S, int accumulate,
                    hipStream_t st) {
@@ -180,8 +181,12 @@ int launch_rhs_dir(const Batch &b, int grid, const SpeciesDev *sp, const SchemeD
   constexpr int WPE = sizeof(R) == 8 ? 2 : 3;
   if (b.total <= 0) return 0;
   const int padded = (b.total + 7) / 8 * 8;
-  rhs_pencil_kernel<NS, DIR, s.ta, s.tb, s.tc, WPE, R, s.g>
-      <<<grid, s.ta * s.tb * s.tc, 0, st>>>(b.views, b.off, b.nb, padded, sp, S, accumulate);
+  if (grid < padded)
+    rhs_pencil_kernel<NS, DIR, s.ta, s.tb, s.tc, WPE, R, s.g, true>
+        <<<grid, s.ta * s.tb * s.tc, 0, st>>>(b.views, b.off, b.nb, padded, sp, S, accumulate);
+  else
+    rhs_pencil_kernel<NS, DIR, s.ta, s.tb, s.tc, WPE, R, s.g, false>
+        <<<padded, s.ta * s.tb * s.tc, 0, st>>>(b.views, b.off, b.nb, padded, sp, S, accumulate);
   return (int)hipGetLastError();
 }
 
@@ -190,8 +195,8 @@ int occupancy_rhs_dir() {
   constexpr PencilTile s = tile_of(D3, DIR, V);
   constexpr int WPE = sizeof(R) == 8 ? 2 : 3;
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, rhs_pencil_kernel<NS, DIR, s.ta, s.tb, s.tc, WPE, R, s.g>,
-                                                   s.ta * s.tb * s.tc, 0) != hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &n, rhs_pencil_kernel<NS, DIR, s.ta, s.tb, s.tc, WPE, R, s.g, true>, s.ta * s.tb * s.tc, 0) != hipSuccess)
     return 0;
   return n;
 }
@@ -219,7 +224,7 @@ void register_rhs(Set &k) {
 #if CFX_PREC == 64
 // ---------------------------------------------------------- total energy ---
 // compute_total_energy (Thermo.cu:85-105) from bv/yk/cv at padded point p.
-__device__ __forceinline__ void total_energy(const BlockView &B, SpeciesPtrT<double> sp, const SchemeDev &S,
+__device__ __forceinline__ void total_energy(const GView &B, SpeciesPtrT<double> sp, const SchemeDev &S,
                                              long long p) {
   const long long np = B.np;
   const double u = B.bv[np + p], v = B.bv[2 * np + p], w = B.bv[3 * np + p];
@@ -250,7 +255,7 @@ __global__ void __launch_bounds__(kTpb) update_kernel(const BlockView *__restric
   const SpeciesPtrT<double> sp = const_table<double>(spg);
   constexpr int NV = 5 + NS;
   const PointWork pw = point_work(off, nb);
-  const BlockView B = views[pw.blk];
+  const GView B = gview(views[pw.blk]);
   const long long np = B.np, ni = B.ni;
   const long long q = pw.base + threadIdx.x;
   if (q >= ni) return;
@@ -346,7 +351,7 @@ __global__ void __launch_bounds__(kTpb) cv_from_bv_kernel(const BlockView *__res
                                                           const SpeciesDev *__restrict__ spg, SchemeDev S) {
   const SpeciesPtrT<double> sp = const_table<double>(spg);
   const PointWork pw = point_work(off, nb);
-  const BlockView B = views[pw.blk];
+  const GView B = gview(views[pw.blk]);
   const long long np = B.np;
   const long long p = pw.base + threadIdx.x;
   if (p >= np) return;
@@ -362,7 +367,7 @@ __global__ void __launch_bounds__(kTpb) cv_from_bv_kernel(const BlockView *__res
 
 // ------------------------------------------------------------ ghost fill ---
 // apply_outflow (BoundCond.cu:455-487) at face point (i, j, k)
-__device__ __forceinline__ void bc_outflow_point(const BlockView &B, const FaceRange &f, int i, int j, int k) {
+__device__ __forceinline__ void bc_outflow_point(const GView &B, const FaceRange &f, int i, int j, int k) {
   int dir[3] = {0, 0, 0};
   dir[f.face] = f.direction;
   const long long np = B.np;
@@ -380,7 +385,7 @@ __device__ __forceinline__ void bc_outflow_point(const BlockView &B, const FaceR
 
 // apply_inflow (BoundCond.cu:489-527); the state's total energy is
 // precomputed on the host with the same formula as compute_total_energy.
-__device__ __forceinline__ void bc_inflow_point(const BlockView &B, const FaceRange &f, int i, int j, int k,
+__device__ __forceinline__ void bc_inflow_point(const GView &B, const FaceRange &f, int i, int j, int k,
                                                 const InflowDev *__restrict__ in) {
   int dir[3] = {0, 0, 0};
   dir[f.face] = f.direction;
@@ -403,7 +408,7 @@ __device__ __forceinline__ void bc_inflow_point(const BlockView &B, const FaceRa
   }
 }
 
-__device__ __forceinline__ double mix_mw(const BlockView &B, SpeciesPtrT<double> sp, const SchemeDev &S,
+__device__ __forceinline__ double mix_mw(const GView &B, SpeciesPtrT<double> sp, const SchemeDev &S,
                                          long long p) {
   if constexpr (NS == 0) {
     return kRu / S.r_air;
@@ -417,7 +422,7 @@ __device__ __forceinline__ double mix_mw(const BlockView &B, SpeciesPtrT<double>
 
 // apply_wall (BoundCond.cu:529-629) over the face's own vertex range (the
 // registered range shrunk back by ngg, BoundCond.cu:229-239).
-__device__ __forceinline__ void bc_wall_point(const BlockView &B, const FaceRange &f, int i, int j, int k,
+__device__ __forceinline__ void bc_wall_point(const GView &B, const FaceRange &f, int i, int j, int k,
                                               SpeciesPtrT<double> sp, const SchemeDev &S, const WallDev &wl) {
   int dir[3] = {0, 0, 0};
   dir[f.face] = f.direction;
@@ -476,7 +481,7 @@ __device__ __forceinline__ void bc_wall_point(const BlockView &B, const FaceRang
 // Slip wall: the boundary point loses its velocity along the face normal
 // n = grad(xi_face)/|grad(xi_face)|; ghost g mirrors interior point g with
 // the normal velocity negated and rho, p, T, Y copied.
-__device__ __forceinline__ void bc_symmetry_point(const BlockView &B, const FaceRange &f, int i, int j, int k,
+__device__ __forceinline__ void bc_symmetry_point(const GView &B, const FaceRange &f, int i, int j, int k,
                                                   SpeciesPtrT<double> sp, const SchemeDev &S) {
   int dir[3] = {0, 0, 0};
   dir[f.face] = f.direction;
@@ -516,7 +521,7 @@ __device__ __forceinline__ void bc_symmetry_point(const BlockView &B, const Face
 // inner_communication (Field.cu:305-358) with the face loop of
 // Mesh.cpp:905-917 at loop point n: shared-face points are averaged between
 // the two blocks, ghost layers take the neighbour's values.
-__device__ __forceinline__ void exchange_point(const BlockView &B, const BlockView &T, const InnerDev &f,
+__device__ __forceinline__ void exchange_point(const GView &B, const GView &T, const InnerDev &f,
                                                const int n[3]) {
   int idx[3], d_idx[3], it[3];
   for (int q = 0; q < 3; ++q) {
@@ -572,7 +577,7 @@ __global__ void __launch_bounds__(kTpb) face_ops_kernel(const BlockView *__restr
   if (n >= op.npts) return;
   const int e0 = op.ext[0], e1 = op.ext[1];
   const int nn[3] = {(int)(n % e0), (int)((n / e0) % e1), (int)(n / ((long long)e0 * e1))};
-  const BlockView B = views[op.blk];
+  const GView B = gview(views[op.blk]);
   const int i = op.r.rs[0] + nn[0], j = op.r.rs[1] + nn[1], k = op.r.rs[2] + nn[2];
   switch (op.kind) {
     case CFX_BC_OUTFLOW: bc_outflow_point(B, op.r, i, j, k); break;
@@ -580,7 +585,7 @@ __global__ void __launch_bounds__(kTpb) face_ops_kernel(const BlockView *__restr
     case CFX_BC_WALL: bc_wall_point(B, op.r, i, j, k, sp, S, op.wl); break;
     case CFX_BC_SYMMETRY: bc_symmetry_point(B, op.r, i, j, k, sp, S); break;
     case kOpExchange: {
-      const BlockView T = views[op.tgt];
+      const GView T = gview(views[op.tgt]);
       exchange_point(B, T, op.f, nn);
       break;
     }
@@ -722,7 +727,7 @@ __global__ void __launch_bounds__(kTpb) props_mlp_kernel(const BlockView *__rest
   __shared__ float yl[kTpb / 64][64][kMlpIO + 1];
   const SpeciesPtrT<float> sp = const_table<float>(spg);
   const PointWork pw = point_work(off, nb);
-  const BlockView B = views[pw.blk];
+  const GView B = gview(views[pw.blk]);
   const long long np = B.np;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
   const long long p0 = pw.base + threadIdx.x;
@@ -739,16 +744,17 @@ __global__ void __launch_bounds__(kTpb) props_mlp_kernel(const BlockView *__rest
 #pragma unroll
   for (int f = 0; f < kMlpIO; ++f) xl[wv][lane][f] = x[f];
   __syncthreads();
+  bf16x8 xo[4];
+  f32x4 yv[4];
 #pragma unroll
-  for (int tt = 0; tt < 4; ++tt) {
-    const int q = 16 * tt + c;
-    bf16x8 xo;
+  for (int tt = 0; tt < 4; ++tt)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) xo[j] = (__bf16)xl[wv][q][4 * g + j], xo[4 + j] = (__bf16)0.0f;
-    const f32x4 yv = mlp_infer_tile(ops, bias, xo);
+    for (int j = 0; j < 4; ++j) xo[tt][j] = (__bf16)xl[wv][16 * tt + c][4 * g + j], xo[tt][4 + j] = (__bf16)0.0f;
+  mlp_infer_tiles<4>(ops, bias, xo, yv);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) yl[wv][q][4 * g + r] = yv[r];
-  }
+  for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) yl[wv][16 * tt + c][4 * g + r] = yv[tt][r];
   __syncthreads();
 #pragma unroll
   for (int o = 0; o < NS + 2; ++o) out[2 + o] = __expf(yl[wv][lane][o] * mlp->t_std[o] + mlp->t_mean[o]);

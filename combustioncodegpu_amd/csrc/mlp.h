@@ -95,7 +95,7 @@ __device__ __forceinline__ bool mlp_inputs(const MlpDev *__restrict__ m, float t
 }
 
 // ----------------------------------------------------------- activations ---
-__device__ __forceinline__ float mlp_sigmoid(float z) { return 1.0f / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float mlp_sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 __device__ __forceinline__ float mlp_silu(float z) { return z * mlp_sigmoid(z); }
 __device__ __forceinline__ float mlp_dsilu(float z) {
   const float s = mlp_sigmoid(z);
@@ -195,6 +195,56 @@ __device__ __forceinline__ f32x4 mlp_infer_tile(const bf16x8 *__restrict__ ops, 
 #pragma unroll
   for (int kb = 0; kb < 2; ++kb) y = mfma_bf16(load_op(ops, kF4 + kb, lane), h[kb], y);
   return y;
+}
+
+// Inference of NTL tiles at once, weight-stationary: every A operand is
+// fetched once and applied to all tiles.
+template <int NTL>
+__device__ __forceinline__ void mlp_infer_tiles(const bf16x8 *__restrict__ ops, const float *__restrict__ bias,
+                                                const bf16x8 *x, f32x4 *y) {
+  const int lane = threadIdx.x & 63;
+  f32x4 z[NTL][4];
+  bf16x8 h[NTL][2];
+#pragma unroll
+  for (int mb = 0; mb < 4; ++mb) {
+    const bf16x8 a = load_op(ops, kF1 + mb, lane);
+    const f32x4 b0 = load_bias(bias, 0, mb, lane);
+#pragma unroll
+    for (int t = 0; t < NTL; ++t) z[t][mb] = mfma_bf16(a, x[t], b0);
+  }
+#pragma unroll
+  for (int layer = 1; layer < 3; ++layer) {
+#pragma unroll
+    for (int t = 0; t < NTL; ++t)
+#pragma unroll
+      for (int kb = 0; kb < 2; ++kb) h[t][kb] = pack_pair(silu4(z[t][2 * kb]), silu4(z[t][2 * kb + 1]));
+    const int base = layer == 1 ? kF2 : kF3;
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+      const f32x4 b0 = load_bias(bias, layer, mb, lane);
+#pragma unroll
+      for (int t = 0; t < NTL; ++t) z[t][mb] = b0;
+#pragma unroll
+      for (int kb = 0; kb < 2; ++kb) {
+        const bf16x8 a = load_op(ops, base + 2 * mb + kb, lane);
+#pragma unroll
+        for (int t = 0; t < NTL; ++t) z[t][mb] = mfma_bf16(a, h[t][kb], z[t][mb]);
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NTL; ++t)
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) h[t][kb] = pack_pair(silu4(z[t][2 * kb]), silu4(z[t][2 * kb + 1]));
+  const f32x4 b3 = load_bias(bias, 3, 0, lane);
+#pragma unroll
+  for (int t = 0; t < NTL; ++t) y[t] = b3;
+#pragma unroll
+  for (int kb = 0; kb < 2; ++kb) {
+    const bf16x8 a = load_op(ops, kF4 + kb, lane);
+#pragma unroll
+    for (int t = 0; t < NTL; ++t) y[t] = mfma_bf16(a, h[t][kb], y[t]);
+  }
 }
 
 }  // namespace cfx

@@ -87,38 +87,35 @@ __device__ __forceinline__ bool pencil_tile(int lid, int nta, int ntb, int ntc, 
   return ib < ntb && ic < ntc;
 }
 
-// The tiles of all blocks of a batch form one sequence (block by block,
-// off[] the per-block prefix).  Workgroups are dealt to the 8 XCDs
-// round-robin, so sequence position t is remapped to lid = (t % 8) * (padded
-// / 8) + t / 8: each XCD works through a contiguous run of tiles (whole
-// blocks where they are small) and keeps their halos in its own L2.  With a
-// grid smaller than the tile count the workgroups stay resident and stride
-// through the sequence (persistent launch).
-template <int NS, int DIR, int TA, int TB, int TC, int WPE, typename R, int G = 1>
-__global__ void __launch_bounds__(TA *TB *TC) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
-    rhs_pencil_kernel(const BlockView *__restrict__ views, const int *__restrict__ off, int nb, int padded,
-                      const SpeciesDev *__restrict__ spg, SchemeDev S, int accumulate) {
-  const SpeciesPtrT<R> sp = const_table<R>(spg);
+// One tile of the RHS of face direction DIR: tseq is the tile's position in
+// the batch's tile sequence (block by block, off[] the per-block prefix).
+// Workgroups are dealt to the 8 XCDs round-robin, so position t is remapped to
+// lid = (t % 8) * (padded / 8) + t / 8: each XCD works through a contiguous
+// run of tiles (whole blocks where they are small) and keeps their halos in
+// its own L2.
+template <int NS, int DIR, int TA, int TB, int TC, typename R, int G>
+__device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ views, const int *__restrict__ off,
+                                                int nb, int padded, int tseq, SpeciesPtrT<R> sp, const SchemeDev &S,
+                                                int accumulate, R *__restrict__ stg, R *__restrict__ fbuf) {
   using P = PencilShape<NS, DIR, TA, TB, TC, R>;
   using BI = typename P::BI;
   using BV = typename P::BV;
   constexpr int NV = P::NV, NT = P::NT, A = P::A, Bx = P::Bx, Cx = P::Cx;
   constexpr int ta_stride = P::tx[A];
-  __shared__ R stg[P::STG];    // a batch of staged variables
-  __shared__ R fbuf[NV * NT];  // each thread's face flux F_inv - F_visc
   using Fd = Fields<R>;
   const int total = off[nb] - off[0];
-  for (int tseq = blockIdx.x; tseq < padded; tseq += gridDim.x) {
+  {
     const int lid = (tseq & 7) * (padded >> 3) + (tseq >> 3);
-    if (lid >= total) continue;  // uniform per workgroup
-    const int wgi = lid + off[0];
-    const int blk = find_seg(off, nb, wgi);
-    const BlockView B = views[blk];
-    const R *__restrict__ BV_ = Fd::bv(B);
-    const R *__restrict__ YK_ = Fd::yk(B);
-    const R *__restrict__ PR_ = Fd::props(B);
-    const R *__restrict__ MT_ = Fd::metric(B);
-    const R *__restrict__ JC_ = Fd::jac(B);
+    if (lid >= total) return;  // uniform per workgroup
+    // uniform per workgroup: keep the block index and the view in SGPRs
+    const int wgi = __builtin_amdgcn_readfirstlane(lid + off[0]);
+    const int blk = __builtin_amdgcn_readfirstlane(find_seg(off, nb, wgi));
+    const GView B = gview(views[blk]);
+    const gptr<const R> __restrict__ BV_ = Fd::bv(B);
+    const gptr<const R> __restrict__ YK_ = Fd::yk(B);
+    const gptr<const R> __restrict__ PR_ = Fd::props(B);
+    const gptr<const R> __restrict__ MT_ = Fd::metric(B);
+    const gptr<const R> __restrict__ JC_ = Fd::jac(B);
 
     const long long np = B.np;
     const int m[3] = {B.mx, B.my, B.mz};
@@ -127,7 +124,7 @@ __global__ void __launch_bounds__(TA *TB *TC) __attribute__((amdgpu_waves_per_eu
     const int ntb = (m[Bx] + TB - 1) / TB;
     const int ntc = (m[Cx] + TC - 1) / TC;
     int ia, ib, ic;
-    if (!pencil_tile<G>(wgi - off[blk], nta, ntb, ntc, ia, ib, ic)) continue;  // uniform per workgroup
+    if (!pencil_tile<G>(wgi - off[blk], nta, ntb, ntc, ia, ib, ic)) return;  // uniform per workgroup
     int o[3];
     o[A] = ia * (TA - 1);
     o[Bx] = ib * TB;
@@ -148,7 +145,7 @@ __global__ void __launch_bounds__(TA *TB *TC) __attribute__((amdgpu_waves_per_eu
 
     // stage K variables (source pointers) over box BX with global origin bo;
     // points beyond the padded range are left unwritten (no valid face reads them)
-    auto stage = [&](auto box, const R *const *src, int K, const int bo[3]) {
+    auto stage = [&](auto box, const gptr<const R> *src, int K, const int bo[3]) {
       using BX = decltype(box);
       __syncthreads();
       for (int e = tid; e < BX::V; e += NT) {
@@ -180,7 +177,7 @@ __global__ void __launch_bounds__(TA *TB *TC) __attribute__((amdgpu_waves_per_eu
       R pl[NV], pr[NV];
 #pragma unroll
       for (int v0 = 0; v0 < NV; v0 += P::KI) {
-        const R *src[P::KI];
+        gptr<const R> src[P::KI];
         const int K = NV - v0 < P::KI ? NV - v0 : P::KI;
 #pragma unroll
         for (int k = 0; k < P::KI; ++k) {
@@ -199,7 +196,7 @@ __global__ void __launch_bounds__(TA *TB *TC) __attribute__((amdgpu_waves_per_eu
 #pragma unroll
       for (int v = 0; v < NV; ++v) F[v] = R(0);
       if (face_ok) {
-        const R *M = MT_ + 3 * A * np;
+        const gptr<const R> M = MT_ + 3 * A * np;
         const R jl = JC_[pl_i], jr = JC_[pr_i];
         k1 = R(0.5) * (jl * M[pl_i] + jr * M[pr_i]);
         k2 = R(0.5) * (jl * M[np + pl_i] + jr * M[np + pr_i]);
@@ -248,7 +245,7 @@ __global__ void __launch_bounds__(TA *TB *TC) __attribute__((amdgpu_waves_per_eu
       };
       R gcv[3], du[3], dv[3], dw[3], dT[3], avg[4];
       {
-        const R *src[P::KV] = {BV_ + np, BV_ + 2 * np, BV_ + 3 * np, BV_ + 5 * np};
+        gptr<const R> src[P::KV] = {BV_ + np, BV_ + 2 * np, BV_ + 3 * np, BV_ + 5 * np};
         stage(BV{}, src, 4, bo);
         grad(0, gcv);
         chain(gcv, du);
@@ -287,7 +284,7 @@ __global__ void __launch_bounds__(TA *TB *TC) __attribute__((amdgpu_waves_per_eu
         R ucx = 0, ucy = 0, ucz = 0, e4 = 0;
 #pragma unroll
         for (int s0 = 0; s0 < NS; s0 += P::KV) {
-          const R *src[P::KV];
+          gptr<const R> src[P::KV];
           const int K = NS - s0 < P::KV ? NS - s0 : P::KV;
 #pragma unroll
           for (int k = 0; k < P::KV; ++k) src[k] = YK_ + (s0 + k < NS ? s0 + k : NS - 1) * np;
@@ -324,7 +321,7 @@ __global__ void __launch_bounds__(TA *TB *TC) __attribute__((amdgpu_waves_per_eu
       int c[3] = {g[0], g[1], g[2]};
       c[A] += 1;
       const long long q = ((long long)c[2] * B.my + c[1]) * B.mx + c[0];
-      R *__restrict__ DQ = Fd::dq(B);
+      const gptr<R> __restrict__ DQ = Fd::dq(B);
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         const R d = -(fbuf[v * NT + tid + ta_stride] - fbuf[v * NT + tid]);
@@ -332,7 +329,26 @@ __global__ void __launch_bounds__(TA *TB *TC) __attribute__((amdgpu_waves_per_eu
         else DQ[v * B.ni + q] = d;
       }
     }
-  }  // tile loop
+  }
+}
+
+// One workgroup per tile (PERSIST = false), or a grid smaller than the tile
+// count whose workgroups stay resident and stride through the sequence
+// (PERSIST = true; a separate kernel because the loop costs registers).
+template <int NS, int DIR, int TA, int TB, int TC, int WPE, typename R, int G, bool PERSIST>
+__global__ void __launch_bounds__(TA *TB *TC) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
+    rhs_pencil_kernel(const BlockView *__restrict__ views, const int *__restrict__ off, int nb, int padded,
+                      const SpeciesDev *__restrict__ spg, SchemeDev S, int accumulate) {
+  using P = PencilShape<NS, DIR, TA, TB, TC, R>;
+  __shared__ R stg[P::STG];            // a batch of staged variables
+  __shared__ R fbuf[P::NV * P::NT];    // each thread's face flux F_inv - F_visc
+  const SpeciesPtrT<R> sp = const_table<R>(spg);
+  if constexpr (PERSIST) {
+    for (int t = blockIdx.x; t < padded; t += gridDim.x)
+      rhs_pencil_tile<NS, DIR, TA, TB, TC, R, G>(views, off, nb, padded, t, sp, S, accumulate, stg, fbuf);
+  } else {
+    rhs_pencil_tile<NS, DIR, TA, TB, TC, R, G>(views, off, nb, padded, blockIdx.x, sp, S, accumulate, stg, fbuf);
+  }
 }
 
 }  // namespace cfx

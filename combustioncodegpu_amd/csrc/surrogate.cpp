@@ -130,9 +130,9 @@ void cfx_mlp_default_config(cfx_mlp_config *c) {
   std::memset(c, 0, sizeof(*c));
   c->n_samples = 2000000;
   c->holdout = 0.02;
-  c->epochs = 40;
-  c->batch = 8192;
-  c->lr = 2e-3;
+  c->epochs = 60;
+  c->batch = 4096;
+  c->lr = 4e-3;
   c->lr_final = 1e-5;
   c->t_min = 250.0;
   c->t_max = 3000.0;

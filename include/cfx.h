@@ -278,9 +278,9 @@ int cfx_autotune(cfx_ctx *ctx, int reps, double *best_ms);
 typedef struct {
   long long n_samples;    /* training set including the held-out part (2,000,000) */
   double holdout;         /* held-out fraction for validation (0.02) */
-  int epochs;             /* passes over the training part (40) */
-  int batch;              /* samples per Adam step, multiple of 32 (8192) */
-  double lr, lr_final;    /* Adam step size, cosine-decayed (2e-3 -> 1e-5) */
+  int epochs;             /* passes over the training part (60) */
+  int batch;              /* samples per Adam step, multiple of 32 (4096) */
+  double lr, lr_final;    /* Adam step size, cosine-decayed (4e-3 -> 1e-5) */
   double t_min, t_max;    /* sampled temperature range [K] (250, 3000) */
   double p_min, p_max;    /* sampled pressure range [Pa], log-uniform (0.5, 2 atm) */
   double dirichlet_alpha; /* mass fractions ~ Dirichlet(alpha) (0.5) */
