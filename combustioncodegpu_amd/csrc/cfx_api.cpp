@@ -405,7 +405,8 @@ int do_update(cfx_ctx *ctx, int stage) {
   const KernelSet64 &k = g_kernels64[ctx->ns];
   CFX_TRY(per_batch(ctx, ctx->t_int, [&](const Batch &b) {
     return ctx->timed_launch(K_UPDATE, [&] {
-      return k.update(b, ctx->sp_dev, ctx->S, stage, ctx->d_dt, ctx->d_flags, ctx->d_newton, ctx->stream);
+      return k.update(b, ctx->sp_dev, ctx->S, ctx->mixed ? 1 : 0, stage, ctx->d_dt, ctx->d_flags, ctx->d_newton,
+                      ctx->stream);
     });
   }));
   return 0;

@@ -33,15 +33,17 @@ struct Batch {
 // Per face direction, the pencil tiles (faces along the direction x two
 // tangential extents) and the tile-group width G that the autotuner chooses
 // among.  Index v of each list is "variant v" of that direction.
+// w32: target waves per SIMD of the FP32 (mixed-precision) kernel; FP64
+// kernels target 2.
 struct PencilTile {
-  int ta, tb, tc, g;
+  int ta, tb, tc, g, w32;
 };
-constexpr int kNVar3 = 4, kNVar2 = 2;
+constexpr int kNVar3 = 6, kNVar2 = 2;
 constexpr PencilTile kTiles3[3][kNVar3] = {
-    {{32, 4, 2, 1}, {32, 4, 4, 1}, {64, 2, 2, 1}, {32, 8, 1, 1}},
-    {{16, 8, 2, 1}, {16, 4, 4, 1}, {32, 4, 2, 1}, {16, 16, 1, 1}},
-    {{16, 8, 2, 1}, {16, 4, 4, 1}, {32, 4, 2, 1}, {16, 16, 1, 1}}};
-constexpr PencilTile kTiles2[2][kNVar2] = {{{32, 8, 1, 1}, {64, 4, 1, 1}}, {{16, 16, 1, 1}, {32, 8, 1, 1}}};
+    {{32, 4, 2, 1, 3}, {32, 4, 4, 1, 3}, {64, 2, 2, 1, 3}, {32, 8, 1, 1, 3}, {32, 8, 1, 1, 4}, {32, 4, 2, 1, 4}},
+    {{16, 8, 2, 1, 3}, {16, 4, 4, 1, 3}, {32, 4, 2, 1, 3}, {16, 16, 1, 1, 3}, {16, 16, 1, 1, 4}, {16, 8, 2, 1, 4}},
+    {{16, 8, 2, 1, 3}, {16, 4, 4, 1, 3}, {32, 4, 2, 1, 3}, {16, 16, 1, 1, 3}, {16, 16, 1, 1, 4}, {16, 8, 2, 1, 4}}};
+constexpr PencilTile kTiles2[2][kNVar2] = {{{32, 8, 1, 1, 3}, {64, 4, 1, 1, 3}}, {{16, 16, 1, 1, 3}, {32, 8, 1, 1, 3}}};
 
 inline int rhs_variants(int dim) { return dim == 3 ? kNVar3 : kNVar2; }
 inline const PencilTile &rhs_tile_shape(int dim, int dir, int v) { return dim == 3 ? kTiles3[dir][v] : kTiles2[dir][v]; }
@@ -68,8 +70,8 @@ struct KernelSet64 {
   int present;
   int (*props)(const Batch &b, const SpeciesDev *sp, const SchemeDev &S, hipStream_t st);
   int (*cv_from_bv)(const Batch &b, const SpeciesDev *sp, const SchemeDev &S, hipStream_t st);
-  int (*update)(const Batch &b, const SpeciesDev *sp, const SchemeDev &S, int stage, const double *dt, int *flags,
-                unsigned long long *newton_fail, hipStream_t st);
+  int (*update)(const Batch &b, const SpeciesDev *sp, const SchemeDev &S, int mixed, int stage, const double *dt,
+                int *flags, unsigned long long *newton_fail, hipStream_t st);
   // ops[0..nops) with workgroup prefix off[0..nops], total workgroups
   int (*face_ops)(const BlockView *views, const FaceOp *ops, const int *off, int nops, int total,
                   const SpeciesDev *sp, const SchemeDev &S, const InflowDev *inflow, hipStream_t st);

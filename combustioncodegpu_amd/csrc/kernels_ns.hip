@@ -178,7 +178,7 @@ template <int DIR, int V, typename R, bool D3>
 int launch_rhs_dir(const Batch &b, int grid, const SpeciesDev *sp, const SchemeDev &S, int accumulate,
                    hipStream_t st) {
   constexpr PencilTile s = tile_of(D3, DIR, V);
-  constexpr int WPE = sizeof(R) == 8 ? 2 : 3;
+  constexpr int WPE = sizeof(R) == 8 ? 2 : s.w32;
   if (b.total <= 0) return 0;
   const int padded = (b.total + 7) / 8 * 8;
   if (grid < padded)
@@ -193,7 +193,7 @@ int launch_rhs_dir(const Batch &b, int grid, const SpeciesDev *sp, const SchemeD
 template <int DIR, int V, typename R, bool D3>
 int occupancy_rhs_dir() {
   constexpr PencilTile s = tile_of(D3, DIR, V);
-  constexpr int WPE = sizeof(R) == 8 ? 2 : 3;
+  constexpr int WPE = sizeof(R) == 8 ? 2 : s.w32;
   int n = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
           &n, rhs_pencil_kernel<NS, DIR, s.ta, s.tb, s.tc, WPE, R, s.g, true>, s.ta * s.tb * s.tc, 0) != hipSuccess)
@@ -207,14 +207,20 @@ void register_rhs(Set &k) {
   k.rhs3[0][1] = {launch_rhs_dir<0, 1, R, true>, occupancy_rhs_dir<0, 1, R, true>};
   k.rhs3[0][2] = {launch_rhs_dir<0, 2, R, true>, occupancy_rhs_dir<0, 2, R, true>};
   k.rhs3[0][3] = {launch_rhs_dir<0, 3, R, true>, occupancy_rhs_dir<0, 3, R, true>};
+  k.rhs3[0][4] = {launch_rhs_dir<0, 4, R, true>, occupancy_rhs_dir<0, 4, R, true>};
+  k.rhs3[0][5] = {launch_rhs_dir<0, 5, R, true>, occupancy_rhs_dir<0, 5, R, true>};
   k.rhs3[1][0] = {launch_rhs_dir<1, 0, R, true>, occupancy_rhs_dir<1, 0, R, true>};
   k.rhs3[1][1] = {launch_rhs_dir<1, 1, R, true>, occupancy_rhs_dir<1, 1, R, true>};
   k.rhs3[1][2] = {launch_rhs_dir<1, 2, R, true>, occupancy_rhs_dir<1, 2, R, true>};
   k.rhs3[1][3] = {launch_rhs_dir<1, 3, R, true>, occupancy_rhs_dir<1, 3, R, true>};
+  k.rhs3[1][4] = {launch_rhs_dir<1, 4, R, true>, occupancy_rhs_dir<1, 4, R, true>};
+  k.rhs3[1][5] = {launch_rhs_dir<1, 5, R, true>, occupancy_rhs_dir<1, 5, R, true>};
   k.rhs3[2][0] = {launch_rhs_dir<2, 0, R, true>, occupancy_rhs_dir<2, 0, R, true>};
   k.rhs3[2][1] = {launch_rhs_dir<2, 1, R, true>, occupancy_rhs_dir<2, 1, R, true>};
   k.rhs3[2][2] = {launch_rhs_dir<2, 2, R, true>, occupancy_rhs_dir<2, 2, R, true>};
   k.rhs3[2][3] = {launch_rhs_dir<2, 3, R, true>, occupancy_rhs_dir<2, 3, R, true>};
+  k.rhs3[2][4] = {launch_rhs_dir<2, 4, R, true>, occupancy_rhs_dir<2, 4, R, true>};
+  k.rhs3[2][5] = {launch_rhs_dir<2, 5, R, true>, occupancy_rhs_dir<2, 5, R, true>};
   k.rhs2[0][0] = {launch_rhs_dir<0, 0, R, false>, occupancy_rhs_dir<0, 0, R, false>};
   k.rhs2[0][1] = {launch_rhs_dir<0, 1, R, false>, occupancy_rhs_dir<0, 1, R, false>};
   k.rhs2[1][0] = {launch_rhs_dir<1, 0, R, false>, occupancy_rhs_dir<1, 0, R, false>};
@@ -248,6 +254,10 @@ __device__ __forceinline__ void total_energy(const GView &B, SpeciesPtrT<double>
 // update_cv_and_bv (TimeAdvanceFunc.cu:301-336) with the RK stage
 // combination q = a*qn + b*(q + dt*dq/J) and the Newton inversion of T
 // (Thermo.cu:108-144, stop |dT| <= 1e-10 K unless CFX_Q_NEWTON_REL).
+// MIXED selects the FP32 dq.  The loads are issued before the first store:
+// the stores go through pointers the compiler must assume alias the loaded
+// arrays.
+template <bool MIXED>
 __global__ void __launch_bounds__(kTpb) update_kernel(const BlockView *__restrict__ views, const int *__restrict__ off,
                                                       int nb, const SpeciesDev *__restrict__ spg, SchemeDev S,
                                                       int stage, const double *__restrict__ dt_dev, int *flags,
@@ -261,34 +271,27 @@ __global__ void __launch_bounds__(kTpb) update_kernel(const BlockView *__restric
   if (q >= ni) return;
   const double A = stage == 1 ? 0.75 : (stage == 2 ? 1.0 / 3.0 : 0.0);
   const double Bc = stage == 1 ? 0.25 : (stage == 2 ? 2.0 / 3.0 : 1.0);
-  const double dt = *dt_dev;
   const bool rel = (S.quirks & CFX_Q_NEWTON_REL) != 0;
+  const bool rk3 = S.rk == 3;
   const long long p = interior_pidx(B, q);
-  // every load is issued before the first store: the stores go through
-  // pointers the compiler must assume alias the loaded arrays
   double cur[NV], dqv[NV], qnv[NV];
 #pragma unroll
   for (int l = 0; l < NV; ++l) cur[l] = B.cv[l * np + p];
-  if (B.mixed) {
 #pragma unroll
-    for (int l = 0; l < NV; ++l) dqv[l] = (double)B.dq32[l * ni + q];
-  } else {
-#pragma unroll
-    for (int l = 0; l < NV; ++l) dqv[l] = B.dq[l * ni + q];
-  }
-  const bool rk3 = S.rk == 3;
-  if (rk3 && stage > 0) {
+  for (int l = 0; l < NV; ++l) dqv[l] = MIXED ? (double)B.dq32[l * ni + q] : B.dq[l * ni + q];
+  const bool use_qn = rk3 && stage > 0;
+  if (use_qn) {
 #pragma unroll
     for (int l = 0; l < NV; ++l) qnv[l] = B.qn[l * ni + q];
-  } else {
-#pragma unroll
-    for (int l = 0; l < NV; ++l) qnv[l] = cur[l];
   }
   const double t_old = B.bv[5 * np + p];
-  const double dtj = (rk3 ? dt : B.dt_local[q]) / B.jac[p];
+  const double dtj = (rk3 ? *dt_dev : B.dt_local[q]) / B.jac[p];
   double c[NV];
 #pragma unroll
-  for (int l = 0; l < NV; ++l) c[l] = rk3 ? A * qnv[l] + Bc * (cur[l] + dqv[l] * dtj) : cur[l] + dqv[l] * dtj;
+  for (int l = 0; l < NV; ++l) {
+    const double base = use_qn ? qnv[l] : cur[l];
+    c[l] = rk3 ? A * base + Bc * (cur[l] + dqv[l] * dtj) : cur[l] + dqv[l] * dtj;
+  }
   if (B.mz == 1) c[3] = 0.0;
   if (rk3 && stage == 0) {
 #pragma unroll
@@ -593,10 +596,13 @@ __global__ void __launch_bounds__(kTpb) face_ops_kernel(const BlockView *__restr
   }
 }
 
-int launch_update(const Batch &b, const SpeciesDev *sp, const SchemeDev &S, int stage, const double *dt, int *flags,
-                  unsigned long long *nf, hipStream_t st) {
+int launch_update(const Batch &b, const SpeciesDev *sp, const SchemeDev &S, int mixed, int stage, const double *dt,
+                  int *flags, unsigned long long *nf, hipStream_t st) {
   if (b.total <= 0) return 0;
-  update_kernel<<<b.total, kTpb, 0, st>>>(b.views, b.off, b.nb, sp, S, stage, dt, flags, nf);
+  if (mixed)
+    update_kernel<true><<<b.total, kTpb, 0, st>>>(b.views, b.off, b.nb, sp, S, stage, dt, flags, nf);
+  else
+    update_kernel<false><<<b.total, kTpb, 0, st>>>(b.views, b.off, b.nb, sp, S, stage, dt, flags, nf);
   return (int)hipGetLastError();
 }
 

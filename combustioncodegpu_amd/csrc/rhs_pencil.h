@@ -18,9 +18,21 @@
 // taken once at the end and dq written (first direction) or accumulated.
 #pragma once
 
+#include <utility>
+
 #include "device_physics.h"
 
 namespace cfx {
+
+// f(integral_constant<int, I>) for I = 0..N-1, unrolled at compile time.
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
 
 template <int DIR>
 struct Axes;
@@ -50,7 +62,7 @@ struct Box {
   static constexpr int sa = str[A], sb = str[Bx], sc = str[Cx];
 };
 
-template <int NS, int DIR, int TA, int TB, int TC, typename R>
+template <int NS, int DIR, int TA, int TB, int TC, typename R, int WPE>
 struct PencilShape {
   static constexpr int A = Axes<DIR>::a, Bx = Axes<DIR>::b, Cx = Axes<DIR>::c;
   static constexpr int NV = 5 + NS;
@@ -66,7 +78,7 @@ struct PencilShape {
   // up to 9 mass fractions at once, which halves the barriers
   static constexpr bool F32 = sizeof(R) == 4;
   static constexpr int KI = F32 ? NV : (NV < 7 ? NV : 7);  // inviscid variables per batch
-  static constexpr int KV = F32 ? (NS > 4 ? (NS < 9 ? NS : 9) : 4) : 4;  // viscous variables per batch
+  static constexpr int KV = F32 ? (WPE >= 4 ? (NS < 5 ? 4 : 5) : (NS > 4 ? (NS < 9 ? NS : 9) : 4)) : 4;  // viscous variables per batch
   static constexpr int STG = KI * BI::V > KV * BV::V ? KI * BI::V : KV * BV::V;
 };
 
@@ -93,11 +105,11 @@ __device__ __forceinline__ bool pencil_tile(int lid, int nta, int ntb, int ntc, 
 // lid = (t % 8) * (padded / 8) + t / 8: each XCD works through a contiguous
 // run of tiles (whole blocks where they are small) and keeps their halos in
 // its own L2.
-template <int NS, int DIR, int TA, int TB, int TC, typename R, int G>
+template <int NS, int DIR, int TA, int TB, int TC, typename R, int G, int WPE>
 __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ views, const int *__restrict__ off,
                                                 int nb, int padded, int tseq, SpeciesPtrT<R> sp, const SchemeDev &S,
                                                 int accumulate, R *__restrict__ stg, R *__restrict__ fbuf) {
-  using P = PencilShape<NS, DIR, TA, TB, TC, R>;
+  using P = PencilShape<NS, DIR, TA, TB, TC, R, WPE>;
   using BI = typename P::BI;
   using BV = typename P::BV;
   constexpr int NV = P::NV, NT = P::NT, A = P::A, Bx = P::Bx, Cx = P::Cx;
@@ -143,18 +155,33 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
     const long long pl_i = face_ok ? (long long)(g[2] + kNgg) * B.sz + (long long)(g[1] + kNgg) * B.sy + (g[0] + kNgg) : 0;
     const long long pr_i = pl_i + st[A];
 
-    // stage K variables (source pointers) over box BX with global origin bo;
-    // points beyond the padded range are left unwritten (no valid face reads them)
-    auto stage = [&](auto box, const gptr<const R> *src, int K, const int bo[3]) {
+    // stage K variables (source pointers) over box BX with global origin bo.
+    // All loads of the batch are issued before the first LDS write (one
+    // memory round trip per batch); coordinates beyond the padded range are
+    // clamped into it, which only fills slots no valid face reads.
+    auto stage = [&](auto box, auto kc, const gptr<const R> *src, const int bo[3]) {
       using BX = decltype(box);
+      constexpr int K = decltype(kc)::value;
+      constexpr int IT = (BX::V + NT - 1) / NT;
+      R tmp[IT][K];
+#pragma unroll
+      for (int it = 0; it < IT; ++it) {
+        const int e0 = tid + it * NT;
+        const int e = e0 < BX::V ? e0 : BX::V - 1;
+        const int x = min(bo[0] + e % BX::L0, m[0] + 1);
+        const int y = min(bo[1] + (e / BX::L0) % BX::L1, m[1] + 1);
+        const int z = min(bo[2] + e / (BX::L0 * BX::L1), m[2] + 1);
+        const long long gi = (long long)(z + kNgg) * B.sz + (long long)(y + kNgg) * B.sy + (x + kNgg);
+#pragma unroll
+        for (int k = 0; k < K; ++k) tmp[it][k] = src[k][gi];
+      }
       __syncthreads();
-      for (int e = tid; e < BX::V; e += NT) {
-        const int x = bo[0] + e % BX::L0;
-        const int y = bo[1] + (e / BX::L0) % BX::L1;
-        const int z = bo[2] + e / (BX::L0 * BX::L1);
-        if (x <= m[0] + 1 && y <= m[1] + 1 && z <= m[2] + 1) {
-          const long long gi = (long long)(z + kNgg) * B.sz + (long long)(y + kNgg) * B.sy + (x + kNgg);
-          for (int k = 0; k < K; ++k) stg[k * BX::V + e] = src[k][gi];
+#pragma unroll
+      for (int it = 0; it < IT; ++it) {
+        const int e = tid + it * NT;
+        if (e < BX::V) {
+#pragma unroll
+          for (int k = 0; k < K; ++k) stg[k * BX::V + e] = tmp[it][k];
         }
       }
       __syncthreads();
@@ -175,23 +202,19 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
       const int li = l[0] + BI::L0 * (l[1] + BI::L1 * l[2]);
       constexpr int sa = BI::sa;
       R pl[NV], pr[NV];
+      static_for<(NV + P::KI - 1) / P::KI>([&](auto bi) {
+        constexpr int v0 = decltype(bi)::value * P::KI;
+        constexpr int K = NV - v0 < P::KI ? NV - v0 : P::KI;
+        gptr<const R> src[K];
 #pragma unroll
-      for (int v0 = 0; v0 < NV; v0 += P::KI) {
-        gptr<const R> src[P::KI];
-        const int K = NV - v0 < P::KI ? NV - v0 : P::KI;
+        for (int k = 0; k < K; ++k) src[k] = v0 + k < 5 ? BV_ + (v0 + k) * np : YK_ + (v0 + k - 5) * np;
+        stage(BI{}, std::integral_constant<int, K>{}, src, bo);
 #pragma unroll
-        for (int k = 0; k < P::KI; ++k) {
-          const int v = v0 + k < NV ? v0 + k : NV - 1;
-          src[k] = v < 5 ? BV_ + v * np : YK_ + (v - 5) * np;
-        }
-        stage(BI{}, src, K, bo);
-#pragma unroll
-        for (int k = 0; k < P::KI; ++k) {
-          if (v0 + k >= NV) break;
+        for (int k = 0; k < K; ++k) {
           const R *q = stg + k * BI::V + li;
           muscl<R>(S, q[-sa], q[0], q[sa], q[2 * sa], pl[v0 + k], pr[v0 + k]);
         }
-      }
+      });
       R F[NV];
 #pragma unroll
       for (int v = 0; v < NV; ++v) F[v] = R(0);
@@ -245,8 +268,8 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
       };
       R gcv[3], du[3], dv[3], dw[3], dT[3], avg[4];
       {
-        gptr<const R> src[P::KV] = {BV_ + np, BV_ + 2 * np, BV_ + 3 * np, BV_ + 5 * np};
-        stage(BV{}, src, 4, bo);
+        const gptr<const R> src[4] = {BV_ + np, BV_ + 2 * np, BV_ + 3 * np, BV_ + 5 * np};
+        stage(BV{}, std::integral_constant<int, 4>{}, src, bo);
         grad(0, gcv);
         chain(gcv, du);
         grad(1, gcv);
@@ -282,17 +305,16 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
         const long long pD = (DIR == 2 && (S.quirks & CFX_Q_HV_RHOD_J)) ? pl_i + B.sy : pr_i;
         R dp[NS], yb[NS];
         R ucx = 0, ucy = 0, ucz = 0, e4 = 0;
+        static_for<(NS + P::KV - 1) / P::KV>([&](auto bi) {
+          constexpr int s0 = decltype(bi)::value * P::KV;
+          constexpr int K = NS - s0 < P::KV ? NS - s0 : P::KV;
+          gptr<const R> src[K];
 #pragma unroll
-        for (int s0 = 0; s0 < NS; s0 += P::KV) {
-          gptr<const R> src[P::KV];
-          const int K = NS - s0 < P::KV ? NS - s0 : P::KV;
+          for (int k = 0; k < K; ++k) src[k] = YK_ + (s0 + k) * np;
+          stage(BV{}, std::integral_constant<int, K>{}, src, bo);
 #pragma unroll
-          for (int k = 0; k < P::KV; ++k) src[k] = YK_ + (s0 + k < NS ? s0 + k : NS - 1) * np;
-          stage(BV{}, src, K, bo);
-#pragma unroll
-          for (int k = 0; k < P::KV; ++k) {
+          for (int k = 0; k < K; ++k) {
             const int s = s0 + k;
-            if (s >= NS) break;
             grad(k, gcv);
             R dy[3];
             chain(gcv, dy);
@@ -307,7 +329,7 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
             ucz += dif * dy[2];
             dp[s] = dif * proj;
           }
-        }
+        });
         const R uc = k1 * ucx + k2 * ucy + k3 * ucz;
         fbuf[4 * NT + tid] -= e4;
 #pragma unroll
@@ -339,15 +361,15 @@ template <int NS, int DIR, int TA, int TB, int TC, int WPE, typename R, int G, b
 __global__ void __launch_bounds__(TA *TB *TC) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
     rhs_pencil_kernel(const BlockView *__restrict__ views, const int *__restrict__ off, int nb, int padded,
                       const SpeciesDev *__restrict__ spg, SchemeDev S, int accumulate) {
-  using P = PencilShape<NS, DIR, TA, TB, TC, R>;
+  using P = PencilShape<NS, DIR, TA, TB, TC, R, WPE>;
   __shared__ R stg[P::STG];            // a batch of staged variables
   __shared__ R fbuf[P::NV * P::NT];    // each thread's face flux F_inv - F_visc
   const SpeciesPtrT<R> sp = const_table<R>(spg);
   if constexpr (PERSIST) {
     for (int t = blockIdx.x; t < padded; t += gridDim.x)
-      rhs_pencil_tile<NS, DIR, TA, TB, TC, R, G>(views, off, nb, padded, t, sp, S, accumulate, stg, fbuf);
+      rhs_pencil_tile<NS, DIR, TA, TB, TC, R, G, WPE>(views, off, nb, padded, t, sp, S, accumulate, stg, fbuf);
   } else {
-    rhs_pencil_tile<NS, DIR, TA, TB, TC, R, G>(views, off, nb, padded, blockIdx.x, sp, S, accumulate, stg, fbuf);
+    rhs_pencil_tile<NS, DIR, TA, TB, TC, R, G, WPE>(views, off, nb, padded, blockIdx.x, sp, S, accumulate, stg, fbuf);
   }
 }
 
