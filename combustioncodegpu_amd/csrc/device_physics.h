@@ -61,18 +61,25 @@ __device__ __forceinline__ float fsqrt<float>(float a) { return __builtin_amdgcn
 // ---------------------------------------------------------------- thermo ---
 // h_l(T) and cp_l(T) of one species (Thermo.cu:46-66): NASA-7 polynomial in
 // Horner form with the low/high set chosen at t_mid and the linear enthalpy
-// extension with frozen cp below t_low.
+// extension with frozen cp below t_low.  The coefficients are wave-uniform
+// (scalar loads from the constant table); T varies per lane.  Both ranges are
+// evaluated with scalar operands and the two results selected per lane:
+// selecting per coefficient instead costs a move and a select for each one,
+// or (if the compiler folds the select into the loads) a vector load from a
+// per-lane address.  The chosen value is bit-for-bit the same.
+template <typename R, typename C>
+__device__ __forceinline__ R poly5(const C &c, R t) {
+  return R(c[0]) + t * (R(c[1]) + t * (R(c[2]) + t * (R(c[3]) + t * R(c[4]))));
+}
+
 template <typename R>
 __device__ __forceinline__ void species_h_cp(SpeciesPtrT<R> sp, int l, R t, R &h, R &cp) {
+  const R cp_lo = poly5<R>(sp->cpc[0][l], t), cp_hi = poly5<R>(sp->cpc[1][l], t);
+  const R h_lo = t * poly5<R>(sp->hc[0][l], t) + R(sp->hc[0][l][5]);
+  const R h_hi = t * poly5<R>(sp->hc[1][l], t) + R(sp->hc[1][l][5]);
   const bool hi = t >= R(sp->t_mid[l]);
-  const R c0 = hi ? R(sp->cpc[1][l][0]) : R(sp->cpc[0][l][0]), c1 = hi ? R(sp->cpc[1][l][1]) : R(sp->cpc[0][l][1]),
-          c2 = hi ? R(sp->cpc[1][l][2]) : R(sp->cpc[0][l][2]), c3 = hi ? R(sp->cpc[1][l][3]) : R(sp->cpc[0][l][3]),
-          c4 = hi ? R(sp->cpc[1][l][4]) : R(sp->cpc[0][l][4]);
-  const R e0 = hi ? R(sp->hc[1][l][0]) : R(sp->hc[0][l][0]), e1 = hi ? R(sp->hc[1][l][1]) : R(sp->hc[0][l][1]),
-          e2 = hi ? R(sp->hc[1][l][2]) : R(sp->hc[0][l][2]), e3 = hi ? R(sp->hc[1][l][3]) : R(sp->hc[0][l][3]),
-          e4 = hi ? R(sp->hc[1][l][4]) : R(sp->hc[0][l][4]), e5 = hi ? R(sp->hc[1][l][5]) : R(sp->hc[0][l][5]);
-  const R cpp = c0 + t * (c1 + t * (c2 + t * (c3 + t * c4)));
-  const R hp = t * (e0 + t * (e1 + t * (e2 + t * (e3 + t * e4)))) + e5;
+  const R cpp = hi ? cp_hi : cp_lo;
+  const R hp = hi ? h_hi : h_lo;
   const R tl = R(sp->t_low[l]);
   const bool below = t < tl;
   cp = below ? R(sp->cp_tlow[l]) : cpp;
@@ -82,11 +89,8 @@ __device__ __forceinline__ void species_h_cp(SpeciesPtrT<R> sp, int l, R t, R &h
 // cp_l(T) only, frozen below t_low (Thermo.cu:68-82)
 template <typename R>
 __device__ __forceinline__ R species_cp(SpeciesPtrT<R> sp, int l, R t) {
-  const bool hi = t >= R(sp->t_mid[l]);
-  const R c0 = hi ? R(sp->cpc[1][l][0]) : R(sp->cpc[0][l][0]), c1 = hi ? R(sp->cpc[1][l][1]) : R(sp->cpc[0][l][1]),
-          c2 = hi ? R(sp->cpc[1][l][2]) : R(sp->cpc[0][l][2]), c3 = hi ? R(sp->cpc[1][l][3]) : R(sp->cpc[0][l][3]),
-          c4 = hi ? R(sp->cpc[1][l][4]) : R(sp->cpc[0][l][4]);
-  const R cpp = c0 + t * (c1 + t * (c2 + t * (c3 + t * c4)));
+  const R cp_lo = poly5<R>(sp->cpc[0][l], t), cp_hi = poly5<R>(sp->cpc[1][l], t);
+  const R cpp = t >= R(sp->t_mid[l]) ? cp_hi : cp_lo;
   return t < R(sp->t_low[l]) ? R(sp->cp_tlow[l]) : cpp;
 }
 

@@ -293,24 +293,19 @@ __global__ void __launch_bounds__(kTpb) update_kernel(const BlockView *__restric
     c[l] = rk3 ? A * base + Bc * (cur[l] + dqv[l] * dtj) : cur[l] + dqv[l] * dtj;
   }
   if (B.mz == 1) c[3] = 0.0;
-  if (rk3 && stage == 0) {
-#pragma unroll
-    for (int l = 0; l < NV; ++l) B.qn[l * ni + q] = cur[l];
-  }
-#pragma unroll
-  for (int l = 0; l < NV; ++l) B.cv[l * np + p] = c[l];
+  // every store comes after the Newton iteration: loads and stores share one
+  // wait counter, so a store issued early would make later waits stall on it
   const double rho = c[0];
   const double ir = 1.0 / rho;
   const double u = c[1] * ir, v = c[2] * ir, w = c[3] * ir;
   const double v2 = u * u + v * v + w * w;
   double T, pr;
+  double Y[NSA];
   if constexpr (NS > 0) {
-    double Y[NSA];
     double winv = 0;
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
       Y[l] = c[5 + l] * ir;
-      B.yk[l * np + p] = Y[l];
       winv += Y[l] * sp->inv_mw[l];
     }
     const double R = kRu * winv;
@@ -338,6 +333,16 @@ __global__ void __launch_bounds__(kTpb) update_kernel(const BlockView *__restric
   } else {
     pr = (S.gamma_air - 1) * (c[4] - 0.5 * rho * v2);
     T = pr * ir / S.r_air;
+  }
+  if (rk3 && stage == 0) {
+#pragma unroll
+    for (int l = 0; l < NV; ++l) B.qn[l * ni + q] = cur[l];
+  }
+#pragma unroll
+  for (int l = 0; l < NV; ++l) B.cv[l * np + p] = c[l];
+  if constexpr (NS > 0) {
+#pragma unroll
+    for (int l = 0; l < NS; ++l) B.yk[l * np + p] = Y[l];
   }
   B.bv[p] = rho;
   B.bv[np + p] = u;
