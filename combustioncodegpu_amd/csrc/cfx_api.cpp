@@ -615,6 +615,11 @@ int cfx_create(const cfx_params *params, const cfx_species *species, const cfx_b
   (void)hipMemset(ctx->d_newton, 0, sizeof(unsigned long long));
   const double dt0 = 0.0;
   (void)hipMemcpy(ctx->d_dt, &dt0, sizeof(double), hipMemcpyHostToDevice);
+  for (int b = 0; b < n_block; ++b)
+    if (ctx->hb[b].np >= (1LL << 29)) {  // the RHS kernels address a block with 32-bit byte offsets
+      ctx->err = "block " + std::to_string(b) + " exceeds 2^29 padded points";
+      return bail(CFX_E_UNSUPPORTED);
+    }
   for (int b = 0; b < n_block; ++b) {
     const HostBlock &H = ctx->hb[b];
     BlockView V{};

@@ -74,6 +74,20 @@ struct BlockView {
 template <class T>
 using gptr = __attribute__((address_space(1))) T *;
 
+// Element at a 32-bit byte offset from a wave-uniform base.  The compiler
+// can then use the scalar-base + 32-bit vector-offset addressing, so several
+// field arrays indexed by the same point share one offset register instead
+// of each needing a 64-bit address computed per lane.  Offsets must stay
+// below 4 GiB (the library rejects blocks with more than 2^29 padded points).
+template <class T>
+__device__ __forceinline__ T ldg(gptr<const T> base, unsigned byte_off) {
+  return *(gptr<const T>)((gptr<const char>)base + byte_off);
+}
+template <class T>
+__device__ __forceinline__ gptr<T> atg(gptr<T> base, unsigned byte_off) {
+  return (gptr<T>)((gptr<char>)base + byte_off);
+}
+
 struct GView {
   int mx, my, mz;
   int px, py, pz;

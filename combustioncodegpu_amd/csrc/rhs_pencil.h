@@ -154,6 +154,9 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
     const bool cell_ok = face_ok && t[A] <= TA - 2 && g[A] + 1 < m[A];
     const long long pl_i = face_ok ? (long long)(g[2] + kNgg) * B.sz + (long long)(g[1] + kNgg) * B.sy + (g[0] + kNgg) : 0;
     const long long pr_i = pl_i + st[A];
+    // byte offsets of the two face points (32-bit: shared by every field array)
+    const unsigned pl_b = (unsigned)pl_i * (unsigned)sizeof(R), pr_b = (unsigned)pr_i * (unsigned)sizeof(R);
+    const int szi = (int)B.sz, syi = (int)B.sy;
 
     // stage K variables (source pointers) over box BX with global origin bo.
     // All loads of the batch are issued before the first LDS write (one
@@ -171,9 +174,9 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
         const int x = min(bo[0] + e % BX::L0, m[0] + 1);
         const int y = min(bo[1] + (e / BX::L0) % BX::L1, m[1] + 1);
         const int z = min(bo[2] + e / (BX::L0 * BX::L1), m[2] + 1);
-        const long long gi = (long long)(z + kNgg) * B.sz + (long long)(y + kNgg) * B.sy + (x + kNgg);
+        const unsigned gb = (unsigned)((z + kNgg) * szi + (y + kNgg) * syi + (x + kNgg)) * (unsigned)sizeof(R);
 #pragma unroll
-        for (int k = 0; k < K; ++k) tmp[it][k] = src[k][gi];
+        for (int k = 0; k < K; ++k) tmp[it][k] = ldg<R>(src[k], gb);
       }
       __syncthreads();
 #pragma unroll
@@ -220,10 +223,10 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
       for (int v = 0; v < NV; ++v) F[v] = R(0);
       if (face_ok) {
         const gptr<const R> M = MT_ + 3 * A * np;
-        const R jl = JC_[pl_i], jr = JC_[pr_i];
-        k1 = R(0.5) * (jl * M[pl_i] + jr * M[pr_i]);
-        k2 = R(0.5) * (jl * M[np + pl_i] + jr * M[np + pr_i]);
-        k3 = R(0.5) * (jl * M[2 * np + pl_i] + jr * M[2 * np + pr_i]);
+        const R jl = ldg<R>(JC_, pl_b), jr = ldg<R>(JC_, pr_b);
+        k1 = R(0.5) * (jl * ldg<R>(M, pl_b) + jr * ldg<R>(M, pr_b));
+        k2 = R(0.5) * (jl * ldg<R>(M + np, pl_b) + jr * ldg<R>(M + np, pr_b));
+        k3 = R(0.5) * (jl * ldg<R>(M + 2 * np, pl_b) + jr * ldg<R>(M + 2 * np, pr_b));
         ausm_from_states<NS, R>(sp, S, pl, pr, k1, k2, k3, F);
       }
 #pragma unroll
@@ -247,8 +250,8 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
         const bool q5 = (S.quirks & CFX_Q_METRIC_AVG) && DIR != 1;
 #pragma unroll
         for (int rc = 0; rc < 9; ++rc) {
-          const R a = MT_[rc * np + pl_i];
-          mf[rc] = (q5 && rc != 0) ? R(0.5) * (a + a) : R(0.5) * (a + MT_[rc * np + pr_i]);
+          const R a = ldg<R>(MT_ + rc * np, pl_b);
+          mf[rc] = (q5 && rc != 0) ? R(0.5) * (a + a) : R(0.5) * (a + ldg<R>(MT_ + rc * np, pr_b));
         }
       } else {
 #pragma unroll
@@ -282,8 +285,8 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
         for (int q = 0; q < 4; ++q) avg[q] = R(0.5) * (stg[q * BV::V + li] + stg[q * BV::V + li + sa]);
       }
       {
-        const R vis = face_ok ? R(0.5) * (PR_[2 * np + pl_i] + PR_[2 * np + pr_i]) : R(0);
-        const R cond = face_ok ? R(0.5) * (PR_[3 * np + pl_i] + PR_[3 * np + pr_i]) : R(0);
+        const R vis = face_ok ? R(0.5) * (ldg<R>(PR_ + 2 * np, pl_b) + ldg<R>(PR_ + 2 * np, pr_b)) : R(0);
+        const R cond = face_ok ? R(0.5) * (ldg<R>(PR_ + 3 * np, pl_b) + ldg<R>(PR_ + 3 * np, pr_b)) : R(0);
         const R third = R(1.0 / 3.0);
         const R txx = vis * (R(4) * du[0] - R(2) * dv[1] - R(2) * dw[2]) * third;
         const R tyy = vis * (R(4) * dv[1] - R(2) * du[0] - R(2) * dw[2]) * third;
@@ -302,7 +305,8 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
       }
       if constexpr (NS > 0) {
         const R tm = avg[3];
-        const long long pD = (DIR == 2 && (S.quirks & CFX_Q_HV_RHOD_J)) ? pl_i + B.sy : pr_i;
+        const unsigned pD_b = (DIR == 2 && (S.quirks & CFX_Q_HV_RHOD_J)) ? (unsigned)(pl_i + B.sy) * (unsigned)sizeof(R)
+                                                                          : pr_b;
         R dp[NS], yb[NS];
         R ucx = 0, ucy = 0, ucz = 0, e4 = 0;
         static_for<(NS + P::KV - 1) / P::KV>([&](auto bi) {
@@ -319,7 +323,7 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
             R dy[3];
             chain(gcv, dy);
             yb[s] = R(0.5) * (stg[k * BV::V + li] + stg[k * BV::V + li + sa]);
-            const R dif = face_ok ? R(0.5) * (PR_[(4 + s) * np + pl_i] + PR_[(4 + s) * np + pD]) : R(0);
+            const R dif = face_ok ? R(0.5) * (ldg<R>(PR_ + (4 + s) * np, pl_b) + ldg<R>(PR_ + (4 + s) * np, pD_b)) : R(0);
             R h, cp;
             species_h_cp<R>(sp, s, tm, h, cp);
             const R proj = k1 * dy[0] + k2 * dy[1] + k3 * dy[2];
@@ -342,13 +346,14 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
     if (cell_ok) {
       int c[3] = {g[0], g[1], g[2]};
       c[A] += 1;
-      const long long q = ((long long)c[2] * B.my + c[1]) * B.mx + c[0];
+      const unsigned qb = (unsigned)((c[2] * B.my + c[1]) * B.mx + c[0]) * (unsigned)sizeof(R);
       const gptr<R> __restrict__ DQ = Fd::dq(B);
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         const R d = -(fbuf[v * NT + tid + ta_stride] - fbuf[v * NT + tid]);
-        if (accumulate) DQ[v * B.ni + q] += d;
-        else DQ[v * B.ni + q] = d;
+        const gptr<R> dst = atg<R>(DQ + v * B.ni, qb);
+        if (accumulate) *dst += d;
+        else *dst = d;
       }
     }
   }
