@@ -50,6 +50,22 @@ void round_table(const A &a, B &b) {
   cp1(a.sqw8, b.sqw8);
 }
 
+// the per-species thermo records from the arrays (same values, regrouped)
+template <typename T>
+void fill_thermo(SpeciesT<T> &s) {
+  for (int l = 0; l < s.ns; ++l) {
+    T *v = s.th[l].v;
+    for (int g = 0; g < 2; ++g) {
+      for (int k = 0; k < 5; ++k) v[kThCp + 5 * g + k] = s.cpc[g][l][k];
+      for (int k = 0; k < 6; ++k) v[kThH + 6 * g + k] = s.hc[g][l][k];
+    }
+    v[kThTmid] = s.t_mid[l];
+    v[kThTlow] = s.t_low[l];
+    v[kThHtlow] = s.h_tlow[l];
+    v[kThCptlow] = s.cp_tlow[l];
+  }
+}
+
 int build_species(const cfx_species *sp, SpeciesDev &tables, std::string &err) {
   std::memset(&tables, 0, sizeof(tables));
   SpeciesT<double> &s = tables.d;
@@ -94,6 +110,8 @@ int build_species(const cfx_species *sp, SpeciesDev &tables, std::string &err) {
       s.sqw8[m][n] = 1.0 / std::sqrt(8 * (1 + s.mw[m] / s.mw[n]));
     }
   round_table(tables.d, tables.f);
+  fill_thermo(tables.d);
+  fill_thermo(tables.f);
   return 0;
 }
 

@@ -24,8 +24,21 @@ constexpr int kMaxSp = CFX_MAX_SPEC;
 // R_u/W_l; below t_low the enthalpy is extended linearly with the frozen cp
 // (h_tlow, cp_tlow), as Thermo.cu:29-36 does.  Held in the arithmetic type
 // of the kernel that reads it, so that every constant is a scalar operand.
+// One species' NASA-7 thermo as one contiguous, 64-byte aligned record of
+// 32 values, read by the point loops as a single vector (two or four wide
+// scalar loads, one wait) instead of one scalar load per array:
+//   [kThCp + 5r + k]  cp polynomial coefficient k of range r (0 low, 1 high)
+//   [kThH + 6r + k]   h = T * (h_0 + h_1 T + ... + h_4 T^4) + h_5, range r
+//   [kThTmid], [kThTlow], [kThHtlow], [kThCptlow]
+constexpr int kThCp = 0, kThH = 10, kThTmid = 22, kThTlow = 23, kThHtlow = 24, kThCptlow = 25, kThLen = 32;
+template <typename T>
+struct alignas(64) ThermoRec {
+  T v[kThLen];
+};
+
 template <typename T>
 struct SpeciesT {
+  ThermoRec<T> th[kMaxSp];   // the thermo arrays below, regrouped per species
   int ns;
   T mw[kMaxSp], inv_mw[kMaxSp], r_sp[kMaxSp];  // W, 1/W, R_u/W
   T t_low[kMaxSp], t_mid[kMaxSp];

@@ -57,42 +57,82 @@ template <typename R>
 __device__ __forceinline__ R fsqrt(R a) { return sqrt(a); }
 template <>
 __device__ __forceinline__ float fsqrt<float>(float a) { return __builtin_amdgcn_sqrtf(a); }
+// exp and log of the transport work: the library functions in FP64; in FP32
+// e^a = 2^(a log2 e) and ln a = log2(a) ln 2 with the hardware exp2/log2
+// (v_exp_f32, v_log_f32), for the moderate arguments of the transport
+// (|a| < 10 in exp, temperatures in log) within a few ulp.
+template <typename R>
+__device__ __forceinline__ R fexp(R a) { return exp(a); }
+template <>
+__device__ __forceinline__ float fexp<float>(float a) { return __builtin_amdgcn_exp2f(a * 1.4426950408889634f); }
+template <typename R>
+__device__ __forceinline__ R flog(R a) { return log(a); }
+template <>
+__device__ __forceinline__ float flog<float>(float a) { return __builtin_amdgcn_logf(a) * 0.6931471805599453f; }
 
 // ---------------------------------------------------------------- thermo ---
 // h_l(T) and cp_l(T) of one species (Thermo.cu:46-66): NASA-7 polynomial in
 // Horner form with the low/high set chosen at t_mid and the linear enthalpy
 // extension with frozen cp below t_low.  The coefficients are wave-uniform
-// (scalar loads from the constant table); T varies per lane.  Both ranges are
-// evaluated with scalar operands and the two results selected per lane:
-// selecting per coefficient instead costs a move and a select for each one,
-// or (if the compiler folds the select into the loads) a vector load from a
-// per-lane address.  The chosen value is bit-for-bit the same.
+// (one record per species); T varies per lane.  Both ranges are evaluated
+// with uniform operands and the two results selected per lane: selecting per
+// coefficient instead costs a move and a select for each one, or (if the
+// compiler folds the select into the loads) a vector load from a per-lane
+// address.  The chosen value is bit-for-bit the same.
+
+// Marks v as computed on every lane at this point.  Without it the compiler
+// turns the selects below into branches and sinks the polynomial work (and
+// its coefficient loads, each with its own wait) into them.
+template <typename R>
+__device__ __forceinline__ R on_all_lanes(R v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+// h and cp of one species from its thermo record c (c[i] = value i of the
+// record: constant memory read with scalar loads, or a copy in LDS).
 template <typename R, typename C>
-__device__ __forceinline__ R poly5(const C &c, R t) {
-  return R(c[0]) + t * (R(c[1]) + t * (R(c[2]) + t * (R(c[3]) + t * R(c[4]))));
+__device__ __forceinline__ void thermo_h_cp(const C &c, R t, R &h, R &cp) {
+  auto p5 = [&](int o) { return R(c[o]) + t * (R(c[o + 1]) + t * (R(c[o + 2]) + t * (R(c[o + 3]) + t * R(c[o + 4])))); };
+  const R cp_lo = p5(kThCp), cp_hi = p5(kThCp + 5);
+  const R h_lo = t * p5(kThH) + R(c[kThH + 5]);
+  const R h_hi = t * p5(kThH + 6) + R(c[kThH + 11]);
+  const bool hi = t >= R(c[kThTmid]);
+  const R cpp = on_all_lanes<R>(hi ? cp_hi : cp_lo);
+  const R hp = on_all_lanes<R>(hi ? h_hi : h_lo);
+  const R tl = R(c[kThTlow]);
+  const bool below = t < tl;
+  cp = below ? R(c[kThCptlow]) : cpp;
+  h = below ? R(c[kThHtlow]) + R(c[kThCptlow]) * (t - tl) : hp;
 }
 
 template <typename R>
 __device__ __forceinline__ void species_h_cp(SpeciesPtrT<R> sp, int l, R t, R &h, R &cp) {
-  const R cp_lo = poly5<R>(sp->cpc[0][l], t), cp_hi = poly5<R>(sp->cpc[1][l], t);
-  const R h_lo = t * poly5<R>(sp->hc[0][l], t) + R(sp->hc[0][l][5]);
-  const R h_hi = t * poly5<R>(sp->hc[1][l], t) + R(sp->hc[1][l][5]);
-  const bool hi = t >= R(sp->t_mid[l]);
-  const R cpp = hi ? cp_hi : cp_lo;
-  const R hp = hi ? h_hi : h_lo;
-  const R tl = R(sp->t_low[l]);
-  const bool below = t < tl;
-  cp = below ? R(sp->cp_tlow[l]) : cpp;
-  h = below ? R(sp->h_tlow[l]) + R(sp->cp_tlow[l]) * (t - tl) : hp;
+  thermo_h_cp<R>(sp->th[l].v, t, h, cp);
 }
 
-// cp_l(T) only, frozen below t_low (Thermo.cu:68-82)
+// cp_l(T) only, frozen below t_low (Thermo.cu:68-82).  Loops over species
+// that need only cp read the per-array layout: the cp coefficients of all
+// species are contiguous there and arrive in a few wide loads.
 template <typename R>
 __device__ __forceinline__ R species_cp(SpeciesPtrT<R> sp, int l, R t) {
-  const R cp_lo = poly5<R>(sp->cpc[0][l], t), cp_hi = poly5<R>(sp->cpc[1][l], t);
-  const R cpp = t >= R(sp->t_mid[l]) ? cp_hi : cp_lo;
+  auto p5 = [&](const auto &c) { return R(c[0]) + t * (R(c[1]) + t * (R(c[2]) + t * (R(c[3]) + t * R(c[4])))); };
+  const R cpp = t >= R(sp->t_mid[l]) ? p5(sp->cpc[1][l]) : p5(sp->cpc[0][l]);
   return t < R(sp->t_low[l]) ? R(sp->cp_tlow[l]) : cpp;
 }
+
+// Where a kernel reads the thermo records from: constant memory, or a copy
+// of the species' records in LDS (loaded once per workgroup).
+template <typename R>
+struct ThermoConst {
+  SpeciesPtrT<R> sp;
+  __device__ auto rec(int l) const { return sp->th[l].v; }
+};
+template <typename R>
+struct ThermoShared {
+  const R *p;
+  __device__ const R *rec(int l) const { return p + l * kThLen; }
+};
 
 // -------------------------------------------------------------- limiters ---
 // minmod (Limiter.cuh:135-138) and Van Albada (tag 1)
@@ -127,8 +167,8 @@ __device__ __forceinline__ void muscl(const SchemeDev &S, R q0, R q1, R q2, R q3
 // pl/pr = (rho,u,v,w,p,Y..) and the face area vector k = 0.5(J_L grad_L +
 // J_R grad_R) of the face direction; the face temperature, energy and gamma
 // follow Reconstruction.cu:79-122.
-template <int NS, typename R>
-__device__ __forceinline__ void ausm_from_states(SpeciesPtrT<R> sp, const SchemeDev &S, const R *pl,
+template <int NS, typename R, typename TH>
+__device__ __forceinline__ void ausm_from_states(SpeciesPtrT<R> sp, const TH &th, const SchemeDev &S, const R *pl,
                                                  const R *pr, R k1, R k2, R k3, R *F) {
   constexpr int NV = 5 + NS;
   R el = R(0.5) * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]);
@@ -148,10 +188,10 @@ __device__ __forceinline__ void ausm_from_states(SpeciesPtrT<R> sp, const Scheme
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
       R h, cp;
-      species_h_cp<R>(sp, l, t_l, h, cp);
+      thermo_h_cp<R>(th.rec(l), t_l, h, cp);
       el += h * pl[5 + l];
       cpl += cp * pl[5 + l];
-      species_h_cp<R>(sp, l, t_r, h, cp);
+      thermo_h_cp<R>(th.rec(l), t_r, h, cp);
       er += h * pr[5 + l];
       cpr += cp * pr[5 + l];
     }

@@ -65,9 +65,9 @@ __device__ __forceinline__ void mixture_gamma_c(SpeciesPtrT<R> sp, R t, const R 
     cpt += Y[l] * cp[l];
   }
   const R Ru = R(kRu);
-  const R gam = cpt / (cpt - Ru * winv);
+  const R gam = fdiv<R>(cpt, cpt - Ru * winv);
   out[0] = gam;
-  out[1] = sqrt(gam * Ru * t * winv);
+  out[1] = fsqrt<R>(gam * Ru * t * winv);
 }
 
 // compute_transport_property (Transport.cu:206-259): out = mu, kappa,
@@ -78,18 +78,18 @@ template <typename R>
 __device__ __forceinline__ void wilke_point(SpeciesPtrT<R> sp, const SchemeDev &S, R t, const R *Y, const R *cp,
                                             R winv, R *out) {
   // Chapman-Enskog species viscosity (Transport.cu:215-220)
-  const R lt = log(t), st = sqrt(t);
+  const R lt = flog<R>(t), st = fsqrt<R>(t);
   R mu_s[NSA], X[NSA], s[NSA], rs[NSA];
-  const R imw = R(1) / winv;
+  const R imw = frcp<R>(winv);
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
     X[l] = Y[l] * imw * R(sp->inv_mw[l]);
     const R tdl = t * R(sp->lj_inv[l]);
     const R a = tdl + R(0.5);
-    const R omega = R(1.147) * exp(R(-0.145) * (lt + R(sp->log_lj_inv[l]))) + R(1) / (a * a);
-    mu_s[l] = R(sp->vis_coeff[l]) * st / omega;
-    s[l] = sqrt(mu_s[l]);
-    rs[l] = R(1) / s[l];
+    const R omega = R(1.147) * fexp<R>(R(-0.145) * (lt + R(sp->log_lj_inv[l]))) + frcp<R>(a * a);
+    mu_s[l] = fdiv<R>(R(sp->vis_coeff[l]) * st, omega);
+    s[l] = fsqrt<R>(mu_s[l]);
+    rs[l] = frcp<R>(s[l]);
   }
   // Wilke mixing and Mason-Saxena conductivity (Transport.cu:221-246)
   R mu = 0, kappa = 0;
@@ -103,8 +103,8 @@ __device__ __forceinline__ void wilke_point(SpeciesPtrT<R> sp, const SchemeDev &
       vt += q * q * R(sp->sqw8[m][n]) * X[n];
     }
     const R lam = mu_s[m] * (cp[m] + R(sp->cond_add[m]));
-    mu += mu_s[m] * X[m] / vt;
-    kappa += lam * X[m] / (R(1.065) * vt - R(0.065) * X[m]);
+    mu += fdiv<R>(mu_s[m] * X[m], vt);
+    kappa += fdiv<R>(lam * X[m], R(1.065) * vt - R(0.065) * X[m]);
   }
   out[0] = mu;
   out[1] = kappa;
@@ -112,7 +112,7 @@ __device__ __forceinline__ void wilke_point(SpeciesPtrT<R> sp, const SchemeDev &
   const R mus = mu * R(S.inv_sc);
 #pragma unroll
   for (int l = 0; l < NS; ++l)
-    out[2 + l] = fabs(X[l] - R(1)) < R(1e-3) ? mus : (R(1) - Y[l]) * mu / ((R(1) - X[l]) * R(S.sc));
+    out[2 + l] = fabs(X[l] - R(1)) < R(1e-3) ? mus : fdiv<R>((R(1) - Y[l]) * mu, (R(1) - X[l]) * R(S.sc));
 }
 
 // Properties over the padded range.  In R = float (mixed precision) the
@@ -745,7 +745,7 @@ __global__ void __launch_bounds__(kTpb) props_mlp_kernel(const BlockView *__rest
   const bool valid = p0 < np;
   const long long p = valid ? p0 : np - 1;  // every lane takes part in the MFMA tiles
   const float t = float(B.bv[5 * np + p]);
-  const float lp = logf(float(B.bv[4 * np + p]));
+  const float lp = flog<float>(float(B.bv[4 * np + p]));
   float Y[NSA], cp[NSA], winv, out[4 + NSA];
 #pragma unroll
   for (int l = 0; l < NS; ++l) Y[l] = float(B.yk[l * np + p]);

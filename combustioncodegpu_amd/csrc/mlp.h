@@ -95,7 +95,11 @@ __device__ __forceinline__ bool mlp_inputs(const MlpDev *__restrict__ m, float t
 }
 
 // ----------------------------------------------------------- activations ---
-__device__ __forceinline__ float mlp_sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+// sigma(z) = 1 / (1 + 2^(-z log2 e)) with the hardware exp2 and reciprocal
+// (v_exp_f32, v_rcp_f32, 1 ulp); an overflow of 2^x to +inf gives sigma = 0.
+__device__ __forceinline__ float mlp_sigmoid(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * -1.4426950408889634f));
+}
 __device__ __forceinline__ float mlp_silu(float z) { return z * mlp_sigmoid(z); }
 __device__ __forceinline__ float mlp_dsilu(float z) {
   const float s = mlp_sigmoid(z);

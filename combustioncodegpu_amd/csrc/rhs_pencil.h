@@ -227,7 +227,7 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
         k1 = R(0.5) * (jl * ldg<R>(M, pl_b) + jr * ldg<R>(M, pr_b));
         k2 = R(0.5) * (jl * ldg<R>(M + np, pl_b) + jr * ldg<R>(M + np, pr_b));
         k3 = R(0.5) * (jl * ldg<R>(M + 2 * np, pl_b) + jr * ldg<R>(M + 2 * np, pr_b));
-        ausm_from_states<NS, R>(sp, S, pl, pr, k1, k2, k3, F);
+        ausm_from_states<NS, R>(sp, ThermoConst<R>{sp}, S, pl, pr, k1, k2, k3, F);
       }
 #pragma unroll
       for (int v = 0; v < NV; ++v) fbuf[v * NT + tid] = F[v];
