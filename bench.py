@@ -224,6 +224,8 @@ def main():
     ap.add_argument("--workload", choices=["c1", "c3"], default="c1")
     ap.add_argument("--rhs-variant", type=int, default=-1)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--mlp", action="store_true",
+                    help="profiling mode: use the surrogate transport (trained first with the default config)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -241,6 +243,9 @@ def main():
         case = cases.c1(mx, my, mz) if args.workload == "c1" else cases.c3()
         case.params["precision"] = abi.MIXED if args.precision == "mixed" else abi.FP64
         g = Solver(case)
+        if args.mlp:
+            g.mlp_train()
+            g.set_transport(abi.TRANSPORT_MLP)
         g.upload_initial()
         g.init_conservative()
         if args.rhs_variant >= 0:

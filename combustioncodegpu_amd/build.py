@@ -19,8 +19,10 @@ LIB = os.path.join(HERE, "libcfx.so")
 ARCH = os.environ.get("CFX_OFFLOAD_ARCH", "gfx950")
 
 # Packed FP32 instructions are disabled for every HIP translation unit.
+# MFMA results are kept in ordinary VGPRs (the surrogate applies SiLU to
+# every accumulator; in AGPRs each one would first need a v_accvgpr_read).
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",
-          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-mllvm", "-amdgpu-mfma-vgpr-form"]
 
 # Species counts with compiled kernels (the species loops are unrolled at
 # compile time); kernels_ns.hip is built once per count and precision.

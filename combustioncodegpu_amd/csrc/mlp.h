@@ -65,6 +65,9 @@ struct MlpDev {
   float t_lo, t_hi, lp_lo, lp_hi;
   float pca[kMlpIO][kMlpIO];  // rows: principal axes
   float pca_inv_var[kMlpIO];  // 1/eigenvalue, 0 for degenerate axes
+  // the same distance as a quadratic form, d2 = sum_{i<=j} quad[i][j] x_i x_j
+  // (quad = P^T diag(inv_var) P with the off-diagonal terms doubled)
+  float quad[kMlpIO][kMlpIO];
   float d2_max;
   float check_tol;            // relative error bound of the sampled Wilke check
   int check_every;            // one wavefront in check_every is checked
@@ -84,11 +87,11 @@ __device__ __forceinline__ bool mlp_inputs(const MlpDev *__restrict__ m, float t
   for (int f = 2; f < kMlpIO; ++f) x[f] = f < NSP + 2 ? (Y[f - 2] - m->x_mean[f]) * m->x_inv_std[f] : 0.0f;
   float d2 = 0.0f;
 #pragma unroll
-  for (int k = 0; k < NSP + 2; ++k) {
+  for (int i = 0; i < NSP + 2; ++i) {
     float s = 0.0f;
 #pragma unroll
-    for (int i = 0; i < NSP + 2; ++i) s += m->pca[k][i] * x[i];
-    d2 += m->pca_inv_var[k] * s * s;
+    for (int j = i; j < NSP + 2; ++j) s += m->quad[i][j] * x[j];
+    d2 += x[i] * s;
   }
   const bool box = t >= m->t_lo && t <= m->t_hi && lp >= m->lp_lo && lp <= m->lp_hi;
   return !box || !(d2 <= m->d2_max);

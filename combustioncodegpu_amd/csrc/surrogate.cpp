@@ -112,7 +112,18 @@ void open_envelope(MlpDev &m) {
   m.d2_max = 1e30f;
 }
 
+// the envelope distance in the principal axes as one quadratic form
+void envelope_quadratic(MlpDev &m) {
+  for (int i = 0; i < kMlpIO; ++i)
+    for (int j = 0; j < kMlpIO; ++j) {
+      double a = 0.0;
+      for (int k = 0; k < kMlpIO; ++k) a += (double)m.pca_inv_var[k] * m.pca[k][i] * m.pca[k][j];
+      m.quad[i][j] = j < i ? 0.0f : (float)(j == i ? a : 2.0 * a);
+    }
+}
+
 int upload_mlp(cfx_ctx *ctx) {
+  envelope_quadratic(ctx->mlp_host);
   CFX_TRY((int)hipMemcpy(ctx->mlp_dev, &ctx->mlp_host, sizeof(MlpDev), hipMemcpyHostToDevice));
   CFX_TRY(launch_mlp_pack(ctx->mlp_params, ctx->mlp_shape, ctx->mlp_ops, ctx->mlp_bias, ctx->stream));
   CFX_TRY((int)hipStreamSynchronize(ctx->stream));
