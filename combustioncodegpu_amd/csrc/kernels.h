@@ -41,8 +41,8 @@ struct PencilTile {
 constexpr int kNVar3 = 6, kNVar2 = 2;
 constexpr PencilTile kTiles3[3][kNVar3] = {
     {{32, 4, 2, 1, 3}, {32, 4, 4, 1, 3}, {64, 2, 2, 1, 3}, {32, 8, 1, 1, 3}, {32, 8, 1, 1, 4}, {32, 4, 2, 1, 4}},
-    {{16, 8, 2, 1, 3}, {8, 32, 1, 1, 3}, {8, 32, 1, 1, 4}, {16, 16, 1, 1, 3}, {16, 16, 1, 1, 4}, {16, 8, 2, 1, 4}},
-    {{16, 8, 2, 1, 3}, {8, 32, 1, 1, 3}, {8, 32, 1, 1, 4}, {16, 16, 1, 1, 3}, {16, 16, 1, 1, 4}, {16, 8, 2, 1, 4}}};
+    {{16, 8, 2, 1, 3}, {8, 32, 1, 0, 4}, {8, 32, 1, 1, 4}, {16, 16, 1, 0, 4}, {16, 16, 1, 1, 4}, {16, 8, 2, 1, 4}},
+    {{16, 8, 2, 1, 3}, {8, 32, 1, 0, 4}, {8, 32, 1, 1, 4}, {16, 16, 1, 0, 4}, {16, 16, 1, 1, 4}, {16, 8, 2, 1, 4}}};
 constexpr PencilTile kTiles2[2][kNVar2] = {{{32, 8, 1, 1, 3}, {64, 4, 1, 1, 3}}, {{16, 16, 1, 1, 3}, {32, 8, 1, 1, 3}}};
 
 inline int rhs_variants(int dim) { return dim == 3 ? kNVar3 : kNVar2; }
@@ -54,6 +54,7 @@ inline int pencil_tiles(const int m[3], int dir, const PencilTile &s) {
   const int a = dir, b = dir == 0 ? 1 : 0, c = dir == 2 ? 1 : 2;
   const int nta = (m[a] + s.ta - 2) / (s.ta - 1);
   const int ntb = (m[b] + s.tb - 1) / s.tb, ntc = (m[c] + s.tc - 1) / s.tc;
+  if (s.g == 0) return nta * ntb * ntc;
   return nta * s.g * s.g * ((ntb + s.g - 1) / s.g) * ((ntc + s.g - 1) / s.g);
 }
 

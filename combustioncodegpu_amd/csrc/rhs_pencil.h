@@ -84,9 +84,18 @@ struct PencilShape {
 
 // Tile order inside a block: the tiles of the (b, c) plane are visited in
 // G x G groups with a fastest, so that tiles sharing a tangential halo run
-// close together.  Returns false for the padding tiles of partial groups.
+// close together; G = 0 visits c fastest, then a, then b (neighbours along c
+// share two of the three rows of the viscous box).  Returns false for the
+// padding tiles of partial groups.
 template <int G>
 __device__ __forceinline__ bool pencil_tile(int lid, int nta, int ntb, int ntc, int &ia, int &ib, int &ic) {
+  if constexpr (G == 0) {
+    ic = lid % ntc;
+    const int q = lid / ntc;
+    ia = q % nta;
+    ib = q / nta;
+    return true;
+  }
   const int gb = (ntb + G - 1) / G;
   ia = lid % nta;
   int q = lid / nta;
