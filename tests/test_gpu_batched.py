@@ -11,7 +11,7 @@ of operations that touch disjoint points.  Checked here:
     so batching may not change a single bit), graph replays interleaved
     with eager work and checked after each;
   - every RHS tile variant and persistence factor, and the autotuner's
-    choice, give identical bytes;
+    choice, give identical bytes (FP64 and mixed precision);
   - the mixed-precision path on a lattice stays within its error bar.
 """
 import numpy as np
@@ -120,6 +120,38 @@ def test_tile_variants_and_autotune_identical():
     got = state(g, nb)
     for b in range(nb):
         assert np.array_equal(got[b], ref[b]), f"autotuned block {b}"
+
+
+def test_mixed_tile_variants_agree():
+    """Mixed precision: every RHS variant gives the state of variant 0 after
+    two steps, on the lattice and on a block whose extents are not multiples
+    of any tile.  The pencil tiles evaluate the same expressions and agree
+    bit for bit; the bar is set at 1e-6 of each variable's maximum (the size
+    of the mixed path's own FP32 rounding after one step, 2.6e-7 in velocity
+    against FP64) so that a variant compiled with another multiply-add fusion
+    would still pass while a wrong index or stencil point (order one) fails."""
+    def fields(g, nb):
+        return [[g.download(b, sel) for sel in (abi.CV, abi.BV, abi.YK)] for b in range(nb)]
+
+    for case in (cases.lattice(SMALL, n_spec=9), cases.c1(70, 37, 41)):
+        case.params["precision"] = abi.MIXED
+        nb = len(case.blocks)
+        g = run(case, 2)
+        ref = fields(g, nb)
+        nv = g.rhs_variants()
+        g.close()
+        for v in range(1, nv):
+            g = run(case, 2, setup=lambda s: s.set_rhs_tiles([v] * 3, [0] * 3))
+            got = fields(g, nb)
+            g.close()
+            worst = 0.0
+            for b in range(nb):
+                for a, r in zip(got[b], ref[b]):
+                    for x, y in zip(a, r):  # per variable
+                        worst = max(worst, float(np.abs(x - y).max() / max(np.abs(y).max(), 1e-300)))
+            print(f"[batched] mixed tiles variant {v} on {nb} blocks: max relative difference {worst:.2e}",
+                  flush=True)
+            assert worst <= 1e-6, f"mixed variant {v}: {worst}"
 
 
 def test_lattice_mixed_precision_error():
