@@ -285,12 +285,15 @@ bool conflict(const std::vector<Touch> &x, const std::vector<Touch> &y) {
 // op it conflicts with, so every conflicting pair keeps the application
 // order and the result equals the one-op-at-a-time loop.  levelize=false
 // gives each op its own level (that loop itself).
-int build_schedule(cfx_ctx *ctx, Schedule &sc, int first, int last, bool levelize) {
+int build_schedule(cfx_ctx *ctx, const std::vector<FaceOp> &all, Schedule &sc, int first, int last, bool levelize,
+                   const std::vector<int> *group = nullptr) {
   const int n = last - first;
   std::vector<int> level(n, 0);
-  if (levelize) {
+  if (!levelize && group) {  // one level per group of pieces of one op (disjoint boxes)
+    for (int i = 0; i < n; ++i) level[i] = (*group)[first + i] - (*group)[first];
+  } else if (levelize) {
     std::vector<std::vector<Touch>> tc(n);
-    for (int i = 0; i < n; ++i) tc[i] = touches(ctx->ops[first + i]);
+    for (int i = 0; i < n; ++i) tc[i] = touches(all[first + i]);
     for (int j = 0; j < n; ++j)
       for (int i = 0; i < j; ++i)
         if (level[i] + 1 > level[j] && conflict(tc[i], tc[j])) level[j] = level[i] + 1;
@@ -302,7 +305,7 @@ int build_schedule(cfx_ctx *ctx, Schedule &sc, int first, int last, bool leveliz
   sc.lvl.assign(1, 0);
   for (int L = 0; L < nlev; ++L) {
     for (int i = 0; i < n; ++i)
-      if (level[i] == L) ordered.push_back(ctx->ops[first + i]);
+      if (level[i] == L) ordered.push_back(all[first + i]);
     sc.lvl.push_back((int)ordered.size());
   }
   sc.off_host.assign(ordered.size() + 1, 0);
@@ -315,6 +318,74 @@ int build_schedule(cfx_ctx *ctx, Schedule &sc, int first, int last, bool leveliz
     if ((rc = (int)hipMemcpy(sc.ops, ordered.data(), sizeof(FaceOp) * ordered.size(), hipMemcpyHostToDevice)))
       return rc;
   return (int)hipMemcpy(sc.off, sc.off_host.data(), sizeof(int) * sc.off_host.size(), hipMemcpyHostToDevice);
+}
+
+// An outflow op whose range is the whole face (the face's points and the
+// ngg layers around them tangentially) copies the face layer into the ghost
+// layers behind it.  Its interior rows (tangential indices inside the block)
+// read only points the update kernel has just written; when no op before it
+// in the application order touches those rows (box conflict as for the
+// levels), applying them inside the update kernel, right after the point's
+// own stores, gives the same values in the same order.  The op is then
+// replaced in the step's list by its ring of edge rows.  Sets the blocks'
+// fuse flags (uploaded with the views).
+void fold_outflow(cfx_ctx *ctx) {
+  ctx->ops_step.clear();
+  ctx->ops_step_src.clear();
+  ctx->n_fused = 0;
+  for (auto &V : ctx->views) {
+    for (int f = 0; f < 6; ++f) V.fuse[f] = 0;
+    V.fuse_any = 0;
+  }
+  const int n = (int)ctx->ops.size();
+  std::vector<std::vector<Touch>> before;
+  for (int i = 0; i < n; ++i) {
+    const FaceOp &op = ctx->ops[i];
+    bool fold = false;
+    FaceOp inner = op;
+    if (op.kind == CFX_BC_OUTFLOW) {
+      const HostBlock &H = ctx->hb[op.blk];
+      const int m[3] = {H.mx, H.my, H.mz};
+      const int a = op.r.face;
+      fold = op.r.rs[a] == op.r.re[a] && op.r.rs[a] == (op.r.direction < 0 ? 0 : m[a] - 1);
+      for (int t = 0; t < 3 && fold; ++t) {
+        if (t == a) continue;
+        fold = op.r.rs[t] == -kNgg && op.r.re[t] == m[t] - 1 + kNgg;
+        inner.r.rs[t] = 0;
+        inner.r.re[t] = m[t] - 1;
+      }
+      if (fold) {
+        const std::vector<Touch> ti = touches(inner);
+        for (const auto &tb : before) fold = fold && !conflict(tb, ti);
+      }
+      if (fold) {
+        // the ring: rows with a tangential index outside the block
+        const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+        auto add = [&](int lo1, int hi1, int lo2, int hi2) {
+          if (lo1 > hi1 || lo2 > hi2) return;
+          FaceOp r = op;
+          r.r.rs[t1] = lo1, r.r.re[t1] = hi1, r.r.rs[t2] = lo2, r.r.re[t2] = hi2;
+          for (int q = 0; q < 3; ++q) r.ext[q] = r.r.re[q] - r.r.rs[q] + 1;
+          r.npts = (long long)r.ext[0] * r.ext[1] * r.ext[2];
+          ctx->ops_step.push_back(r);
+          ctx->ops_step_src.push_back(i);
+        };
+        add(-kNgg, -1, -kNgg, m[t2] - 1 + kNgg);
+        add(m[t1], m[t1] - 1 + kNgg, -kNgg, m[t2] - 1 + kNgg);
+        add(0, m[t1] - 1, -kNgg, -1);
+        add(0, m[t1] - 1, m[t2], m[t2] - 1 + kNgg);
+        BlockView &V = ctx->views[op.blk];
+        V.fuse[2 * a + (op.r.direction < 0 ? 0 : 1)] = 1;
+        V.fuse_any = 1;
+        ++ctx->n_fused;
+      }
+    }
+    if (!fold) {
+      ctx->ops_step.push_back(op);
+      ctx->ops_step_src.push_back(i);
+    }
+    before.push_back(touches(op));
+  }
 }
 
 // BC ops in the application order (BoundCond.cu:251-313), then the inner-face
@@ -356,10 +427,19 @@ int build_ops(cfx_ctx *ctx) {
     }
   const int n = (int)ctx->ops.size(), nbc = ctx->n_bc_ops;
   int rc;
-  if ((rc = build_schedule(ctx, ctx->sched_seq, 0, n, false)) || (rc = build_schedule(ctx, ctx->sched_bc, 0, nbc, true)) ||
-      (rc = build_schedule(ctx, ctx->sched_ex, nbc, n, true)) || (rc = build_schedule(ctx, ctx->sched_all, 0, n, true)))
+  if ((rc = build_schedule(ctx, ctx->ops, ctx->sched_seq, 0, n, false)) ||
+      (rc = build_schedule(ctx, ctx->ops, ctx->sched_bc, 0, nbc, true)) ||
+      (rc = build_schedule(ctx, ctx->ops, ctx->sched_ex, nbc, n, true)) ||
+      (rc = build_schedule(ctx, ctx->ops, ctx->sched_all, 0, n, true)))
     return rc;
-  return 0;
+  // the step's ghost fill, with the outflow faces folded into the update
+  fold_outflow(ctx);
+  const int ns = (int)ctx->ops_step.size();
+  if ((rc = build_schedule(ctx, ctx->ops_step, ctx->sched_seq_step, 0, ns, false, &ctx->ops_step_src)) ||
+      (rc = build_schedule(ctx, ctx->ops_step, ctx->sched_all_step, 0, ns, true)))
+    return rc;
+  return (int)hipMemcpy(ctx->views_dev, ctx->views.data(), sizeof(BlockView) * ctx->views.size(),
+                        hipMemcpyHostToDevice);
 }
 
 // levels [l0, l1) of a schedule, one launch each
@@ -440,10 +520,13 @@ int do_exchange(cfx_ctx *ctx) {
   return run_levels(ctx, ctx->sched_seq, ctx->n_bc_ops, ctx->sched_seq.levels());
 }
 
-// boundary conditions followed by the inner-face exchange
-int do_ghosts(cfx_ctx *ctx) {
-  if (ctx->batched) return run_levels(ctx, ctx->sched_all, 0, ctx->sched_all.levels());
-  return run_levels(ctx, ctx->sched_seq, 0, ctx->sched_seq.levels());
+// boundary conditions followed by the inner-face exchange; after_update:
+// the step's schedule, whose folded outflow rows the update kernel has filled
+int do_ghosts(cfx_ctx *ctx, bool after_update) {
+  const Schedule &all = after_update ? ctx->sched_all_step : ctx->sched_all;
+  const Schedule &seq = after_update ? ctx->sched_seq_step : ctx->sched_seq;
+  if (ctx->batched) return run_levels(ctx, all, 0, all.levels());
+  return run_levels(ctx, seq, 0, seq.levels());
 }
 
 int do_elim(cfx_ctx *ctx) {
@@ -493,7 +576,7 @@ int do_step(cfx_ctx *ctx) {
       if ((rc = do_dt(ctx, 0, 1))) return rc;
     }
     if ((rc = do_update(ctx, s))) return rc;
-    if ((rc = do_ghosts(ctx))) return rc;
+    if ((rc = do_ghosts(ctx, true))) return rc;
     if ((rc = do_elim(ctx))) return rc;
     if ((rc = do_props(ctx))) return rc;
   }
@@ -793,7 +876,7 @@ int cfx_init_conservative(cfx_ctx *ctx) {
   if (!ctx) return CFX_E_ARG;
   const KernelSet64 &k = g_kernels64[ctx->ns];
   CFX_TRY(per_batch(ctx, ctx->t_pad, [&](const Batch &b) { return k.cv_from_bv(b, ctx->sp_dev, ctx->S, ctx->stream); }));
-  if (int rc = do_ghosts(ctx)) return rc;
+  if (int rc = do_ghosts(ctx, false)) return rc;
   if (int rc = do_props(ctx)) return rc;
   return 0;
 }
@@ -987,8 +1070,8 @@ int cfx_set_execution(cfx_ctx *ctx, int batched, int graph) {
 
 int cfx_schedule_info(cfx_ctx *ctx, int *n_ops, int *n_levels) {
   if (!ctx) return CFX_E_ARG;
-  if (n_ops) *n_ops = (int)ctx->ops.size();
-  if (n_levels) *n_levels = ctx->sched_all.levels();
+  if (n_ops) *n_ops = (int)ctx->ops_step.size();
+  if (n_levels) *n_levels = ctx->sched_all_step.levels();
   return 0;
 }
 

@@ -77,6 +77,11 @@ struct BlockView {
   // properties kernel over the padded range) and the FP32 dq; null in FP64 mode
   int mixed;
   float *bv32, *yk32, *props32, *metric32, *jac32, *dq32;
+  // fuse[2a + s]: the outflow condition on face (axis a, low s = 0 / high
+  // s = 1) covers the whole face and is applied by the update kernel to the
+  // ghost layers of the face's interior rows (see the ghost-fill schedule)
+  int fuse[6];
+  int fuse_any;
 };
 
 // Pointers in the global address space.  A view loaded from device memory
@@ -108,6 +113,8 @@ struct GView {
   gptr<double> cv, bv, yk, props, metric, jac, dq, qn, dt_local, bv_last;
   int mixed;
   gptr<float> bv32, yk32, props32, metric32, jac32, dq32;
+  int fuse[6];
+  int fuse_any;
 };
 
 __device__ __forceinline__ GView gview(const BlockView &B) {
@@ -121,6 +128,8 @@ __device__ __forceinline__ GView gview(const BlockView &B) {
   g.mixed = B.mixed;
   g.bv32 = (gptr<float>)B.bv32, g.yk32 = (gptr<float>)B.yk32, g.props32 = (gptr<float>)B.props32;
   g.metric32 = (gptr<float>)B.metric32, g.jac32 = (gptr<float>)B.jac32, g.dq32 = (gptr<float>)B.dq32;
+  for (int f = 0; f < 6; ++f) g.fuse[f] = B.fuse[f];
+  g.fuse_any = B.fuse_any;
   return g;
 }
 

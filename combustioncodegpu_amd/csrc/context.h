@@ -77,6 +77,12 @@ struct cfx_ctx {
   std::vector<FaceOp> ops;                     // BC ops in application order, then exchange ops
   int n_bc_ops = 0;
   Schedule sched_seq, sched_bc, sched_ex, sched_all;
+  // the step's ghost fill: ops with the whole-face outflow conditions that
+  // the update kernel applies (their interior rows) replaced by the rest
+  std::vector<FaceOp> ops_step;
+  std::vector<int> ops_step_src;  // index of the op in `ops` each step op comes from
+  Schedule sched_seq_step, sched_all_step;
+  int n_fused = 0;
   bool batched = true, use_graph = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;

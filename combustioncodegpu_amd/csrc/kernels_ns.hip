@@ -351,6 +351,32 @@ __global__ void __launch_bounds__(kTpb) update_kernel(const BlockView *__restric
   B.bv[4 * np + p] = pr;
   B.bv[5 * np + p] = T;
   if (!(rho > 0.0) || !(T > 0.0) || !(pr > 0.0) || !isfinite(rho) || !isfinite(T)) atomicOr(flags, 1);
+  // apply_outflow (BoundCond.cu:455-487) of the faces folded into the
+  // update: a point of the face layer writes its new state into the ghost
+  // layers behind it (the same cache lines as its own stores for x faces)
+  if (B.fuse_any) {
+    const unsigned uq = (unsigned)q, mx = (unsigned)B.mx, my = (unsigned)B.my;
+    const int ijk[3] = {(int)(uq % mx), (int)((uq / mx) % my), (int)(uq / (mx * my))};
+    const int mm[3] = {B.mx, B.my, B.mz};
+    const long long stride[3] = {1, B.sy, B.sz};
+    const double bvn[6] = {rho, u, v, w, pr, T};
+    for (int a = 0; a < 3; ++a)
+      for (int side = 0; side < 2; ++side) {
+        if (!B.fuse[2 * a + side] || ijk[a] != (side ? mm[a] - 1 : 0)) continue;
+        const long long step = side ? stride[a] : -stride[a];
+        for (int g = 1; g <= kNgg; ++g) {
+          const long long qg = p + g * step;
+#pragma unroll
+          for (int l = 0; l < 6; ++l) B.bv[l * np + qg] = bvn[l];
+#pragma unroll
+          for (int l = 0; l < NV; ++l) B.cv[l * np + qg] = c[l];
+          if constexpr (NS > 0) {
+#pragma unroll
+            for (int l = 0; l < NS; ++l) B.yk[l * np + qg] = Y[l];
+          }
+        }
+      }
+  }
 }
 
 // compute_cv_from_bv (Field.cu:242-268) over the padded range

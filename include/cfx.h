@@ -242,8 +242,12 @@ int cfx_set_profiling(cfx_ctx *ctx, int on);
  *   graph = 1:   cfx_step replays one captured step graph (batched or not).
  * Results are identical in every mode. */
 int cfx_set_execution(cfx_ctx *ctx, int batched, int graph);
-/* Ghost-fill operations (boundary faces + inner faces) of one stage and the
- * number of levels (launches) they take when batched. */
+/* Ghost-fill operations (boundary faces + inner faces) of one step stage and
+ * the number of levels (launches) they take when batched.  An outflow face
+ * that covers a whole block face and is not touched by an earlier operation
+ * is applied by the update kernel to its interior rows (same values, same
+ * order), so only its edge rows count here; cfx_apply_bc and
+ * cfx_init_conservative still run every face in full. */
 int cfx_schedule_info(cfx_ctx *ctx, int *n_ops, int *n_levels);
 
 /* RHS tile selection and autotuner (no counterpart in the reference, whose
