@@ -855,6 +855,15 @@ int cfx_download(cfx_ctx *ctx, int blk, int what, double *dst) {
     for (size_t i = 0; i < n32; ++i) dst[i] = tmp[i];
     return 0;
   }
+  if (V.mixed && what == CFX_YK) {  // the mixed path keeps Y only as rho*Y / rho
+    std::vector<double> cv(ctx->nv * np);
+    CFX_TRY((int)hipMemcpy(cv.data(), V.cv, sizeof(double) * cv.size(), hipMemcpyDeviceToHost));
+    for (size_t p = 0; p < np; ++p) {
+      const double ir = 1.0 / cv[p];
+      for (int l = 0; l < ctx->ns; ++l) dst[l * np + p] = cv[(5 + l) * np + p] * ir;
+    }
+    return 0;
+  }
   switch (what) {
     case CFX_CV: src = V.cv, n = ctx->nv * np; break;
     case CFX_BV: src = V.bv, n = 6 * np; break;

@@ -184,9 +184,17 @@ __global__ void eliminate_k_kernel(BlockView view, int ns) {
       B.bv[l * np + pp] = B.bv[l * np + p0];
       B.bv[l * np + pm] = B.bv[l * np + p0];
     }
-    for (int l = 0; l < ns; ++l) {
-      B.yk[l * np + pp] = B.yk[l * np + p0];
-      B.yk[l * np + pm] = B.yk[l * np + p0];
+    if (B.mixed) {  // the mixed path derives Y from cv: copy rho and rho*Y instead
+      for (int l = 0; l < 5 + ns; ++l) {
+        if (l >= 1 && l < 5) continue;
+        B.cv[l * np + pp] = B.cv[l * np + p0];
+        B.cv[l * np + pm] = B.cv[l * np + p0];
+      }
+    } else {
+      for (int l = 0; l < ns; ++l) {
+        B.yk[l * np + pp] = B.yk[l * np + p0];
+        B.yk[l * np + pm] = B.yk[l * np + p0];
+      }
     }
   }
 }

@@ -39,6 +39,14 @@ __device__ __forceinline__ long long interior_pidx(const GView &B, long long q) 
   return pidx(B, (int)i, (int)(r % my), (int)(r / my));
 }
 
+// Mass fraction l at padded point p.  The mixed-precision path keeps no FP64
+// copy of Y after the initial upload: there Y_l = (rho Y_l) * (1 / rho) from
+// the conserved state, in the update kernel's operation order, which gives
+// at every updated point the value the update would have stored.
+__device__ __forceinline__ double mass_fraction(const GView &B, long long p, int l) {
+  return B.mixed ? B.cv[(5 + l) * B.np + p] * (1.0 / B.cv[p]) : B.yk[l * B.np + p];
+}
+
 // Point kernels: workgroup -> (block, first point of its kTpb chunk).
 struct PointWork {
   int blk;
@@ -139,8 +147,14 @@ __global__ void __launch_bounds__(kTpb) props_kernel(const BlockView *__restrict
     out[3] = mu * c_temp / R(S.gamma_air - 1) * R(S.inv_pr);
   } else {
     R cp[NSA], winv;
+    if constexpr (f32) {
+      const double ir = 1.0 / B.cv[p];
 #pragma unroll
-    for (int l = 0; l < NS; ++l) Y[l] = R(B.yk[l * np + p]);
+      for (int l = 0; l < NS; ++l) Y[l] = R(B.cv[(5 + l) * np + p] * ir);
+    } else {
+#pragma unroll
+      for (int l = 0; l < NS; ++l) Y[l] = R(B.yk[l * np + p]);
+    }
     mixture_gamma_c<R>(sp, t, Y, cp, winv, out);
     wilke_point<R>(sp, S, t, Y, cp, winv, out + 2);
   }
@@ -340,7 +354,7 @@ __global__ void __launch_bounds__(kTpb) update_kernel(const BlockView *__restric
   }
 #pragma unroll
   for (int l = 0; l < NV; ++l) B.cv[l * np + p] = c[l];
-  if constexpr (NS > 0) {
+  if constexpr (NS > 0 && !MIXED) {  // the mixed path derives Y from cv where it needs it
 #pragma unroll
     for (int l = 0; l < NS; ++l) B.yk[l * np + p] = Y[l];
   }
@@ -370,7 +384,7 @@ __global__ void __launch_bounds__(kTpb) update_kernel(const BlockView *__restric
           for (int l = 0; l < 6; ++l) B.bv[l * np + qg] = bvn[l];
 #pragma unroll
           for (int l = 0; l < NV; ++l) B.cv[l * np + qg] = c[l];
-          if constexpr (NS > 0) {
+          if constexpr (NS > 0 && !MIXED) {
 #pragma unroll
             for (int l = 0; l < NS; ++l) B.yk[l * np + qg] = Y[l];
           }
@@ -412,8 +426,10 @@ __device__ __forceinline__ void bc_outflow_point(const GView &B, const FaceRange
     for (int l = 0; l < 6; ++l) B.bv[l * np + q] = B.bv[l * np + p];
 #pragma unroll
     for (int l = 0; l < 5 + NS; ++l) B.cv[l * np + q] = B.cv[l * np + p];
+    if (!B.mixed) {
 #pragma unroll
-    for (int l = 0; l < NS; ++l) B.yk[l * np + q] = B.yk[l * np + p];
+      for (int l = 0; l < NS; ++l) B.yk[l * np + q] = B.yk[l * np + p];
+    }
   }
 }
 
@@ -431,7 +447,7 @@ __device__ __forceinline__ void bc_inflow_point(const GView &B, const FaceRange 
     for (int l = 0; l < 6; ++l) B.bv[l * np + q] = in->st[l];
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
-      B.yk[l * np + q] = in->y[l];
+      if (!B.mixed) B.yk[l * np + q] = in->y[l];
       B.cv[(5 + l) * np + q] = rho * in->y[l];
     }
     B.cv[q] = rho;
@@ -442,14 +458,13 @@ __device__ __forceinline__ void bc_inflow_point(const GView &B, const FaceRange 
   }
 }
 
-__device__ __forceinline__ double mix_mw(const GView &B, SpeciesPtrT<double> sp, const SchemeDev &S,
-                                         long long p) {
+__device__ __forceinline__ double mix_mw(SpeciesPtrT<double> sp, const SchemeDev &S, const double *Y) {
   if constexpr (NS == 0) {
     return kRu / S.r_air;
   } else {
     double w = 0;
 #pragma unroll
-    for (int l = 0; l < NS; ++l) w += B.yk[l * B.np + p] * sp->inv_mw[l];
+    for (int l = 0; l < NS; ++l) w += Y[l] * sp->inv_mw[l];
     return 1.0 / w;
   }
 }
@@ -466,9 +481,13 @@ __device__ __forceinline__ void bc_wall_point(const GView &B, const FaceRange &f
   double t_wall = wl.temperature;
   if (wl.adiabatic) t_wall = B.bv[5 * np + pin];
   const double pw = B.bv[4 * np + pin];
+  double Yw[NSA];
 #pragma unroll
-  for (int l = 0; l < NS; ++l) B.yk[l * np + p] = B.yk[l * np + pin];
-  double mw = mix_mw(B, sp, S, p);
+  for (int l = 0; l < NS; ++l) {
+    Yw[l] = mass_fraction(B, pin, l);
+    if (!B.mixed) B.yk[l * np + p] = Yw[l];
+  }
+  double mw = mix_mw(sp, S, Yw);
   B.bv[p] = pw * mw / (t_wall * kRu);
   B.bv[np + p] = 0;
   B.bv[2 * np + p] = 0;
@@ -480,14 +499,17 @@ __device__ __forceinline__ void bc_wall_point(const GView &B, const FaceRange &f
   B.cv[2 * np + p] = 0;
   B.cv[3 * np + p] = 0;
 #pragma unroll
-  for (int l = 0; l < NS; ++l) B.cv[(l + 5) * np + p] = B.yk[l * np + p] * B.bv[p];
+  for (int l = 0; l < NS; ++l) B.cv[(l + 5) * np + p] = Yw[l] * B.bv[p];
   total_energy(B, sp, S, p);
   for (int g = 1; g <= kNgg; ++g) {
     const long long qi = pidx(B, i - g * dir[0], j - g * dir[1], k - g * dir[2]);
     const long long qg = pidx(B, i + g * dir[0], j + g * dir[1], k + g * dir[2]);
 #pragma unroll
-    for (int l = 0; l < NS; ++l) B.yk[l * np + qg] = B.yk[l * np + qi];
-    mw = mix_mw(B, sp, S, qg);
+    for (int l = 0; l < NS; ++l) {
+      Yw[l] = mass_fraction(B, qi, l);
+      if (!B.mixed) B.yk[l * np + qg] = Yw[l];
+    }
+    mw = mix_mw(sp, S, Yw);
     const double u_i = B.bv[np + qi], v_i = B.bv[2 * np + qi], w_i = B.bv[3 * np + qi];
     const double p_i = B.bv[4 * np + qi], t_i = B.bv[5 * np + qi];
     double t_g = t_i;
@@ -507,7 +529,7 @@ __device__ __forceinline__ void bc_wall_point(const GView &B, const FaceRange &f
     B.cv[2 * np + qg] = -rho_g * v_i;
     B.cv[3 * np + qg] = -rho_g * w_i;
 #pragma unroll
-    for (int l = 0; l < NS; ++l) B.cv[(l + 5) * np + qg] = B.yk[l * np + qg] * rho_g;
+    for (int l = 0; l < NS; ++l) B.cv[(l + 5) * np + qg] = Yw[l] * rho_g;
     total_energy(B, sp, S, qg);
   }
 }
@@ -543,8 +565,9 @@ __device__ __forceinline__ void bc_symmetry_point(const GView &B, const FaceRang
     B.bv[5 * np + qg] = B.bv[5 * np + qi];
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
-      B.yk[l * np + qg] = B.yk[l * np + qi];
-      B.cv[(5 + l) * np + qg] = B.bv[qg] * B.yk[l * np + qg];
+      const double y = mass_fraction(B, qi, l);
+      if (!B.mixed) B.yk[l * np + qg] = y;
+      B.cv[(5 + l) * np + qg] = B.bv[qg] * y;
     }
     B.cv[qg] = B.bv[qg];
     for (int c = 0; c < 3; ++c) B.cv[(1 + c) * np + qg] = B.bv[qg] * B.bv[(1 + c) * np + qg];
@@ -580,19 +603,23 @@ __device__ __forceinline__ void exchange_point(const GView &B, const GView &T, c
       B.cv[l * np + p] = a;
       T.cv[l * tnp + t] = a;
     }
+    if (!B.mixed) {
 #pragma unroll
-    for (int l = 0; l < NS; ++l) {
-      const double a = 0.5 * (T.yk[l * tnp + t] + B.yk[l * np + p]);
-      B.yk[l * np + p] = a;
-      T.yk[l * tnp + t] = a;
+      for (int l = 0; l < NS; ++l) {
+        const double a = 0.5 * (T.yk[l * tnp + t] + B.yk[l * np + p]);
+        B.yk[l * np + p] = a;
+        T.yk[l * tnp + t] = a;
+      }
     }
   } else {
 #pragma unroll
     for (int l = 0; l < 6; ++l) B.bv[l * np + p] = T.bv[l * tnp + t];
 #pragma unroll
     for (int l = 0; l < 5 + NS; ++l) B.cv[l * np + p] = T.cv[l * tnp + t];
+    if (!B.mixed) {
 #pragma unroll
-    for (int l = 0; l < NS; ++l) B.yk[l * np + p] = T.yk[l * tnp + t];
+      for (int l = 0; l < NS; ++l) B.yk[l * np + p] = T.yk[l * tnp + t];
+    }
   }
 }
 
@@ -773,8 +800,11 @@ __global__ void __launch_bounds__(kTpb) props_mlp_kernel(const BlockView *__rest
   const float t = float(B.bv[5 * np + p]);
   const float lp = flog<float>(float(B.bv[4 * np + p]));
   float Y[NSA], cp[NSA], winv, out[4 + NSA];
+  {
+    const double ir = 1.0 / B.cv[p];
 #pragma unroll
-  for (int l = 0; l < NS; ++l) Y[l] = float(B.yk[l * np + p]);
+    for (int l = 0; l < NS; ++l) Y[l] = float(B.cv[(5 + l) * np + p] * ir);
+  }
   mixture_gamma_c<float>(sp, t, Y, cp, winv, out);
   float x[kMlpIO];
   const bool outside = mlp_inputs<NS>(mlp, t, lp, Y, x);

@@ -62,7 +62,8 @@ extern "C" {
 
 /* precision modes */
 #define CFX_FP64   0 /* plain: everything in FP64 */
-#define CFX_MIXED  1 /* FP32 reconstruction/fluxes/transport; FP64 Newton, RK and reductions */
+#define CFX_MIXED  1 /* FP32 reconstruction/fluxes/transport; FP64 Newton, RK and reductions;
+                     * Y is carried in the FP64 conserved state only (rho Y_l) */
 
 /* transport models */
 #define CFX_TRANSPORT_WILKE 0
@@ -71,7 +72,10 @@ extern "C" {
 /* what cfx_download returns */
 #define CFX_CV     0 /* n_var x padded          */
 #define CFX_BV     1 /* 6 x padded              */
-#define CFX_YK     2 /* n_spec x padded         */
+#define CFX_YK     2 /* n_spec x padded; CFX_MIXED keeps no FP64 copy of Y
+                        * after the initial upload and returns (rho Y_l) / rho
+                        * from the conserved state (equal, at updated points,
+                        * to the value an FP64 copy would hold) */
 #define CFX_PROPS  3 /* (4 + n_spec) x padded: gamma, c, mu, kappa, rho*D_1..ns */
 #define CFX_DQ     4 /* n_var x interior (mx*my*mz, i fastest) */
 #define CFX_METRIC 5 /* 10 x padded: xi_x,xi_y,xi_z,eta_x..,zeta_z, jacobian */
