@@ -390,6 +390,31 @@ def lattice(sizes, n_spec=9, seed=11, dx=1e-3, cfl=0.5, rk_stages=3, limiter=1, 
     return Case(name, params, sp, blocks, [], [], bvs, yks)
 
 
+def c1_walls(mx=24, my=20, mz=16, n_spec=9, seed=11, dx=1e-3, rk_stages=3, limiter=1):
+    """The C1 fields and species on one block with every boundary kind:
+    inflow at i-min (T = 1400 K, p = 101325 Pa, 150 m/s along x, the mean
+    composition of the field), outflow at i-max, an isothermal no-slip wall
+    (600 K) at j-min, an adiabatic no-slip wall at j-max, slip walls at k-min
+    and k-max."""
+    c = c1(mx, my, mz, n_spec=n_spec, seed=seed, dx=dx, rk_stages=rk_stages, limiter=limiter)
+    blk = c.blocks[0]
+    X, Y, Z = mx - 1, my - 1, mz - 1
+    blk.bc = [
+        ((0, 0, 0), (0, Y, Z), abi.BC_INFLOW, 0),
+        ((X, 0, 0), (X, Y, Z), abi.BC_OUTFLOW, 0),
+        ((0, 0, 0), (X, 0, Z), abi.BC_WALL, 0),
+        ((0, Y, 0), (X, Y, Z), abi.BC_WALL, 1),
+        ((0, 0, 0), (X, Y, 0), abi.BC_SYMMETRY, 0),
+        ((0, 0, Z), (X, Y, Z), abi.BC_SYMMETRY, 0),
+    ]
+    yk = c.yk0[0] if n_spec else None
+    ymean = None if yk is None else np.ascontiguousarray(yk.reshape(n_spec, -1).mean(axis=1))
+    inflow = [dict(mach=-1.0, velocity=150.0, pressure=101325.0, temperature=1400.0, u=1.0, v=0.0, w=0.0,
+                   yk=ymean)]
+    wall = [dict(adiabatic=0, temperature=600.0), dict(adiabatic=1)]
+    return Case("c1_walls", dict(c.params), c.species, [blk], inflow, wall, c.bv0, c.yk0)
+
+
 def c3(nb=(4, 4, 4), lo=16, hi=160, seed=2, n_spec=9, **kw):
     """64 blocks (4 x 4 x 4 lattice) with extents drawn from [16, 160]: about
     40M points, the c1 species and field generator."""

@@ -33,12 +33,20 @@ def field_errors(g, o, case):
     return err
 
 
-@pytest.mark.parametrize("name", ["c1s", "c0"])
+def mixed_case(name):
+    if name == "c1s":
+        return cases.c1(24, 20, 16)
+    if name == "c1s_walls":  # inflow, outflow, isothermal and adiabatic walls, slip walls, 9 species
+        return cases.c1_walls(22, 18, 14)
+    return cases.c0()
+
+
+@pytest.mark.parametrize("name", ["c1s", "c0", "c1s_walls"])
 def test_mixed_precision_per_field_error(name):
     from combustioncodegpu_amd.lib import Solver
     from oracle.pyoracle import Oracle
-    case = cases.c1(24, 20, 16) if name == "c1s" else cases.c0()
-    mixed = cases.c1(24, 20, 16) if name == "c1s" else cases.c0()
+    case = mixed_case(name)
+    mixed = mixed_case(name)
     mixed.params["precision"] = abi.MIXED
     print(f"[mixed] case={name}", flush=True)
     o, g = Oracle(case), Solver(mixed)

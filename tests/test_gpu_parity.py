@@ -41,11 +41,13 @@ def make(name):
         return cases.c1(18, 16, 12, rk_stages=1)
     if name == "c1s_inviscid":
         return cases.c1(18, 16, 12, viscous=0)
+    if name == "c1s_walls":
+        return cases.c1_walls(22, 18, 14)
     raise KeyError(name)
 
 
 CASES = ["c0", "c0_euler", "c0_va", "c1s", "c1s_minmod", "c1s_ns2", "c1s_air", "c1s_curv_ref",
-         "c1s_curv_corr", "c1s_euler", "c1s_inviscid"]
+         "c1s_curv_corr", "c1s_euler", "c1s_inviscid", "c1s_walls"]
 
 
 def pair(name):
@@ -104,7 +106,8 @@ def test_one_step_parity(name):
     assert rc == 0 and nfail == 0
 
 
-@pytest.mark.parametrize("name,steps", [("c0", 20), ("c1s", 8), ("c1s_curv_corr", 5), ("c0_euler", 20)])
+@pytest.mark.parametrize("name,steps", [("c0", 20), ("c1s", 8), ("c1s_curv_corr", 5), ("c0_euler", 20),
+                                        ("c1s_walls", 5)])
 def test_multi_step_parity(name, steps):
     case, o, g = pair(name)
     o.step(steps)
