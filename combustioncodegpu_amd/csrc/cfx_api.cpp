@@ -542,7 +542,8 @@ int do_props(cfx_ctx *ctx) {
     auto fn = g_kernels32[ctx->ns].props_mlp;
     CFX_TRY(per_batch(ctx, ctx->t_pad, [&](const Batch &b) {
       return ctx->timed_launch(K_PROPS, [&] {
-        return fn(b, ctx->sp_dev, ctx->S, ctx->mlp_dev, ctx->mlp_ops, ctx->mlp_bias, ctx->mlp_counters, ctx->stream);
+        return fn(b, ctx->sp_dev, ctx->S, ctx->mlp_dev, ctx->mlp_ops_inf, ctx->mlp_bias_inf, ctx->mlp_counters,
+                  ctx->stream);
       });
     }));
     return 0;

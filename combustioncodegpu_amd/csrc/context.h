@@ -92,8 +92,10 @@ struct cfx_ctx {
   int transport = CFX_TRANSPORT_WILKE;
   MlpShape mlp_shape{};
   float *mlp_params = nullptr, *mlp_m = nullptr, *mlp_v = nullptr;
-  void *mlp_ops = nullptr;
+  void *mlp_ops = nullptr;      // training operands
   float *mlp_bias = nullptr;
+  void *mlp_ops_inf = nullptr;  // inference operands (folded sigmoid factor)
+  float *mlp_bias_inf = nullptr;
   MlpDev mlp_host{};
   MlpDev *mlp_dev = nullptr;
   unsigned long long *mlp_counters = nullptr;

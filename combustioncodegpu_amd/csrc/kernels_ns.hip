@@ -775,11 +775,117 @@ struct Register64 {
 // ----------------------------------------------- properties, MLP transport ---
 // The mixed-precision properties with mu, kappa and rho*D from the surrogate.
 // Each wavefront evaluates its 64 points as four 16-point MFMA tiles (inputs
-// and outputs pass through LDS between the point-per-lane and the tile
-// layouts).  Points outside the training envelope take the Wilke routine;
-// one wavefront in check_every also evaluates Wilke at all its points and
-// keeps Wilke where the surrogate is off by more than check_tol.
+// and outputs pass through a per-wavefront LDS buffer between the
+// point-per-lane and the tile layouts).  Points outside the training envelope
+// take the Wilke routine; one wavefront in check_every also evaluates Wilke at
+// all its points and keeps Wilke where the surrogate is off by more than
+// check_tol.  The work runs in three phases so that each holds only its own
+// registers: (1) FP32 primitives, Y, gamma, c and the network inputs, with
+// every output but the transport coefficients stored at once; (2) the
+// network; (3) the coefficients, with the Wilke fallback recomputing its
+// inputs from the FP64 state.
 // counters: [0] points outside, [1] points checked, [2] checks failed.
+
+// Orders the LDS accesses of one wavefront (its lanes exchange values through
+// its own slice of LDS, so no workgroup barrier is needed).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Phase 1 from the FP32 primitives b32 (rho, u, v, w, p, T) and Y of padded
+// point p (byte offset o4 in the FP32 arrays): the network inputs go to
+// xl[lane], and with store set gamma, c and the FP32 copies are written.
+// Returns whether the point lies outside the training envelope.
+__device__ __forceinline__ bool mlp_phase1(const GView &B, SpeciesPtrT<float> sp, const MlpDev *__restrict__ mlp,
+                                           const float *b32, const float *Y, float (*xl)[kMlpIO + 1], bool store,
+                                           unsigned o4) {
+  const int lane = threadIdx.x & 63;
+  const long long np = B.np;
+  float cp[NSA], winv, gc[2], x[kMlpIO];
+  mixture_gamma_c<float>(sp, b32[5], Y, cp, winv, gc);
+  const bool outside = mlp_inputs<NS>(mlp, b32[5], flog<float>(b32[4]), Y, x);
+#pragma unroll
+  for (int f = 0; f < kMlpIO; ++f) xl[lane][f] = x[f];
+  if (store) {
+    *atg<float>(B.props32, o4) = gc[0];
+    *atg<float>(B.props32 + np, o4) = gc[1];
+#pragma unroll
+    for (int v = 0; v < 6; ++v) *atg<float>(B.bv32 + v * np, o4) = b32[v];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) *atg<float>(B.yk32 + l * np, o4) = Y[l];
+  }
+  return outside;
+}
+
+// Phases 2 and 3 for the wavefront: the network on xl, the coefficients, the
+// Wilke fallback (its inputs reloaded from the FP64 state at byte offset o8),
+// the counters over the lanes with store set, and their stores.
+__device__ __forceinline__ void mlp_phase23(const GView &B, SpeciesPtrT<float> sp, const SchemeDev &S,
+                                            const MlpDev *__restrict__ mlp, const bf16x8 *__restrict__ ops,
+                                            const float *__restrict__ bias, unsigned long long *__restrict__ counters,
+                                            float (*xl)[kMlpIO + 1], bool outside, bool store, unsigned o8,
+                                            unsigned o4) {
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+  wave_sync();
+  {
+    bf16x8 xo[4];
+    f32x4 yv[4];
+#pragma unroll
+    for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) xo[tt][j] = (__bf16)xl[16 * tt + c][4 * g + j], xo[tt][4 + j] = (__bf16)0.0f;
+    wave_sync();
+    mlp_infer_tiles<4>(ops, bias, xo, yv);
+#pragma unroll
+    for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) xl[16 * tt + c][4 * g + r] = yv[tt][r];
+  }
+  wave_sync();
+  // fresh copies of the uniform bases, so that the field addresses of the
+  // first phase need not stay live (in SGPRs) through the network
+  gptr<const double> cvb = B.cv, bvb = B.bv;
+  gptr<float> prb = B.props32;
+  long long npl = B.np;
+  asm volatile("" : "+s"(cvb), "+s"(bvb), "+s"(prb), "+s"(npl));
+  float tr[2 + NSA];
+#pragma unroll
+  for (int o = 0; o < NS + 2; ++o) tr[o] = __expf(xl[lane][o] * mlp->t_std[o] + mlp->t_mean[o]);
+  const int gw = blockIdx.x * (kTpb / 64) + wv;
+  const bool check = (gw % mlp->check_every) == 0;
+  bool fail = false;
+  if (check || outside) {
+    float Y[NSA], cp[NSA], winv, gc[2], w[2 + NSA];
+    const float t = float(ldg<double>(bvb + 5 * npl, o8));
+    const double ir = 1.0 / ldg<double>(cvb, o8);
+#pragma unroll
+    for (int l = 0; l < NS; ++l) Y[l] = float(ldg<double>(cvb + (5 + l) * npl, o8) * ir);
+    mixture_gamma_c<float>(sp, t, Y, cp, winv, gc);
+    wilke_point<float>(sp, S, t, Y, cp, winv, w);
+    if (check) {
+#pragma unroll
+      for (int o = 0; o < NS + 2; ++o) fail = fail || !(fabsf(tr[o] - w[o]) <= mlp->check_tol * fabsf(w[o]));
+    }
+    if (outside || fail) {
+#pragma unroll
+      for (int o = 0; o < NS + 2; ++o) tr[o] = w[o];
+    }
+  }
+  const unsigned long long n_out = __ballot(store && outside), n_chk = __ballot(store && check),
+                           n_fail = __ballot(store && fail);
+  if (lane == 0) {
+    if (n_out) atomicAdd(&counters[0], (unsigned long long)__popcll(n_out));
+    if (n_chk) atomicAdd(&counters[1], (unsigned long long)__popcll(n_chk));
+    if (n_fail) atomicAdd(&counters[2], (unsigned long long)__popcll(n_fail));
+  }
+  if (!store) return;
+#pragma unroll
+  for (int o = 0; o < NS + 2; ++o) *atg<float>(prb + (2 + o) * npl, o4) = tr[o];
+}
+
+// Surrogate properties over the padded range.
 __global__ void __launch_bounds__(kTpb) props_mlp_kernel(const BlockView *__restrict__ views,
                                                          const int *__restrict__ off, int nb,
                                                          const SpeciesDev *__restrict__ spg, SchemeDev S,
@@ -787,76 +893,27 @@ __global__ void __launch_bounds__(kTpb) props_mlp_kernel(const BlockView *__rest
                                                          const bf16x8 *__restrict__ ops,
                                                          const float *__restrict__ bias,
                                                          unsigned long long *__restrict__ counters) {
-  __shared__ float xl[kTpb / 64][64][kMlpIO + 1];
-  __shared__ float yl[kTpb / 64][64][kMlpIO + 1];
+  __shared__ float buf[kTpb / 64][64][kMlpIO + 1];
   const SpeciesPtrT<float> sp = const_table<float>(spg);
   const PointWork pw = point_work(off, nb);
   const GView B = gview(views[pw.blk]);
   const long long np = B.np;
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
   const long long p0 = pw.base + threadIdx.x;
   const bool valid = p0 < np;
   const long long p = valid ? p0 : np - 1;  // every lane takes part in the MFMA tiles
-  const float t = float(B.bv[5 * np + p]);
-  const float lp = flog<float>(float(B.bv[4 * np + p]));
-  float Y[NSA], cp[NSA], winv, out[4 + NSA];
+  // field arrays are addressed by a wave-uniform base and a 32-bit byte offset
+  const unsigned o8 = unsigned(p) * 8u, o4 = unsigned(p) * 4u;
+  bool outside;
   {
-    const double ir = 1.0 / B.cv[p];
+    float b32[6], Y[NSA];
 #pragma unroll
-    for (int l = 0; l < NS; ++l) Y[l] = float(B.cv[(5 + l) * np + p] * ir);
+    for (int v = 0; v < 6; ++v) b32[v] = float(ldg<double>(B.bv + v * np, o8));
+    const double ir = 1.0 / ldg<double>(B.cv, o8);
+#pragma unroll
+    for (int l = 0; l < NS; ++l) Y[l] = float(ldg<double>(B.cv + (5 + l) * np, o8) * ir);
+    outside = mlp_phase1(B, sp, mlp, b32, Y, buf[threadIdx.x >> 6], valid, o4);
   }
-  mixture_gamma_c<float>(sp, t, Y, cp, winv, out);
-  float x[kMlpIO];
-  const bool outside = mlp_inputs<NS>(mlp, t, lp, Y, x);
-#pragma unroll
-  for (int f = 0; f < kMlpIO; ++f) xl[wv][lane][f] = x[f];
-  __syncthreads();
-  bf16x8 xo[4];
-  f32x4 yv[4];
-#pragma unroll
-  for (int tt = 0; tt < 4; ++tt)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) xo[tt][j] = (__bf16)xl[wv][16 * tt + c][4 * g + j], xo[tt][4 + j] = (__bf16)0.0f;
-  mlp_infer_tiles<4>(ops, bias, xo, yv);
-#pragma unroll
-  for (int tt = 0; tt < 4; ++tt)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) yl[wv][16 * tt + c][4 * g + r] = yv[tt][r];
-  __syncthreads();
-#pragma unroll
-  for (int o = 0; o < NS + 2; ++o) out[2 + o] = __expf(yl[wv][lane][o] * mlp->t_std[o] + mlp->t_mean[o]);
-  const int gw = blockIdx.x * (kTpb / 64) + wv;
-  const bool check = (gw % mlp->check_every) == 0;
-  bool fail = false;
-  if (check || outside) {
-    float w[2 + NSA];
-    wilke_point<float>(sp, S, t, Y, cp, winv, w);
-    if (check) {
-#pragma unroll
-      for (int o = 0; o < NS + 2; ++o) fail = fail || !(fabsf(out[2 + o] - w[o]) <= mlp->check_tol * fabsf(w[o]));
-    }
-    if (outside || fail) {
-#pragma unroll
-      for (int o = 0; o < NS + 2; ++o) out[2 + o] = w[o];
-    }
-  }
-  const unsigned long long n_out = __ballot(valid && outside), n_chk = __ballot(valid && check),
-                           n_fail = __ballot(valid && fail);
-  if (lane == 0) {
-    if (n_out) atomicAdd(&counters[0], (unsigned long long)__popcll(n_out));
-    if (n_chk) atomicAdd(&counters[1], (unsigned long long)__popcll(n_chk));
-    if (n_fail) atomicAdd(&counters[2], (unsigned long long)__popcll(n_fail));
-  }
-  if (!valid) return;
-  float b32[6];
-#pragma unroll
-  for (int v = 0; v < 6; ++v) b32[v] = float(B.bv[v * np + p]);
-#pragma unroll
-  for (int v = 0; v < 4 + NS; ++v) B.props32[v * np + p] = out[v];
-#pragma unroll
-  for (int v = 0; v < 6; ++v) B.bv32[v * np + p] = b32[v];
-#pragma unroll
-  for (int l = 0; l < NS; ++l) B.yk32[l * np + p] = Y[l];
+  mlp_phase23(B, sp, S, mlp, ops, bias, counters, buf[threadIdx.x >> 6], outside, valid, o8, o4);
 }
 
 int launch_props_mlp(const Batch &b, const SpeciesDev *sp, const SchemeDev &S, const MlpDev *mlp, const void *ops,

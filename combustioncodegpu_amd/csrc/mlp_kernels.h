@@ -18,7 +18,10 @@ int launch_mlp_train(const float *xs, const float *ts, long long n_train, long l
 // Adam step with bias corrections c1 = 1/(1-b1^t), c2 = 1/(1-b2^t)
 int launch_mlp_adam(float *params, float *m, float *v, const float *partial, int n_partial, int P, float lr,
                     float b1, float b2, float eps, float c1, float c2, float *grad_out, hipStream_t st);
-int launch_mlp_pack(const float *params, const MlpShape &shp, void *ops, float *bias, hipStream_t st);
+// fold = 0: the training operands; fold = 1: the inference operands (the
+// sigmoid's exponent factor folded in, see mlp.h)
+int launch_mlp_pack(const float *params, const MlpShape &shp, void *ops, float *bias, int fold, hipStream_t st);
+// ops/bias: the inference operands
 int launch_mlp_predict(const float *xs, long long n, const void *ops, const float *bias, int nout, float *ys,
                        hipStream_t st);
 int launch_mlp_moments(const double *x, long long n, int nf, int cross, int groups, double *out, hipStream_t st);

@@ -175,9 +175,14 @@ __global__ void __launch_bounds__(256) mlp_adam_kernel(float *__restrict__ param
 }
 
 // BF16 A operands of the forward and backward products, and the biases
-// padded to 64 per layer, from the FP32 master parameters.
+// padded to 64 per layer, from the FP32 master parameters.  fold: the
+// inference operands, with -log2(e) on the layer-1 weights and the layer 1-3
+// biases and -ln(2) on the output weights (float products, then BF16).
+constexpr float kFold = -1.4426950408889634f, kUnfold = -0.69314718055994531f;
+
 __global__ void __launch_bounds__(256) mlp_pack_kernel(const float *__restrict__ p, MlpShape shp,
-                                                       __bf16 *__restrict__ ops, float *__restrict__ bias) {
+                                                       __bf16 *__restrict__ ops, float *__restrict__ bias,
+                                                       int fold) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e < 4 * kMlpH) {
     const int layer = e / kMlpH, row = e % kMlpH;
@@ -186,7 +191,7 @@ __global__ void __launch_bounds__(256) mlp_pack_kernel(const float *__restrict__
     else if (layer == 1) b = p[shp.b2() + row];
     else if (layer == 2) b = p[shp.b3() + row];
     else if (row < shp.nout) b = p[shp.b4() + row];
-    bias[e] = b;
+    bias[e] = fold && layer < 3 ? b * kFold : b;
   }
   if (e >= kNBlocks * kOpBlock) return;
   const int blk = e / kOpBlock, lane = (e % kOpBlock) / 8, j = e % 8;
@@ -195,6 +200,7 @@ __global__ void __launch_bounds__(256) mlp_pack_kernel(const float *__restrict__
   if (blk < kF2) {  // W1, M-block blk
     const int row = 16 * blk + c;
     if (k < shp.nin) val = p[shp.w1() + row * shp.nin + k];
+    if (fold) val *= kFold;
   } else if (blk < kF4) {  // W2 / W3
     const int q = blk < kF3 ? blk - kF2 : blk - kF3, mb = q / 2, kb = q % 2;
     const int w = blk < kF3 ? shp.w2() : shp.w3();
@@ -202,6 +208,7 @@ __global__ void __launch_bounds__(256) mlp_pack_kernel(const float *__restrict__
   } else if (blk < kB4) {  // W4, K-block
     const int kb = blk - kF4;
     if (c < shp.nout) val = p[shp.w4() + c * kMlpH + 32 * kb + k];
+    if (fold) val *= kUnfold;
   } else if (blk < kB3) {  // W4^T: rows = hidden units, k = outputs
     const int mb = blk - kB4;
     if (k < shp.nout) val = p[shp.w4() + k * kMlpH + 16 * mb + c];
@@ -213,8 +220,9 @@ __global__ void __launch_bounds__(256) mlp_pack_kernel(const float *__restrict__
   ops[e] = (__bf16)val;
 }
 
-// Normalised predictions of normalised inputs (validation): n a multiple of
-// 16 is not required; the last tile repeats a valid sample.
+// Normalised predictions of normalised inputs (validation), on the folded
+// inference operands: n a multiple of 16 is not required; the last tile
+// repeats a valid sample.
 __global__ void __launch_bounds__(64) mlp_predict_kernel(const float *__restrict__ xs, long long n,
                                                          const bf16x8 *__restrict__ ops,
                                                          const float *__restrict__ bias, int nout,
@@ -286,8 +294,8 @@ int launch_mlp_adam(float *params, float *m, float *v, const float *partial, int
   return (int)hipGetLastError();
 }
 
-int launch_mlp_pack(const float *params, const MlpShape &shp, void *ops, float *bias, hipStream_t st) {
-  mlp_pack_kernel<<<(kNBlocks * kOpBlock + 255) / 256, 256, 0, st>>>(params, shp, (__bf16 *)ops, bias);
+int launch_mlp_pack(const float *params, const MlpShape &shp, void *ops, float *bias, int fold, hipStream_t st) {
+  mlp_pack_kernel<<<(kNBlocks * kOpBlock + 255) / 256, 256, 0, st>>>(params, shp, (__bf16 *)ops, bias, fold);
   return (int)hipGetLastError();
 }
 

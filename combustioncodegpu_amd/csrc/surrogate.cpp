@@ -89,6 +89,8 @@ int mlp_alloc(cfx_ctx *ctx) {
   ctx->mlp_v = ctx->dalloc<float>(P, rc);
   ctx->mlp_ops = ctx->dalloc<__bf16>((size_t)kNBlocks * kOpBlock, rc);
   ctx->mlp_bias = ctx->dalloc<float>(4 * kMlpH, rc);
+  ctx->mlp_ops_inf = ctx->dalloc<__bf16>((size_t)kNBlocks * kOpBlock, rc);
+  ctx->mlp_bias_inf = ctx->dalloc<float>(4 * kMlpH, rc);
   ctx->mlp_dev = ctx->dalloc<MlpDev>(1, rc);
   ctx->mlp_counters = ctx->dalloc<unsigned long long>(3, rc);
   if (rc) return ctx->fail(rc, "hipMalloc surrogate");
@@ -125,7 +127,8 @@ void envelope_quadratic(MlpDev &m) {
 int upload_mlp(cfx_ctx *ctx) {
   envelope_quadratic(ctx->mlp_host);
   CFX_TRY((int)hipMemcpy(ctx->mlp_dev, &ctx->mlp_host, sizeof(MlpDev), hipMemcpyHostToDevice));
-  CFX_TRY(launch_mlp_pack(ctx->mlp_params, ctx->mlp_shape, ctx->mlp_ops, ctx->mlp_bias, ctx->stream));
+  CFX_TRY(launch_mlp_pack(ctx->mlp_params, ctx->mlp_shape, ctx->mlp_ops, ctx->mlp_bias, 0, ctx->stream));
+  CFX_TRY(launch_mlp_pack(ctx->mlp_params, ctx->mlp_shape, ctx->mlp_ops_inf, ctx->mlp_bias_inf, 1, ctx->stream));
   CFX_TRY((int)hipStreamSynchronize(ctx->stream));
   return 0;
 }
@@ -310,7 +313,7 @@ int cfx_mlp_train(cfx_ctx *ctx, const cfx_mlp_config *cfg_in, cfx_mlp_report *re
       const double lr = cfg.lr_final + 0.5 * (cfg.lr - cfg.lr_final) * (1.0 + std::cos(M_PI * frac));
       const double c1 = 1.0 / (1.0 - std::pow(b1, (double)(step + 1)));
       const double c2 = 1.0 / (1.0 - std::pow(b2, (double)(step + 1)));
-      CFX_TRY(launch_mlp_pack(ctx->mlp_params, shp, ctx->mlp_ops, ctx->mlp_bias, ctx->stream));
+      CFX_TRY(launch_mlp_pack(ctx->mlp_params, shp, ctx->mlp_ops, ctx->mlp_bias, 0, ctx->stream));
       CFX_TRY(launch_mlp_train(xs, ts, n_train, b * cfg.batch, offset, mult, waves, ctx->mlp_ops, ctx->mlp_bias,
                                shp, scale, partial, loss, ctx->stream));
       CFX_TRY(launch_mlp_adam(ctx->mlp_params, ctx->mlp_m, ctx->mlp_v, partial, waves, P, (float)lr, (float)b1,
@@ -325,7 +328,8 @@ int cfx_mlp_train(cfx_ctx *ctx, const cfx_mlp_config *cfg_in, cfx_mlp_report *re
       }
     }
   }
-  CFX_TRY(launch_mlp_pack(ctx->mlp_params, shp, ctx->mlp_ops, ctx->mlp_bias, ctx->stream));
+  CFX_TRY(launch_mlp_pack(ctx->mlp_params, shp, ctx->mlp_ops, ctx->mlp_bias, 0, ctx->stream));
+  CFX_TRY(launch_mlp_pack(ctx->mlp_params, shp, ctx->mlp_ops_inf, ctx->mlp_bias_inf, 1, ctx->stream));
   (void)hipEventRecord(e2, ctx->stream);
   // ---- validation on the held-out samples
   double rmse = 0.0, vmax[3] = {0, 0, 0}, vp99[3] = {0, 0, 0};
@@ -333,7 +337,8 @@ int cfx_mlp_train(cfx_ctx *ctx, const cfx_mlp_config *cfg_in, cfx_mlp_report *re
     float *ys = sc.get<float>((size_t)n_val * kMlpIO, rc);
     if (rc) return ctx->fail(rc, "hipMalloc validation");
     CFX_TRY((int)hipMemsetAsync(ys, 0, sizeof(float) * n_val * kMlpIO, ctx->stream));
-    CFX_TRY(launch_mlp_predict(xs + n_train * kMlpIO, n_val, ctx->mlp_ops, ctx->mlp_bias, nout, ys, ctx->stream));
+    CFX_TRY(launch_mlp_predict(xs + n_train * kMlpIO, n_val, ctx->mlp_ops_inf, ctx->mlp_bias_inf, nout, ys,
+                               ctx->stream));
     std::vector<float> hy((size_t)n_val * kMlpIO), ht2((size_t)n_val * kMlpIO);
     CFX_TRY((int)hipMemcpyAsync(hy.data(), ys, sizeof(float) * hy.size(), hipMemcpyDeviceToHost, ctx->stream));
     CFX_TRY((int)hipMemcpyAsync(ht2.data(), ts + n_train * kMlpIO, sizeof(float) * ht2.size(),
@@ -454,7 +459,7 @@ int cfx_mlp_gradient(cfx_ctx *ctx, const float *x, const float *t, int n, float 
   CFX_TRY((int)hipStreamSynchronize(ctx->stream));
   CFX_TRY((int)hipMemcpy(dx, x, sizeof(float) * n * kMlpIO, hipMemcpyHostToDevice));
   CFX_TRY((int)hipMemcpy(dt, t, sizeof(float) * n * kMlpIO, hipMemcpyHostToDevice));
-  CFX_TRY(launch_mlp_pack(ctx->mlp_params, shp, ctx->mlp_ops, ctx->mlp_bias, ctx->stream));
+  CFX_TRY(launch_mlp_pack(ctx->mlp_params, shp, ctx->mlp_ops, ctx->mlp_bias, 0, ctx->stream));
   const float scale = 1.0f / ((float)n * (float)shp.nout);
   CFX_TRY(launch_mlp_train(dx, dt, n, 0, 0, 1, waves, ctx->mlp_ops, ctx->mlp_bias, shp, scale, partial, loss,
                            ctx->stream));
@@ -482,7 +487,7 @@ int cfx_mlp_predict(cfx_ctx *ctx, const float *x, int n, float *y) {
   CFX_TRY((int)hipStreamSynchronize(ctx->stream));
   CFX_TRY((int)hipMemcpy(dx, x, sizeof(float) * n * kMlpIO, hipMemcpyHostToDevice));
   CFX_TRY((int)hipMemset(dy, 0, sizeof(float) * n * kMlpIO));
-  CFX_TRY(launch_mlp_predict(dx, n, ctx->mlp_ops, ctx->mlp_bias, ctx->mlp_shape.nout, dy, ctx->stream));
+  CFX_TRY(launch_mlp_predict(dx, n, ctx->mlp_ops_inf, ctx->mlp_bias_inf, ctx->mlp_shape.nout, dy, ctx->stream));
   CFX_TRY((int)hipStreamSynchronize(ctx->stream));
   CFX_TRY((int)hipMemcpy(y, dy, sizeof(float) * n * kMlpIO, hipMemcpyDeviceToHost));
   return 0;

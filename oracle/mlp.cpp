@@ -10,6 +10,15 @@
 // the backward pass.  With emulate == 0 it is plain double arithmetic, which
 // the tests pin with central finite differences.
 //
+// The forward pass of inference (orc_mlp_forward) follows the GPU's
+// inference operands: the factor -log2(e) of the sigmoid's exponent is folded
+// into the layer-1 weights and the biases of layers 1-3, so that the
+// pre-activations are u = -log2(e) z and the activations carry the same
+// factor, h' = u / (1 + 2^u) = -log2(e) SiLU(z); the output weights carry
+// -ln(2) to undo it.  The scaled parameters are float products, as on the
+// GPU.  emulate = 2 folds without rounding (equal to emulate = 0 up to the
+// float scaling of the parameters).
+//
 // Parameters (float, flat): W1[64][nin] b1[64] W2[64][64] b2[64] W3[64][64]
 // b3[64] W4[nout][64] b4[nout].  Samples: rows of 16 floats (zero padded).
 #include <cmath>
@@ -38,11 +47,15 @@ double dsilu(double z) {
   return s * (1.0 + z * (1.0 - s));
 }
 
+constexpr float kFold = -1.4426950408889634f;    // -log2(e), as the GPU's float
+constexpr float kUnfold = -0.69314718055994531f;  // -ln(2)
+
 struct Net {
   int nin, nout;
+  bool folded = false;
   std::vector<double> W1, b1, W2, b2, W3, b3, W4, b4;  // rounded if emulating
   int o_w1, o_b1, o_w2, o_b2, o_w3, o_b3, o_w4, o_b4, count;
-  Net(const float *p, int nin_, int nout_, bool em) : nin(nin_), nout(nout_) {
+  Net(const float *p, int nin_, int nout_, bool em, bool fold = false) : nin(nin_), nout(nout_), folded(fold) {
     o_w1 = 0;
     o_b1 = H * nin;
     o_w2 = o_b1 + H;
@@ -52,20 +65,26 @@ struct Net {
     o_w4 = o_b3 + H;
     o_b4 = o_w4 + nout * H;
     count = o_b4 + nout;
-    auto take = [&](int off, int n, bool round) {
+    auto take = [&](int off, int n, bool round, float scale) {
       std::vector<double> v(n);
-      for (int i = 0; i < n; ++i) v[i] = round ? bf16(p[off + i]) : (double)p[off + i];
+      for (int i = 0; i < n; ++i) {
+        const float f = scale == 1.0f ? p[off + i] : p[off + i] * scale;
+        v[i] = round ? bf16(f) : (double)f;
+      }
       return v;
     };
-    W1 = take(o_w1, H * nin, em);
-    b1 = take(o_b1, H, false);
-    W2 = take(o_w2, H * H, em);
-    b2 = take(o_b2, H, false);
-    W3 = take(o_w3, H * H, em);
-    b3 = take(o_b3, H, false);
-    W4 = take(o_w4, nout * H, em);
-    b4 = take(o_b4, nout, false);
+    const float s = fold ? kFold : 1.0f;
+    W1 = take(o_w1, H * nin, em, s);
+    b1 = take(o_b1, H, false, s);
+    W2 = take(o_w2, H * H, em, 1.0f);
+    b2 = take(o_b2, H, false, s);
+    W3 = take(o_w3, H * H, em, 1.0f);
+    b3 = take(o_b3, H, false, s);
+    W4 = take(o_w4, nout * H, em, fold ? kUnfold : 1.0f);
+    b4 = take(o_b4, nout, false, 1.0f);
   }
+  // activation of pre-activation s: SiLU(s), or u / (1 + 2^u) when folded
+  double act(double v) const { return folded ? v / (1.0 + std::exp2(v)) : silu(v); }
 };
 
 struct Acts {
@@ -79,19 +98,19 @@ void forward(const Net &n, const float *x, bool em, Acts &a) {
     double s = n.b1[o];
     for (int i = 0; i < n.nin; ++i) s += n.W1[o * n.nin + i] * a.h0[i];
     a.z1[o] = s;
-    a.h1[o] = r(silu(s));
+    a.h1[o] = r(n.act(s));
   }
   for (int o = 0; o < H; ++o) {
     double s = n.b2[o];
     for (int i = 0; i < H; ++i) s += n.W2[o * H + i] * a.h1[i];
     a.z2[o] = s;
-    a.h2[o] = r(silu(s));
+    a.h2[o] = r(n.act(s));
   }
   for (int o = 0; o < H; ++o) {
     double s = n.b3[o];
     for (int i = 0; i < H; ++i) s += n.W3[o * H + i] * a.h2[i];
     a.z3[o] = s;
-    a.h3[o] = r(silu(s));
+    a.h3[o] = r(n.act(s));
   }
   for (int o = 0; o < n.nout; ++o) {
     double s = n.b4[o];
@@ -104,13 +123,16 @@ void forward(const Net &n, const float *x, bool em, Acts &a) {
 
 extern "C" {
 
-// y[n][16]: network outputs (zero beyond nout)
+// y[n][16]: network outputs (zero beyond nout); emulate: 0 exact, 1 the GPU's
+// inference (folded, BF16 operands), 2 folded without rounding
 int orc_mlp_forward(const float *params, int nin, int nout, const float *x, int n, int emulate, double *y) {
   if (nin < 1 || nin > IO || nout < 1 || nout > IO || n < 0) return -1;
-  const Net net(params, nin, nout, emulate != 0);
+  if (emulate < 0 || emulate > 2) return -1;
+  const bool em = emulate == 1;
+  const Net net(params, nin, nout, em, emulate != 0);
   for (int s = 0; s < n; ++s) {
     Acts a;
-    forward(net, x + (size_t)s * IO, emulate != 0, a);
+    forward(net, x + (size_t)s * IO, em, a);
     for (int o = 0; o < IO; ++o) y[(size_t)s * IO + o] = o < nout ? a.y[o] : 0.0;
   }
   return 0;

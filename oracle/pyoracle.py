@@ -134,11 +134,13 @@ def _fp(a):
 
 
 def mlp_forward(params, nin, nout, x, emulate=True):
-    """Network outputs y[n][16] of inputs x[n][16] (float rows)."""
+    """Network outputs y[n][16] of inputs x[n][16] (float rows).  emulate:
+    True (or 1) the GPU's inference rounding, False (0) exact, 2 the GPU's
+    folded form of the network without rounding."""
     p = np.ascontiguousarray(params, dtype=np.float32)
     xx = np.ascontiguousarray(x, dtype=np.float32)
     y = np.zeros((xx.shape[0], 16))
-    rc = lib().orc_mlp_forward(_fp(p), nin, nout, _fp(xx), xx.shape[0], 1 if emulate else 0, abi.dptr(y))
+    rc = lib().orc_mlp_forward(_fp(p), nin, nout, _fp(xx), xx.shape[0], int(emulate), abi.dptr(y))
     assert rc == 0
     return y
 

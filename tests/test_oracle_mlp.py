@@ -96,3 +96,22 @@ def test_emulated_gradient_close_to_exact():
     _, ge = pyoracle.mlp_loss_grad(p, nin, nout, x, t, emulate=True)
     rel = np.abs(ge - g).max() / np.abs(g).max()
     assert 1e-5 < rel < 5e-2, rel
+
+
+def test_folded_inference_is_the_same_network():
+    """The GPU's inference form (the sigmoid's exponent factor -log2 e folded
+    into the layer-1 weights and the layer 1-3 biases, -ln 2 into the output
+    weights) is the same function: unrounded it matches the exact network up
+    to the float scaling of the parameters, and with the GPU's BF16 rounding
+    to O(2^-8).  A factor missing from any one bias or weight block moves the
+    outputs by O(1)."""
+    nin = nout = 11
+    p = random_params(nin, nout, 8)
+    x, _ = samples(256, nin, nout, 9)
+    y = pyoracle.mlp_forward(p, nin, nout, x, emulate=False)[:, :nout]
+    yf = pyoracle.mlp_forward(p, nin, nout, x, emulate=2)[:, :nout]
+    ye = pyoracle.mlp_forward(p, nin, nout, x, emulate=True)[:, :nout]
+    scale = np.abs(y).max()
+    assert np.abs(yf - y).max() / scale < 1e-6
+    rel = np.abs(ye - y).max() / scale
+    assert 1e-5 < rel < 5e-2, rel
