@@ -117,7 +117,14 @@ __device__ __forceinline__ void species_h_cp(SpeciesPtrT<R> sp, int l, R t, R &h
 template <typename R>
 __device__ __forceinline__ R species_cp(SpeciesPtrT<R> sp, int l, R t) {
   auto p5 = [&](const auto &c) { return R(c[0]) + t * (R(c[1]) + t * (R(c[2]) + t * (R(c[3]) + t * R(c[4])))); };
-  const R cpp = t >= R(sp->t_mid[l]) ? p5(sp->cpc[1][l]) : p5(sp->cpc[0][l]);
+  R cpp;
+  if constexpr (sizeof(R) == 4) {
+    // both ranges from scalar loads, then a per-lane select
+    const R lo = p5(sp->cpc[0][l]), hi = p5(sp->cpc[1][l]);
+    cpp = on_all_lanes<R>(t >= R(sp->t_mid[l]) ? hi : lo);
+  } else {
+    cpp = t >= R(sp->t_mid[l]) ? p5(sp->cpc[1][l]) : p5(sp->cpc[0][l]);
+  }
   return t < R(sp->t_low[l]) ? R(sp->cp_tlow[l]) : cpp;
 }
 

@@ -819,15 +819,16 @@ __device__ __forceinline__ bool mlp_phase1(const GView &B, SpeciesPtrT<float> sp
   return outside;
 }
 
-// Phases 2 and 3 for the wavefront: the network on xl, the coefficients, the
-// Wilke fallback (its inputs reloaded from the FP64 state at byte offset o8),
-// the counters over the lanes with store set, and their stores.
+// Phases 2 and 3 for the wavefront (global index gw): the network on xl with
+// the operands staged in LDS, the coefficients, the Wilke fallback (its
+// inputs reloaded from the FP64 state at byte offset o8), the counters over
+// the lanes with store set, and their stores.
 __device__ __forceinline__ void mlp_phase23(const GView &B, SpeciesPtrT<float> sp, const SchemeDev &S,
-                                            const MlpDev *__restrict__ mlp, const bf16x8 *__restrict__ ops,
-                                            const float *__restrict__ bias, unsigned long long *__restrict__ counters,
+                                            const MlpDev *__restrict__ mlp, lds_ptr<const bf16x8> ops,
+                                            lds_ptr<const f32x4> bias, unsigned long long *__restrict__ counters,
                                             float (*xl)[kMlpIO + 1], bool outside, bool store, unsigned o8,
-                                            unsigned o4) {
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+                                            unsigned o4, int gw) {
+  const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
   wave_sync();
   {
     bf16x8 xo[4];
@@ -853,7 +854,6 @@ __device__ __forceinline__ void mlp_phase23(const GView &B, SpeciesPtrT<float> s
   float tr[2 + NSA];
 #pragma unroll
   for (int o = 0; o < NS + 2; ++o) tr[o] = __expf(xl[lane][o] * mlp->t_std[o] + mlp->t_mean[o]);
-  const int gw = blockIdx.x * (kTpb / 64) + wv;
   const bool check = (gw % mlp->check_every) == 0;
   bool fail = false;
   if (check || outside) {
@@ -885,35 +885,71 @@ __device__ __forceinline__ void mlp_phase23(const GView &B, SpeciesPtrT<float> s
   for (int o = 0; o < NS + 2; ++o) *atg<float>(prb + (2 + o) * npl, o4) = tr[o];
 }
 
-// Surrogate properties over the padded range.
+// Surrogate properties over the padded range.  A persistent grid: workgroup
+// b walks the kTpb-point chunks b, b + gridDim.x, ... of the batch, and loads
+// the FP64 inputs of its next chunk before the network runs on the current
+// one, so that those loads overlap the arithmetic.  The inference operands
+// are staged in LDS once per workgroup: the network then issues no vector
+// memory instruction whose wait would also wait for the prefetch.
+constexpr int kMlpInfBlocks = kB4;  // forward A operands (the first kB4 blocks)
+constexpr int kMlpIn = 7 + NS;      // rho, u, v, w, p, T, rho and rho*Y_l of one point
+
+__device__ __forceinline__ void mlp_fetch(const BlockView *__restrict__ views, const int *__restrict__ off, int nb,
+                                          int w, double *in) {
+  const int gw = w + off[0];
+  const int blk = find_seg(off, nb, gw);
+  const GView B = gview(views[blk]);
+  const long long np = B.np;
+  const long long p0 = (long long)(gw - off[blk]) * kTpb + threadIdx.x;
+  const unsigned o8 = unsigned(p0 < np ? p0 : np - 1) * 8u;
+#pragma unroll
+  for (int v = 0; v < 6; ++v) in[v] = ldg<double>(B.bv + v * np, o8);
+  in[6] = ldg<double>(B.cv, o8);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) in[7 + l] = ldg<double>(B.cv + (5 + l) * np, o8);
+}
+
 __global__ void __launch_bounds__(kTpb) props_mlp_kernel(const BlockView *__restrict__ views,
-                                                         const int *__restrict__ off, int nb,
+                                                         const int *__restrict__ off, int nb, int total,
                                                          const SpeciesDev *__restrict__ spg, SchemeDev S,
                                                          const MlpDev *__restrict__ mlp,
                                                          const bf16x8 *__restrict__ ops,
                                                          const float *__restrict__ bias,
                                                          unsigned long long *__restrict__ counters) {
   __shared__ float buf[kTpb / 64][64][kMlpIO + 1];
+  __shared__ bf16x8 ops_l[kMlpInfBlocks * 64];
+  __shared__ f32x4 bias_l[4 * kMlpH / 4];
+  for (int e = threadIdx.x; e < kMlpInfBlocks * 64; e += kTpb) ops_l[e] = ops[e];
+  if (threadIdx.x < kMlpH) bias_l[threadIdx.x] = reinterpret_cast<const f32x4 *>(bias)[threadIdx.x];
+  __syncthreads();
+  const lds_ptr<const bf16x8> lops = (lds_ptr<const bf16x8>)ops_l;
+  const lds_ptr<const f32x4> lbias = (lds_ptr<const f32x4>)bias_l;
   const SpeciesPtrT<float> sp = const_table<float>(spg);
-  const PointWork pw = point_work(off, nb);
-  const GView B = gview(views[pw.blk]);
-  const long long np = B.np;
-  const long long p0 = pw.base + threadIdx.x;
-  const bool valid = p0 < np;
-  const long long p = valid ? p0 : np - 1;  // every lane takes part in the MFMA tiles
-  // field arrays are addressed by a wave-uniform base and a 32-bit byte offset
-  const unsigned o8 = unsigned(p) * 8u, o4 = unsigned(p) * 4u;
-  bool outside;
-  {
+  float(*xl)[kMlpIO + 1] = buf[threadIdx.x >> 6];
+  double in[kMlpIn];
+  int w = blockIdx.x;
+  mlp_fetch(views, off, nb, w, in);
+  for (; w < total; w += gridDim.x) {
+    const int gw = w + off[0];
+    const int blk = find_seg(off, nb, gw);
+    const GView B = gview(views[blk]);
+    const long long np = B.np;
+    const long long p0 = (long long)(gw - off[blk]) * kTpb + threadIdx.x;
+    const bool valid = p0 < np;
+    const long long p = valid ? p0 : np - 1;  // every lane takes part in the MFMA tiles
+    // field arrays are addressed by a wave-uniform base and a 32-bit byte offset
+    const unsigned o8 = unsigned(p) * 8u, o4 = unsigned(p) * 4u;
     float b32[6], Y[NSA];
 #pragma unroll
-    for (int v = 0; v < 6; ++v) b32[v] = float(ldg<double>(B.bv + v * np, o8));
-    const double ir = 1.0 / ldg<double>(B.cv, o8);
+    for (int v = 0; v < 6; ++v) b32[v] = float(in[v]);
+    const double ir = 1.0 / in[6];
 #pragma unroll
-    for (int l = 0; l < NS; ++l) Y[l] = float(ldg<double>(B.cv + (5 + l) * np, o8) * ir);
-    outside = mlp_phase1(B, sp, mlp, b32, Y, buf[threadIdx.x >> 6], valid, o4);
+    for (int l = 0; l < NS; ++l) Y[l] = float(in[7 + l] * ir);
+    wave_sync();  // the previous chunk's reads of xl come first
+    const bool outside = mlp_phase1(B, sp, mlp, b32, Y, xl, valid, o4);
+    if (w + (int)gridDim.x < total) mlp_fetch(views, off, nb, w + gridDim.x, in);
+    mlp_phase23(B, sp, S, mlp, lops, lbias, counters, xl, outside, valid, o8, o4, w * (kTpb / 64) + (threadIdx.x >> 6));
   }
-  mlp_phase23(B, sp, S, mlp, ops, bias, counters, buf[threadIdx.x >> 6], outside, valid, o8, o4);
 }
 
 int launch_props_mlp(const Batch &b, const SpeciesDev *sp, const SchemeDev &S, const MlpDev *mlp, const void *ops,
@@ -922,8 +958,19 @@ int launch_props_mlp(const Batch &b, const SpeciesDev *sp, const SchemeDev &S, c
     return CFX_E_UNSUPPORTED;
   } else {
     if (b.total <= 0) return 0;
-    props_mlp_kernel<<<b.total, kTpb, 0, st>>>(b.views, b.off, b.nb, sp, S, mlp, (const bf16x8 *)ops, bias,
-                                              counters);
+    // persistent grid: the resident workgroups of the whole device
+    static int resident = 0;
+    if (!resident) {
+      int dev = 0, n_cu = 0, per_cu = 0;
+      (void)hipGetDevice(&dev);
+      if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) n_cu = 256;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, props_mlp_kernel, kTpb, 0) != hipSuccess)
+        per_cu = 1;
+      resident = std::max(1, n_cu * std::max(1, per_cu));
+    }
+    const int grid = std::min(b.total, resident);
+    props_mlp_kernel<<<grid, kTpb, 0, st>>>(b.views, b.off, b.nb, b.total, sp, S, mlp, (const bf16x8 *)ops, bias,
+                                           counters);
     return (int)hipGetLastError();
   }
 }

@@ -142,13 +142,23 @@ __device__ __forceinline__ f32x4 silu4_folded(f32x4 u) {
   return h;
 }
 
+// operands and biases in global memory or (lds_*) staged in LDS
+template <class T>
+using lds_ptr = __attribute__((address_space(3))) T *;
+
 __device__ __forceinline__ bf16x8 load_op(const bf16x8 *__restrict__ ops, int block, int lane) {
+  return ops[block * 64 + lane];
+}
+__device__ __forceinline__ bf16x8 load_op(lds_ptr<const bf16x8> ops, int block, int lane) {
   return ops[block * 64 + lane];
 }
 
 // bias rows 4g..4g+3 of M-block mb of layer l (biases padded to 64 per layer)
 __device__ __forceinline__ f32x4 load_bias(const float *__restrict__ bias, int layer, int mb, int lane) {
   return *reinterpret_cast<const f32x4 *>(bias + layer * kMlpH + 16 * mb + 4 * (lane >> 4));
+}
+__device__ __forceinline__ f32x4 load_bias(lds_ptr<const f32x4> bias, int layer, int mb, int lane) {
+  return bias[(layer * kMlpH + 16 * mb) / 4 + (lane >> 4)];
 }
 
 // Activations of one 16-point tile kept for the backward pass.
@@ -223,9 +233,8 @@ __device__ __forceinline__ f32x4 mlp_infer_tile(const bf16x8 *__restrict__ ops, 
 
 // Inference of NTL tiles at once on the folded operands, weight-stationary:
 // every A operand is fetched once and applied to all tiles.
-template <int NTL>
-__device__ __forceinline__ void mlp_infer_tiles(const bf16x8 *__restrict__ ops, const float *__restrict__ bias,
-                                                const bf16x8 *x, f32x4 *y) {
+template <int NTL, class OpPtr, class BiasPtr>
+__device__ __forceinline__ void mlp_infer_tiles(OpPtr ops, BiasPtr bias, const bf16x8 *x, f32x4 *y) {
   const int lane = threadIdx.x & 63;
   f32x4 z[NTL][4];
   bf16x8 h[NTL][2];
