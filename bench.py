@@ -70,36 +70,31 @@ def cpu_baseline(case, threads):
                       f"{cs} cell-stages in {dt:.2f} s, OpenMP on {used} host threads"}
 
 
-def measure(case, args, name, precision="fp64", batched=True, graph=False, autotune=False, roofline=True,
-            mlp=False):
-    """Throughput of one execution form on a case: warmup, then exactly
-    args.steps steps between synchronisations, wall clock; per-kernel times
-    from HIP events in a separate profiled run of 3 eager steps.  With mlp
-    the surrogate is trained on the GPU first (outside the timed region) and
-    selected as the transport."""
+def prepare(case, args, name, precision="fp64", batched=True, graph=False, autotune=False, mlp=False, **_):
+    """A context of one execution form, ready to step: surrogate trained (with
+    mlp), tiles autotuned (with autotune), warmed up and health-checked."""
     from combustioncodegpu_amd import abi
     from combustioncodegpu_amd.lib import Solver
     case.params["precision"] = abi.MIXED if precision == "mixed" else abi.FP64
     g = Solver(case)
     case.params["precision"] = abi.FP64
-    mlp_report = None
+    info = {}
     if mlp:
         log(f"[bench] {name}: training the transport surrogate")
         t0 = time.perf_counter()
-        mlp_report = g.mlp_train()
-        mlp_report["wall_s"] = round(time.perf_counter() - t0, 2)
+        info["mlp_report"] = g.mlp_train()
+        info["mlp_report"]["wall_s"] = round(time.perf_counter() - t0, 2)
         g.set_transport(abi.TRANSPORT_MLP)
     g.set_execution(batched=batched, graph=graph)
     g.upload_initial()
     g.init_conservative()
     g.sync()
-    tiles = None
     if autotune:
         log(f"[bench] {name}: autotune")
         var, per, ms = g.autotune(reps=3)
         g.upload_initial()
         g.init_conservative()
-        tiles = {"variant": var, "persist": per, "rhs_ms_per_direction": [round(x, 4) for x in ms]}
+        info["tiles"] = {"variant": var, "persist": per, "rhs_ms_per_direction": [round(x, 4) for x in ms]}
     log(f"[bench] {name}: warmup")
     g.step(args.warmup)
     g.sync()
@@ -107,54 +102,85 @@ def measure(case, args, name, precision="fp64", batched=True, graph=False, autot
     if rc != 0:
         log(f"[bench] {name}: health check failed rc={rc} newton_fail={nfail}")
         g.close()
-        return None
-    log(f"[bench] {name}: timed region ({args.steps} steps)")
-    g.sync()
-    t0 = time.perf_counter()
-    ev_ms = g.time_steps(args.steps)
-    g.sync()
-    wall = time.perf_counter() - t0
-    value = case.n_cells * 3 * args.steps / wall / 1e6
-    # per-kernel durations from HIP events on the context's stream (eager)
-    g.set_profiling(True)
-    g.time_steps(3)
-    kt = g.kernel_times()
-    g.set_profiling(False)
-    n_ops, n_lev = g.schedule_info()
-    out = {
-        "name": name, "value": round(value, 3), "ms_per_step": round(1e3 * wall / args.steps, 4),
-        "event_ms_per_step": round(ev_ms / args.steps, 4),
-        "kernel_ms_per_step": {k: round(v[0] / 3, 4) for k, v in kt.items()},
-        "launches_per_step": {k: round(v[1] / 3, 1) for k, v in kt.items()},
-        "execution": {"batched": batched, "graph": graph, "blocks": len(case.blocks),
-                      "ghost_fill_ops_per_stage": n_ops, "ghost_fill_levels_per_stage": n_lev},
-    }
-    if tiles:
-        out["rhs_tiles"] = tiles
-    if mlp:
-        out["mlp"] = {"training": mlp_report, "counters": g.mlp_stats()}
-    # one timed RHS event per direction launch (batched: all blocks at once)
-    rhs_ms, rhs_n = kt["rhs"]
-    stages = 3 * 3
-    rhs_stage_s = rhs_ms / stages / 1e3
-    if roofline:
-        bpc = BYTES_PER_CELL_STAGE_FP64 if precision == "fp64" else BYTES_PER_CELL_STAGE_MIXED
-        achieved = bpc * case.n_cells / rhs_stage_s / 1e9
-        traffic = None
-        tpath = os.path.join(ROOT, "profiles", f"rhs_c1_{precision}_traffic.json")
-        if os.path.exists(tpath):
-            with open(tpath) as f:
-                traffic = json.load(f).get("hbm_bytes_per_stage")
-        out["roofline"] = {
-            "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-            "kernel": "rhs_pencil_kernel (LDS-staged RHS, one launch per face direction)",
-            "kernel_avg_ms_per_stage": round(rhs_stage_s * 1e3, 4),
-            "launches_per_stage": round(rhs_n / stages, 2), "bytes_per_cell_stage": round(bpc, 2)}
-    out["dt"] = g.get_dt()
-    out["residual"] = [float(x) for x in g.residual()]
-    g.close()
-    return out
+        return None, None
+    return g, info
+
+
+def measure_group(case, args, specs, rounds=5):
+    """Throughput of several execution forms of one case.  Every form is set
+    up first; the timed region then interleaves them: in each of `rounds`
+    rounds every form runs its share of args.steps steps between
+    synchronisations (wall clock), so that all forms see the same clock and
+    thermal history of the GPU; each form's total is exactly args.steps
+    steps.  Per-kernel times come from HIP events in a separate profiled run
+    of 3 eager steps.  specs: (name, keyword arguments) per form; mlp trains
+    the surrogate on the GPU first (outside the timed region)."""
+    ctxs = {}
+    for name, kw in specs:
+        g, info = prepare(case, args, name, **kw)
+        if g is None:
+            for c, _ in ctxs.values():
+                c.close()
+            return None
+        ctxs[name] = (g, info)
+    n_rounds = max(1, min(rounds, args.steps))
+    shares = [args.steps // n_rounds + (1 if r < args.steps % n_rounds else 0) for r in range(n_rounds)]
+    wall = {name: 0.0 for name in ctxs}
+    ev = {name: 0.0 for name in ctxs}
+    log(f"[bench] timed region: {args.steps} steps per form in {n_rounds} interleaved rounds")
+    for k in shares:
+        for name, (g, _) in ctxs.items():
+            g.sync()
+            t0 = time.perf_counter()
+            ev[name] += g.time_steps(k)
+            g.sync()
+            wall[name] += time.perf_counter() - t0
+    outs = {}
+    for (name, kw), (g, info) in zip(specs, ctxs.values()):
+        precision = kw.get("precision", "fp64")
+        value = case.n_cells * 3 * args.steps / wall[name] / 1e6
+        # per-kernel durations from HIP events on the context's stream (eager)
+        g.set_profiling(True)
+        g.time_steps(3)
+        kt = g.kernel_times()
+        g.set_profiling(False)
+        n_ops, n_lev = g.schedule_info()
+        out = {
+            "name": name, "value": round(value, 3), "ms_per_step": round(1e3 * wall[name] / args.steps, 4),
+            "event_ms_per_step": round(ev[name] / args.steps, 4),
+            "kernel_ms_per_step": {k: round(v[0] / 3, 4) for k, v in kt.items()},
+            "launches_per_step": {k: round(v[1] / 3, 1) for k, v in kt.items()},
+            "execution": {"batched": kw.get("batched", True), "graph": kw.get("graph", False),
+                          "blocks": len(case.blocks), "ghost_fill_ops_per_stage": n_ops,
+                          "ghost_fill_levels_per_stage": n_lev},
+        }
+        if "tiles" in info:
+            out["rhs_tiles"] = info["tiles"]
+        if "mlp_report" in info:
+            out["mlp"] = {"training": info["mlp_report"], "counters": g.mlp_stats()}
+        # one timed RHS event per direction launch (batched: all blocks at once)
+        rhs_ms, rhs_n = kt["rhs"]
+        stages = 3 * 3
+        rhs_stage_s = rhs_ms / stages / 1e3
+        if kw.get("roofline", True):
+            bpc = BYTES_PER_CELL_STAGE_FP64 if precision == "fp64" else BYTES_PER_CELL_STAGE_MIXED
+            achieved = bpc * case.n_cells / rhs_stage_s / 1e9
+            traffic = None
+            tpath = os.path.join(ROOT, "profiles", f"rhs_c1_{precision}_traffic.json")
+            if os.path.exists(tpath):
+                with open(tpath) as f:
+                    traffic = json.load(f).get("hbm_bytes_per_stage")
+            out["roofline"] = {
+                "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "kernel": "rhs_pencil_kernel (LDS-staged RHS, one launch per face direction)",
+                "kernel_avg_ms_per_stage": round(rhs_stage_s * 1e3, 4),
+                "launches_per_stage": round(rhs_n / stages, 2), "bytes_per_cell_stage": round(bpc, 2)}
+        out["dt"] = g.get_dt()
+        out["residual"] = [float(x) for x in g.residual()]
+        g.close()
+        outs[name] = out
+    return outs
 
 
 def mixed_errors(case, checkpoints=(1, 10, 100), mlp=True):
@@ -266,18 +292,16 @@ def main():
     t0 = time.perf_counter()
     case = cases.c1(mx, my, mz)
     log(f"[bench] generated in {time.perf_counter() - t0:.1f} s")
-    forms = {}
     # the plain FP64 drop-in: eager launches as the reference's Driver issues them
-    forms["fp64"] = measure(case, args, "fp64", "fp64", batched=False, graph=False)
-    forms["mixed"] = measure(case, args, "mixed", "mixed", batched=False, graph=False)
-    forms["mixed_graph_tuned"] = measure(case, args, "mixed_graph_tuned", "mixed", batched=True, graph=True,
-                                         autotune=True)
-    forms["fp64_graph_tuned"] = measure(case, args, "fp64_graph_tuned", "fp64", batched=True, graph=True,
-                                        autotune=True)
+    specs = [("fp64", dict(precision="fp64", batched=False, graph=False)),
+             ("mixed", dict(precision="mixed", batched=False, graph=False)),
+             ("mixed_graph_tuned", dict(precision="mixed", batched=True, graph=True, autotune=True)),
+             ("fp64_graph_tuned", dict(precision="fp64", batched=True, graph=True, autotune=True))]
     if not args.no_mlp:
-        forms["mixed_graph_tuned_mlp"] = measure(case, args, "mixed_graph_tuned_mlp", "mixed", batched=True,
-                                                 graph=True, autotune=True, mlp=True)
-    if any(v is None for v in forms.values()):
+        specs.append(("mixed_graph_tuned_mlp", dict(precision="mixed", batched=True, graph=True, autotune=True,
+                                                     mlp=True)))
+    forms = measure_group(case, args, specs)
+    if forms is None:
         return 3
     log("[bench] mixed-precision error against the FP64 path")
     errors = mixed_errors(case, mlp=not args.no_mlp)
@@ -285,14 +309,12 @@ def main():
     if not args.no_mlp and not args.no_c4:
         log("[bench] generating the 384x256x256 surrogate workload")
         c4 = cases.c1(384, 256, 256)
-        c4_forms = {
-            "fp64": measure(c4, args, "c4_fp64", "fp64", batched=False, graph=False, roofline=False),
-            "mixed_graph_tuned": measure(c4, args, "c4_mixed", "mixed", batched=True, graph=True, autotune=True,
-                                         roofline=False),
-            "mixed_graph_tuned_mlp": measure(c4, args, "c4_mixed_mlp", "mixed", batched=True, graph=True,
-                                             autotune=True, roofline=False, mlp=True),
-        }
-        if any(v is None for v in c4_forms.values()):
+        c4_forms = measure_group(c4, args, [
+            ("fp64", dict(precision="fp64", batched=False, graph=False, roofline=False)),
+            ("mixed_graph_tuned", dict(precision="mixed", batched=True, graph=True, autotune=True, roofline=False)),
+            ("mixed_graph_tuned_mlp", dict(precision="mixed", batched=True, graph=True, autotune=True,
+                                           roofline=False, mlp=True))])
+        if c4_forms is None:
             return 3
         c4_out = {
             "workload": "3-D block 384x256x256, 9 species, mixed precision, graph-captured, autotuned; "
@@ -313,15 +335,13 @@ def main():
         t0 = time.perf_counter()
         c3 = cases.c3()
         log(f"[bench] generated {c3.n_cells} points in {len(c3.blocks)} blocks in {time.perf_counter() - t0:.1f} s")
-        c3_forms = {
-            "fp64_per_block": measure(c3, args, "c3_fp64_per_block", "fp64", batched=False, graph=False,
-                                      roofline=False),
-            "fp64_batched_graph_tuned": measure(c3, args, "c3_fp64_batched", "fp64", batched=True, graph=True,
-                                                autotune=True, roofline=False),
-            "mixed_batched_graph_tuned": measure(c3, args, "c3_mixed_batched", "mixed", batched=True, graph=True,
-                                                 autotune=True, roofline=False),
-        }
-        if any(v is None for v in c3_forms.values()):
+        c3_forms = measure_group(c3, args, [
+            ("fp64_per_block", dict(precision="fp64", batched=False, graph=False, roofline=False)),
+            ("fp64_batched_graph_tuned", dict(precision="fp64", batched=True, graph=True, autotune=True,
+                                              roofline=False)),
+            ("mixed_batched_graph_tuned", dict(precision="mixed", batched=True, graph=True, autotune=True,
+                                               roofline=False))])
+        if c3_forms is None:
             return 3
         sizes = sorted({(b.mx, b.my, b.mz) for b in c3.blocks})
         c3_out = {

@@ -819,15 +819,15 @@ __device__ __forceinline__ bool mlp_phase1(const GView &B, SpeciesPtrT<float> sp
   return outside;
 }
 
-// Phases 2 and 3 for the wavefront (global index gw): the network on xl with
-// the operands staged in LDS, the coefficients, the Wilke fallback (its
-// inputs reloaded from the FP64 state at byte offset o8), the counters over
-// the lanes with store set, and their stores.
+// Phases 2 and 3 for the wavefront: the network on xl with the operands
+// staged in LDS, the coefficients, the Wilke fallback (its inputs reloaded
+// from the FP64 state at byte offset o8; at every point when check is set),
+// the counters over the lanes with store set, and their stores.
 __device__ __forceinline__ void mlp_phase23(const GView &B, SpeciesPtrT<float> sp, const SchemeDev &S,
                                             const MlpDev *__restrict__ mlp, lds_ptr<const bf16x8> ops,
                                             lds_ptr<const f32x4> bias, unsigned long long *__restrict__ counters,
                                             float (*xl)[kMlpIO + 1], bool outside, bool store, unsigned o8,
-                                            unsigned o4, int gw) {
+                                            unsigned o4, bool check) {
   const int lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
   wave_sync();
   {
@@ -854,7 +854,6 @@ __device__ __forceinline__ void mlp_phase23(const GView &B, SpeciesPtrT<float> s
   float tr[2 + NSA];
 #pragma unroll
   for (int o = 0; o < NS + 2; ++o) tr[o] = __expf(xl[lane][o] * mlp->t_std[o] + mlp->t_mean[o]);
-  const bool check = (gw % mlp->check_every) == 0;
   bool fail = false;
   if (check || outside) {
     float Y[NSA], cp[NSA], winv, gc[2], w[2 + NSA];
@@ -929,7 +928,11 @@ __global__ void __launch_bounds__(kTpb) props_mlp_kernel(const BlockView *__rest
   double in[kMlpIn];
   int w = blockIdx.x;
   mlp_fetch(views, off, nb, w, in);
-  for (; w < total; w += gridDim.x) {
+  // the sampled check takes every check_every-th chunk-wavefront of each
+  // workgroup, phase-shifted by the workgroup, so every workgroup carries
+  // the same share of Wilke evaluations
+  int ck = blockIdx.x + (threadIdx.x >> 6);
+  for (; w < total; w += gridDim.x, ck += kTpb / 64) {
     const int gw = w + off[0];
     const int blk = find_seg(off, nb, gw);
     const GView B = gview(views[blk]);
@@ -948,7 +951,7 @@ __global__ void __launch_bounds__(kTpb) props_mlp_kernel(const BlockView *__rest
     wave_sync();  // the previous chunk's reads of xl come first
     const bool outside = mlp_phase1(B, sp, mlp, b32, Y, xl, valid, o4);
     if (w + (int)gridDim.x < total) mlp_fetch(views, off, nb, w + gridDim.x, in);
-    mlp_phase23(B, sp, S, mlp, lops, lbias, counters, xl, outside, valid, o8, o4, w * (kTpb / 64) + (threadIdx.x >> 6));
+    mlp_phase23(B, sp, S, mlp, lops, lbias, counters, xl, outside, valid, o8, o4, ck % mlp->check_every == 0);
   }
 }
 
