@@ -111,6 +111,20 @@ __device__ __forceinline__ void species_h_cp(SpeciesPtrT<R> sp, int l, R t, R &h
   thermo_h_cp<R>(sp->th[l].v, t, h, cp);
 }
 
+// h_l(T) alone (the same value as thermo_h_cp's h), for loops that need no
+// cp: the marker on the selected cp would otherwise keep both cp
+// polynomials alive.
+template <typename R>
+__device__ __forceinline__ R species_h(SpeciesPtrT<R> sp, int l, R t) {
+  const auto &c = sp->th[l].v;
+  auto p5 = [&](int o) { return R(c[o]) + t * (R(c[o + 1]) + t * (R(c[o + 2]) + t * (R(c[o + 3]) + t * R(c[o + 4])))); };
+  const R h_lo = t * p5(kThH) + R(c[kThH + 5]);
+  const R h_hi = t * p5(kThH + 6) + R(c[kThH + 11]);
+  const R hp = on_all_lanes<R>(t >= R(c[kThTmid]) ? h_hi : h_lo);
+  const R tl = R(c[kThTlow]);
+  return t < tl ? R(c[kThHtlow]) + R(c[kThCptlow]) * (t - tl) : hp;
+}
+
 // cp_l(T) only, frozen below t_low (Thermo.cu:68-82).  Loops over species
 // that need only cp read the per-array layout: the cp coefficients of all
 // species are contiguous there and arrive in a few wide loads.

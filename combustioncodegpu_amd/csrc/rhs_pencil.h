@@ -333,8 +333,13 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
             chain(gcv, dy);
             yb[s] = R(0.5) * (stg[k * BV::V + li] + stg[k * BV::V + li + sa]);
             const R dif = face_ok ? R(0.5) * (ldg<R>(PR_ + (4 + s) * np, pl_b) + ldg<R>(PR_ + (4 + s) * np, pD_b)) : R(0);
-            R h, cp;
-            species_h_cp<R>(sp, s, tm, h, cp);
+            R h;
+            if constexpr (P::F32) {
+              h = species_h<R>(sp, s, tm);
+            } else {  // FP64: the h-only form costs registers (spills) there
+              R cp;
+              species_h_cp<R>(sp, s, tm, h, cp);
+            }
             const R proj = k1 * dy[0] + k2 * dy[1] + k3 * dy[2];
             e4 += dif * h * proj;
             ucx += dif * dy[0];
