@@ -777,13 +777,13 @@ struct Register64 {
 // Each wavefront evaluates its 64 points as four 16-point MFMA tiles (inputs
 // and outputs pass through a per-wavefront LDS buffer between the
 // point-per-lane and the tile layouts).  Points outside the training envelope
-// take the Wilke routine; one wavefront in check_every also evaluates Wilke at
-// all its points and keeps Wilke where the surrogate is off by more than
-// check_tol.  The work runs in three phases so that each holds only its own
-// registers: (1) FP32 primitives, Y, gamma, c and the network inputs, with
-// every output but the transport coefficients stored at once; (2) the
-// network; (3) the coefficients, with the Wilke fallback recomputing its
-// inputs from the FP64 state.
+// take the Wilke routine; one chunk-wavefront in check_every of each
+// workgroup also evaluates Wilke at all its points and keeps Wilke where the
+// surrogate is off by more than check_tol.  The work runs in three phases so
+// that each holds only its own registers: (1) FP32 primitives, Y, gamma, c
+// and the network inputs, with every output but the transport coefficients
+// stored at once; (2) the network; (3) the coefficients, with the Wilke
+// fallback recomputing its inputs from the FP64 state.
 // counters: [0] points outside, [1] points checked, [2] checks failed.
 
 // Orders the LDS accesses of one wavefront (its lanes exchange values through
