@@ -289,14 +289,17 @@ __global__ void __launch_bounds__(kTpb) update_kernel(const BlockView *__restric
   const bool rk3 = S.rk == 3;
   const long long p = interior_pidx(B, q);
   double cur[NV], dqv[NV], qnv[NV];
+  // dq and the stage base qn are streamed once per stage: non-temporal
+  // accesses keep them from displacing the state in the caches
 #pragma unroll
   for (int l = 0; l < NV; ++l) cur[l] = B.cv[l * np + p];
 #pragma unroll
-  for (int l = 0; l < NV; ++l) dqv[l] = MIXED ? (double)B.dq32[l * ni + q] : B.dq[l * ni + q];
+  for (int l = 0; l < NV; ++l)
+    dqv[l] = MIXED ? (double)__builtin_nontemporal_load(&B.dq32[l * ni + q]) : __builtin_nontemporal_load(&B.dq[l * ni + q]);
   const bool use_qn = rk3 && stage > 0;
   if (use_qn) {
 #pragma unroll
-    for (int l = 0; l < NV; ++l) qnv[l] = B.qn[l * ni + q];
+    for (int l = 0; l < NV; ++l) qnv[l] = __builtin_nontemporal_load(&B.qn[l * ni + q]);
   }
   const double t_old = B.bv[5 * np + p];
   const double dtj = (rk3 ? *dt_dev : B.dt_local[q]) / B.jac[p];
@@ -350,7 +353,7 @@ __global__ void __launch_bounds__(kTpb) update_kernel(const BlockView *__restric
   }
   if (rk3 && stage == 0) {
 #pragma unroll
-    for (int l = 0; l < NV; ++l) B.qn[l * ni + q] = cur[l];
+    for (int l = 0; l < NV; ++l) __builtin_nontemporal_store(cur[l], &B.qn[l * ni + q]);
   }
 #pragma unroll
   for (int l = 0; l < NV; ++l) B.cv[l * np + p] = c[l];
