@@ -5,6 +5,8 @@
 // Each build registers its launchers in g_kernels64[n] / g_kernels32[n].
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "device_physics.h"
 #include "kernels.h"
 #include "mlp.h"
@@ -964,17 +966,22 @@ int launch_props_mlp(const Batch &b, const SpeciesDev *sp, const SchemeDev &S, c
     return CFX_E_UNSUPPORTED;
   } else {
     if (b.total <= 0) return 0;
-    // persistent grid: the resident workgroups of the whole device
-    static int resident = 0;
-    if (!resident) {
-      int dev = 0, n_cu = 0, per_cu = 0;
-      (void)hipGetDevice(&dev);
+    // persistent grid: the resident workgroups of the current device (cached
+    // per device; concurrent first calls compute the same value)
+    constexpr int kMaxDev = 64;
+    static std::atomic<int> resident[kMaxDev] = {};
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    int r = dev >= 0 && dev < kMaxDev ? resident[dev].load(std::memory_order_relaxed) : 0;
+    if (!r) {
+      int n_cu = 0, per_cu = 0;
       if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) n_cu = 256;
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, props_mlp_kernel, kTpb, 0) != hipSuccess)
         per_cu = 1;
-      resident = std::max(1, n_cu * std::max(1, per_cu));
+      r = std::max(1, n_cu * std::max(1, per_cu));
+      if (dev >= 0 && dev < kMaxDev) resident[dev].store(r, std::memory_order_relaxed);
     }
-    const int grid = std::min(b.total, resident);
+    const int grid = std::min(b.total, r);
     props_mlp_kernel<<<grid, kTpb, 0, st>>>(b.views, b.off, b.nb, b.total, sp, S, mlp, (const bf16x8 *)ops, bias,
                                            counters);
     return (int)hipGetLastError();
