@@ -8,11 +8,10 @@
 //
 // Inviscid phase (InviscidScheme.cu:220-304 via Reconstruction.cu:45-122):
 // the 5+NS reconstructed variables are staged over the box (TA+3) x TB x TC
-// (the 4-point MUSCL stencil along a), KI variables per batch.  FP64 tiles
-// load a batch into registers and write it to LDS between two barriers;
-// FP32 tiles load batches straight into two alternating LDS buffers with
-// global-to-LDS loads, the next batch in flight while the current one is
-// used (one barrier per batch).
+// (the 4-point MUSCL stencil along a), KI variables per batch.  Batches go
+// straight into two alternating LDS buffers with global-to-LDS loads, the
+// next batch in flight while the current one is used (one barrier per
+// batch).
 // Viscous phase (ViscousScheme.cu:11-342): u, v, w, T and the mass fractions
 // are staged over (TA+1) x (TB+2) x (TC+2) (the face points plus the
 // tangential neighbours of the 4-point averages), KV per batch.  Face-point
@@ -77,22 +76,26 @@ struct PencilShape {
   static constexpr int tx[3] = {1, T0, T0 * T1};
   using BI = Box<DIR, TA + 3, TB, TC>;
   using BV = Box<DIR, TA + 1, TB + 2, TC + 2>;
-  // variables staged per batch: 7 inviscid, 4 viscous (FP64) or 3 (FP32)
   static constexpr bool F32 = sizeof(R) == 4;
-  // FP32 tiles stage through two LDS buffers filled by global-to-LDS loads
+  // The tiles stage through two LDS buffers filled by global-to-LDS loads
   // (no registers in between): the next batch is in flight while the
-  // current one is used.  Each variable's slice is padded to whole
-  // wavefronts, since a wavefront's load writes 64 consecutive slots.
-  static constexpr bool DMA = F32;
-  static constexpr int VPI = DMA ? (BI::V + 63) / 64 * 64 : BI::V;  // LDS stride of one inviscid variable
-  static constexpr int VPV = DMA ? (BV::V + 63) / 64 * 64 : BV::V;  // ... of one viscous variable
-  // DMA batch sizes (measured: 7 and 3 keep every FP32 tile within the LDS
-  // share of its occupancy target; larger batches that fill the share
-  // exactly cost a workgroup per CU on the WPE = 3 tiles)
-  static constexpr int KI = NV < 7 ? NV : 7;  // inviscid variables per batch
-  static constexpr int KV = DMA ? 3 : 4;      // viscous variables per batch
+  // current one is used.  A wavefront's dword load writes 64 consecutive
+  // dwords, EPW elements (FP64: two lanes per element, one per word), so
+  // every variable's slice is padded to whole wavefront loads.
+  static constexpr int W = (int)sizeof(R) / 4;  // dwords per element
+  static constexpr int EPW = 64 / W;
+  static constexpr int VPI = (BI::V + EPW - 1) / EPW * EPW;  // LDS stride of one inviscid variable
+  static constexpr int VPV = (BV::V + EPW - 1) / EPW * EPW;  // ... of one viscous variable
+  // batch sizes: 7 inviscid variables; 3 viscous, 2 where the FP64 buffers
+  // would pass the LDS share of the tile's occupancy target (measured:
+  // batches that fill the share exactly cost a workgroup per CU)
+  static constexpr int KI = NV < 7 ? NV : 7;
+  static constexpr int lds_share = 163840 / (WPE * 256 / NT > 0 ? WPE * 256 / NT : 1) - 2048;
+  static constexpr int lds_bytes(int kv) {
+    return (2 * (KI * VPI > kv * VPV ? KI * VPI : kv * VPV) + NV * NT) * (int)sizeof(R);
+  }
+  static constexpr int KV = F32 || lds_bytes(3) <= lds_share ? 3 : 2;
   static constexpr int STG = KI * VPI > KV * VPV ? KI * VPI : KV * VPV;
-  static constexpr int NBUF = DMA ? 2 : 1;
 };
 
 // Tile order inside a block: the tiles of the (b, c) plane are visited in
@@ -180,393 +183,208 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
     const unsigned pl_b = (unsigned)pl_i * (unsigned)sizeof(R), pr_b = (unsigned)pr_i * (unsigned)sizeof(R);
     const int szi = (int)B.sz, syi = (int)B.sy;
 
-    // stage K variables (source pointers) over box BX with global origin bo.
-    // All loads of the batch are issued before the first LDS write (one
-    // memory round trip per batch); coordinates beyond the padded range are
-    // clamped into it, which only fills slots no valid face reads.
-    auto stage = [&](auto box, auto kc, const gptr<const R> *src, const int bo[3]) {
+    R k1 = 0, k2 = 0, k3 = 0;
+    // ---- batches through two LDS buffers.  Batch n lands in buffer n % 2;
+    // after the barrier that publishes batch n, batch n + 1 is issued into
+    // the other buffer (whose last readers passed that barrier) and stays
+    // in flight while batch n is used.
+    constexpr int VPI = P::VPI, VPV = P::VPV, KI = P::KI, KV = P::KV;
+    constexpr int NI = (NV + KI - 1) / KI;         // inviscid batches
+    constexpr int NVV = 4 + NS;                    // viscous variables: u, v, w, T, Y_l
+    constexpr int NVB = (NVV + KV - 1) / KV;       // viscous batches
+    const bool viscous = S.viscous == 2;
+    R *const buf0 = stg, *const buf1 = stg + P::STG;
+    // issue variables [v0, v0 + K) of box BX (origin bo, LDS stride VP)
+    // into buf: lane e of the box loads element e of each variable, the
+    // wavefront writing 64 consecutive slots from its first element on
+    // (FP64: lane pairs share an element, one loading each word)
+    auto issue = [&](auto box, auto kc, auto vpc, auto srcf, const int bo[3], R *buf) {
       using BX = decltype(box);
-      constexpr int K = decltype(kc)::value;
-      constexpr int IT = (BX::V + NT - 1) / NT;
-      R tmp[IT][K];
+      constexpr int K = decltype(kc)::value, VP = decltype(vpc)::value;
+      constexpr int W = P::W, EPW = P::EPW, NW = NT / 64;
+      constexpr int NC = (BX::V + EPW - 1) / EPW;  // wavefront loads per variable
+      constexpr int IT = (NC + NW - 1) / NW;
+      const int wave = tid >> 6, lane = tid & 63;
 #pragma unroll
       for (int it = 0; it < IT; ++it) {
-        const int e0 = tid + it * NT;
-        const int e = e0 < BX::V ? e0 : BX::V - 1;
-        const int x = min(bo[0] + e % BX::L0, m[0] + 1);
-        const int y = min(bo[1] + (e / BX::L0) % BX::L1, m[1] + 1);
-        const int z = min(bo[2] + e / (BX::L0 * BX::L1), m[2] + 1);
-        const unsigned gb = (unsigned)((z + kNgg) * szi + (y + kNgg) * syi + (x + kNgg)) * (unsigned)sizeof(R);
+        const int chunk = it * NW + wave;
+        if (chunk < NC) {  // uniform per wavefront
+          const int e0 = chunk * EPW + lane / W;
+          const int e = e0 < BX::V ? e0 : BX::V - 1;
+          const int x = min(bo[0] + e % BX::L0, m[0] + 1);
+          const int y = min(bo[1] + (e / BX::L0) % BX::L1, m[1] + 1);
+          const int z = min(bo[2] + e / (BX::L0 * BX::L1), m[2] + 1);
+          const unsigned gb = (unsigned)((z + kNgg) * szi + (y + kNgg) * syi + (x + kNgg)) * (unsigned)sizeof(R) +
+                              (unsigned)(lane % W) * 4u;
 #pragma unroll
-        for (int k = 0; k < K; ++k) tmp[it][k] = ldg<R>(src[k], gb);
-      }
-      __syncthreads();
-#pragma unroll
-      for (int it = 0; it < IT; ++it) {
-        const int e = tid + it * NT;
-        if (e < BX::V) {
-#pragma unroll
-          for (int k = 0; k < K; ++k) stg[k * BX::V + e] = tmp[it][k];
+          for (int k = 0; k < K; ++k)
+            __builtin_amdgcn_global_load_lds(
+                (const __attribute__((address_space(1))) void *)((gptr<const char>)srcf(k) + gb),
+                (__attribute__((address_space(3))) void *)(buf + k * VP + chunk * EPW), 4, 0, 0);
         }
       }
+    };
+    // the loads of this thread (and so of the batch, after the barrier)
+    // have written LDS: vmcnt(0) with lgkmcnt and expcnt left open
+    auto land = [&]() {
+      __builtin_amdgcn_s_waitcnt(0x0F70);
       __syncthreads();
     };
+    int boi[3], bov[3];
+    boi[A] = o[A] - 2, boi[Bx] = o[Bx], boi[Cx] = o[Cx];
+    bov[A] = o[A] - 1, bov[Bx] = o[Bx] - 1, bov[Cx] = o[Cx] - 1;
+    auto issue_inv = [&](auto nc, R *buf) {
+      constexpr int v0 = decltype(nc)::value * KI;
+      constexpr int K = NV - v0 < KI ? NV - v0 : KI;
+      issue(BI{}, std::integral_constant<int, K>{}, std::integral_constant<int, VPI>{},
+            [&](int k) { return v0 + k < 5 ? BV_ + (v0 + k) * np : YK_ + (v0 + k - 5) * np; }, boi, buf);
+    };
+    auto issue_vis = [&](auto nc, R *buf) {
+      constexpr int j0 = decltype(nc)::value * KV;
+      constexpr int K = NVV - j0 < KV ? NVV - j0 : KV;
+      issue(BV{}, std::integral_constant<int, K>{}, std::integral_constant<int, VPV>{},
+            [&](int k) { return j0 + k < 4 ? BV_ + (j0 + k < 3 ? j0 + k + 1 : 5) * np : YK_ + (j0 + k - 4) * np; },
+            bov, buf);
+    };
 
-    R k1 = 0, k2 = 0, k3 = 0;
-    if constexpr (P::DMA) {
-      // ---- batches through two LDS buffers.  Batch n lands in buffer n % 2;
-      // after the barrier that publishes batch n, batch n + 1 is issued into
-      // the other buffer (whose last readers passed that barrier) and stays
-      // in flight while batch n is used.
-      constexpr int VPI = P::VPI, VPV = P::VPV, KI = P::KI, KV = P::KV;
-      constexpr int NI = (NV + KI - 1) / KI;         // inviscid batches
-      constexpr int NVV = 4 + NS;                    // viscous variables: u, v, w, T, Y_l
-      constexpr int NVB = (NVV + KV - 1) / KV;       // viscous batches
-      const bool viscous = S.viscous == 2;
-      R *const buf0 = stg, *const buf1 = stg + P::STG;
-      // issue variables [v0, v0 + K) of box BX (origin bo, LDS stride VP)
-      // into buf: lane e of the box loads element e of each variable, the
-      // wavefront writing 64 consecutive slots from its first element on
-      auto issue = [&](auto box, auto kc, auto vpc, auto srcf, const int bo[3], R *buf) {
-        using BX = decltype(box);
-        constexpr int K = decltype(kc)::value, VP = decltype(vpc)::value;
-        constexpr int IT = (BX::V + NT - 1) / NT;
-        const int wbase = tid & ~63;
-#pragma unroll
-        for (int it = 0; it < IT; ++it) {
-          const int base = it * NT + wbase;
-          if (base < BX::V) {  // uniform per wavefront
-            const int e0 = tid + it * NT;
-            const int e = e0 < BX::V ? e0 : BX::V - 1;
-            const int x = min(bo[0] + e % BX::L0, m[0] + 1);
-            const int y = min(bo[1] + (e / BX::L0) % BX::L1, m[1] + 1);
-            const int z = min(bo[2] + e / (BX::L0 * BX::L1), m[2] + 1);
-            const unsigned gb = (unsigned)((z + kNgg) * szi + (y + kNgg) * syi + (x + kNgg)) * (unsigned)sizeof(R);
-#pragma unroll
-            for (int k = 0; k < K; ++k)
-              __builtin_amdgcn_global_load_lds(
-                  (const __attribute__((address_space(1))) void *)((gptr<const char>)srcf(k) + gb),
-                  (__attribute__((address_space(3))) void *)(buf + k * VP + base), 4, 0, 0);
-          }
-        }
-      };
-      // the loads of this thread (and so of the batch, after the barrier)
-      // have written LDS: vmcnt(0) with lgkmcnt and expcnt left open
-      auto land = [&]() {
-        __builtin_amdgcn_s_waitcnt(0x0F70);
-        __syncthreads();
-      };
-      int boi[3], bov[3];
-      boi[A] = o[A] - 2, boi[Bx] = o[Bx], boi[Cx] = o[Cx];
-      bov[A] = o[A] - 1, bov[Bx] = o[Bx] - 1, bov[Cx] = o[Cx] - 1;
-      auto issue_inv = [&](auto nc, R *buf) {
-        constexpr int v0 = decltype(nc)::value * KI;
+    // ---- inviscid phase: F_inv into this thread's face slot
+    {
+      int l[3];
+      l[A] = t[A] + 1;
+      l[Bx] = t[Bx];
+      l[Cx] = t[Cx];
+      const int li = l[0] + BI::L0 * (l[1] + BI::L1 * l[2]);
+      constexpr int sa = BI::sa;
+      R pl[NV], pr[NV];
+      issue_inv(std::integral_constant<int, 0>{}, buf0);
+      static_for<NI>([&](auto bi) {
+        constexpr int n = decltype(bi)::value;
+        constexpr int v0 = n * KI;
         constexpr int K = NV - v0 < KI ? NV - v0 : KI;
-        issue(BI{}, std::integral_constant<int, K>{}, std::integral_constant<int, VPI>{},
-              [&](int k) { return v0 + k < 5 ? BV_ + (v0 + k) * np : YK_ + (v0 + k - 5) * np; }, boi, buf);
+        R *const cur = n % 2 == 0 ? buf0 : buf1;
+        R *const nxt = n % 2 == 0 ? buf1 : buf0;
+        land();
+        if constexpr (n + 1 < NI) {
+          issue_inv(std::integral_constant<int, n + 1>{}, nxt);
+        } else {
+          if (viscous) issue_vis(std::integral_constant<int, 0>{}, nxt);
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const R *q = cur + k * VPI + li;
+          muscl<R>(S, q[-sa], q[0], q[sa], q[2 * sa], pl[v0 + k], pr[v0 + k]);
+        }
+      });
+      R F[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) F[v] = R(0);
+      if (face_ok) {
+        const gptr<const R> M = MT_ + 3 * A * np;
+        const R jl = ldg<R>(JC_, pl_b), jr = ldg<R>(JC_, pr_b);
+        k1 = R(0.5) * (jl * ldg<R>(M, pl_b) + jr * ldg<R>(M, pr_b));
+        k2 = R(0.5) * (jl * ldg<R>(M + np, pl_b) + jr * ldg<R>(M + np, pr_b));
+        k3 = R(0.5) * (jl * ldg<R>(M + 2 * np, pl_b) + jr * ldg<R>(M + 2 * np, pr_b));
+        ausm_from_states<NS, R>(sp, ThermoConst<R>{sp}, S, pl, pr, k1, k2, k3, F);
+      }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) fbuf[v * NT + tid] = F[v];
+    }
+
+    // ---- viscous phase: F_visc subtracted from the slot as it is formed
+    if (viscous) {
+      int l[3];
+      l[A] = t[A];
+      l[Bx] = t[Bx] + 1;
+      l[Cx] = t[Cx] + 1;
+      const int li = l[0] + BV::L0 * (l[1] + BV::L1 * l[2]);
+      constexpr int sa = BV::sa, sb = BV::sb, sc = BV::sc;
+      // face-point data, loaded before the staged batches are used
+      R mf[9], vis = 0, cond = 0, dif[NS > 0 ? NS : 1];
+      const unsigned pD_b = (NS > 0 && DIR == 2 && (S.quirks & CFX_Q_HV_RHOD_J))
+                                ? (unsigned)(pl_i + B.sy) * (unsigned)sizeof(R) : pr_b;
+      if (face_ok) {
+        const bool q5 = (S.quirks & CFX_Q_METRIC_AVG) && DIR != 1;
+#pragma unroll
+        for (int rc = 0; rc < 9; ++rc) {
+          const R a = ldg<R>(MT_ + rc * np, pl_b);
+          mf[rc] = (q5 && rc != 0) ? R(0.5) * (a + a) : R(0.5) * (a + ldg<R>(MT_ + rc * np, pr_b));
+        }
+        vis = R(0.5) * (ldg<R>(PR_ + 2 * np, pl_b) + ldg<R>(PR_ + 2 * np, pr_b));
+        cond = R(0.5) * (ldg<R>(PR_ + 3 * np, pl_b) + ldg<R>(PR_ + 3 * np, pr_b));
+#pragma unroll
+        for (int s2 = 0; s2 < NS; ++s2)
+          dif[s2] = R(0.5) * (ldg<R>(PR_ + (4 + s2) * np, pl_b) + ldg<R>(PR_ + (4 + s2) * np, pD_b));
+      } else {
+#pragma unroll
+        for (int rc = 0; rc < 9; ++rc) mf[rc] = R(0);
+#pragma unroll
+        for (int s2 = 0; s2 < NS; ++s2) dif[s2] = R(0);
+      }
+      auto chain = [&](const R gc[3], R out[3]) {
+        out[0] = gc[0] * mf[0] + gc[1] * mf[3] + gc[2] * mf[6];
+        out[1] = gc[0] * mf[1] + gc[1] * mf[4] + gc[2] * mf[7];
+        out[2] = gc[0] * mf[2] + gc[1] * mf[5] + gc[2] * mf[8];
       };
-      auto issue_vis = [&](auto nc, R *buf) {
-        constexpr int j0 = decltype(nc)::value * KV;
+      R dvel[4][3], avg[4];
+      R dp[NS > 0 ? NS : 1], yb[NS > 0 ? NS : 1];
+      R ucx = 0, ucy = 0, ucz = 0, e4 = 0;
+      static_for<NVB>([&](auto bi) {
+        constexpr int nb2 = decltype(bi)::value;
+        constexpr int n = NI + nb2;
+        constexpr int j0 = nb2 * KV;
         constexpr int K = NVV - j0 < KV ? NVV - j0 : KV;
-        issue(BV{}, std::integral_constant<int, K>{}, std::integral_constant<int, VPV>{},
-              [&](int k) { return j0 + k < 4 ? BV_ + (j0 + k < 3 ? j0 + k + 1 : 5) * np : YK_ + (j0 + k - 4) * np; },
-              bov, buf);
-      };
-
-      // ---- inviscid phase: F_inv into this thread's face slot
-      {
-        int l[3];
-        l[A] = t[A] + 1;
-        l[Bx] = t[Bx];
-        l[Cx] = t[Cx];
-        const int li = l[0] + BI::L0 * (l[1] + BI::L1 * l[2]);
-        constexpr int sa = BI::sa;
-        R pl[NV], pr[NV];
-        issue_inv(std::integral_constant<int, 0>{}, buf0);
-        static_for<NI>([&](auto bi) {
-          constexpr int n = decltype(bi)::value;
-          constexpr int v0 = n * KI;
-          constexpr int K = NV - v0 < KI ? NV - v0 : KI;
-          R *const cur = n % 2 == 0 ? buf0 : buf1;
-          R *const nxt = n % 2 == 0 ? buf1 : buf0;
-          land();
-          if constexpr (n + 1 < NI) {
-            issue_inv(std::integral_constant<int, n + 1>{}, nxt);
-          } else {
-            if (viscous) issue_vis(std::integral_constant<int, 0>{}, nxt);
-          }
-#pragma unroll
-          for (int k = 0; k < K; ++k) {
-            const R *q = cur + k * VPI + li;
-            muscl<R>(S, q[-sa], q[0], q[sa], q[2 * sa], pl[v0 + k], pr[v0 + k]);
-          }
-        });
-        R F[NV];
-#pragma unroll
-        for (int v = 0; v < NV; ++v) F[v] = R(0);
-        if (face_ok) {
-          const gptr<const R> M = MT_ + 3 * A * np;
-          const R jl = ldg<R>(JC_, pl_b), jr = ldg<R>(JC_, pr_b);
-          k1 = R(0.5) * (jl * ldg<R>(M, pl_b) + jr * ldg<R>(M, pr_b));
-          k2 = R(0.5) * (jl * ldg<R>(M + np, pl_b) + jr * ldg<R>(M + np, pr_b));
-          k3 = R(0.5) * (jl * ldg<R>(M + 2 * np, pl_b) + jr * ldg<R>(M + 2 * np, pr_b));
-          ausm_from_states<NS, R>(sp, ThermoConst<R>{sp}, S, pl, pr, k1, k2, k3, F);
-        }
-#pragma unroll
-        for (int v = 0; v < NV; ++v) fbuf[v * NT + tid] = F[v];
-      }
-
-      // ---- viscous phase: F_visc subtracted from the slot as it is formed
-      if (viscous) {
-        int l[3];
-        l[A] = t[A];
-        l[Bx] = t[Bx] + 1;
-        l[Cx] = t[Cx] + 1;
-        const int li = l[0] + BV::L0 * (l[1] + BV::L1 * l[2]);
-        constexpr int sa = BV::sa, sb = BV::sb, sc = BV::sc;
-        // face-point data, loaded before the staged batches are used
-        R mf[9], vis = 0, cond = 0, dif[NS > 0 ? NS : 1];
-        const unsigned pD_b = (NS > 0 && DIR == 2 && (S.quirks & CFX_Q_HV_RHOD_J))
-                                  ? (unsigned)(pl_i + B.sy) * (unsigned)sizeof(R) : pr_b;
-        if (face_ok) {
-          const bool q5 = (S.quirks & CFX_Q_METRIC_AVG) && DIR != 1;
-#pragma unroll
-          for (int rc = 0; rc < 9; ++rc) {
-            const R a = ldg<R>(MT_ + rc * np, pl_b);
-            mf[rc] = (q5 && rc != 0) ? R(0.5) * (a + a) : R(0.5) * (a + ldg<R>(MT_ + rc * np, pr_b));
-          }
-          vis = R(0.5) * (ldg<R>(PR_ + 2 * np, pl_b) + ldg<R>(PR_ + 2 * np, pr_b));
-          cond = R(0.5) * (ldg<R>(PR_ + 3 * np, pl_b) + ldg<R>(PR_ + 3 * np, pr_b));
-#pragma unroll
-          for (int s2 = 0; s2 < NS; ++s2)
-            dif[s2] = R(0.5) * (ldg<R>(PR_ + (4 + s2) * np, pl_b) + ldg<R>(PR_ + (4 + s2) * np, pD_b));
-        } else {
-#pragma unroll
-          for (int rc = 0; rc < 9; ++rc) mf[rc] = R(0);
-#pragma unroll
-          for (int s2 = 0; s2 < NS; ++s2) dif[s2] = R(0);
-        }
-        auto chain = [&](const R gc[3], R out[3]) {
-          out[0] = gc[0] * mf[0] + gc[1] * mf[3] + gc[2] * mf[6];
-          out[1] = gc[0] * mf[1] + gc[1] * mf[4] + gc[2] * mf[7];
-          out[2] = gc[0] * mf[2] + gc[1] * mf[5] + gc[2] * mf[8];
-        };
-        R dvel[4][3], avg[4];
-        R dp[NS > 0 ? NS : 1], yb[NS > 0 ? NS : 1];
-        R ucx = 0, ucy = 0, ucz = 0, e4 = 0;
-        static_for<NVB>([&](auto bi) {
-          constexpr int nb2 = decltype(bi)::value;
-          constexpr int n = NI + nb2;
-          constexpr int j0 = nb2 * KV;
-          constexpr int K = NVV - j0 < KV ? NVV - j0 : KV;
-          R *const cur = n % 2 == 0 ? buf0 : buf1;
-          R *const nxt = n % 2 == 0 ? buf1 : buf0;
-          land();
-          if constexpr (nb2 + 1 < NVB) issue_vis(std::integral_constant<int, nb2 + 1>{}, nxt);
-          static_for<K>([&](auto kk) {
-            constexpr int k = decltype(kk)::value;
-            constexpr int j = j0 + k;
-            const R *q = cur + k * VPV + li;
-            R gc[3], d[3];
-            gc[A] = q[sa] - q[0];
-            gc[Bx] = R(0.25) * (q[sb] - q[-sb] + q[sa + sb] - q[sa - sb]);
-            gc[Cx] = R(0.25) * (q[sc] - q[-sc] + q[sa + sc] - q[sa - sc]);
-            chain(gc, d);
-            if constexpr (j < 4) {
-              dvel[j][0] = d[0], dvel[j][1] = d[1], dvel[j][2] = d[2];
-              avg[j] = R(0.5) * (q[0] + q[sa]);
-              if constexpr (j == 3) {
-                const R *du = dvel[0], *dv = dvel[1], *dw = dvel[2], *dT = dvel[3];
-                const R third = R(1.0 / 3.0);
-                const R txx = vis * (R(4) * du[0] - R(2) * dv[1] - R(2) * dw[2]) * third;
-                const R tyy = vis * (R(4) * dv[1] - R(2) * du[0] - R(2) * dw[2]) * third;
-                const R tzz = vis * (R(4) * dw[2] - R(2) * du[0] - R(2) * dv[1]) * third;
-                const R txy = vis * (du[1] + dv[0]);
-                const R txz = vis * (du[2] + dw[0]);
-                const R tyz = vis * (dv[2] + dw[1]);
-                const R G1 = k1 * txx + k2 * txy + k3 * txz;
-                const R G2 = k1 * txy + k2 * tyy + k3 * tyz;
-                const R G3 = k1 * txz + k2 * tyz + k3 * tzz;
-                const R G4 = avg[0] * G1 + avg[1] * G2 + avg[2] * G3 + cond * (k1 * dT[0] + k2 * dT[1] + k3 * dT[2]);
-                fbuf[1 * NT + tid] -= G1;
-                fbuf[2 * NT + tid] -= G2;
-                fbuf[3 * NT + tid] -= G3;
-                fbuf[4 * NT + tid] -= G4;
-              }
-            } else {
-              constexpr int s2 = j - 4;
-              yb[s2] = R(0.5) * (q[0] + q[sa]);
-              const R h = species_h<R>(sp, s2, avg[3]);
-              const R proj = k1 * d[0] + k2 * d[1] + k3 * d[2];
-              e4 += dif[s2] * h * proj;
-              ucx += dif[s2] * d[0];
-              ucy += dif[s2] * d[1];
-              ucz += dif[s2] * d[2];
-              dp[s2] = dif[s2] * proj;
-            }
-          });
-        });
-        if constexpr (NS > 0) {
-          const R uc = k1 * ucx + k2 * ucy + k3 * ucz;
-          fbuf[4 * NT + tid] -= e4;
-#pragma unroll
-          for (int s2 = 0; s2 < NS; ++s2) fbuf[(5 + s2) * NT + tid] -= dp[s2] - yb[s2] * uc;
-        }
-      }
-    } else {
-
-      // ---- inviscid phase: F_inv into this thread's face slot
-      {
-        int bo[3];
-        bo[A] = o[A] - 2;
-        bo[Bx] = o[Bx];
-        bo[Cx] = o[Cx];
-        int l[3];
-        l[A] = t[A] + 1;
-        l[Bx] = t[Bx];
-        l[Cx] = t[Cx];
-        const int li = l[0] + BI::L0 * (l[1] + BI::L1 * l[2]);
-        constexpr int sa = BI::sa;
-        R pl[NV], pr[NV];
-        static_for<(NV + P::KI - 1) / P::KI>([&](auto bi) {
-          constexpr int v0 = decltype(bi)::value * P::KI;
-          constexpr int K = NV - v0 < P::KI ? NV - v0 : P::KI;
-          gptr<const R> src[K];
-  #pragma unroll
-          for (int k = 0; k < K; ++k) src[k] = v0 + k < 5 ? BV_ + (v0 + k) * np : YK_ + (v0 + k - 5) * np;
-          stage(BI{}, std::integral_constant<int, K>{}, src, bo);
-  #pragma unroll
-          for (int k = 0; k < K; ++k) {
-            const R *q = stg + k * BI::V + li;
-            muscl<R>(S, q[-sa], q[0], q[sa], q[2 * sa], pl[v0 + k], pr[v0 + k]);
-          }
-        });
-        R F[NV];
-  #pragma unroll
-        for (int v = 0; v < NV; ++v) F[v] = R(0);
-        if (face_ok) {
-          const gptr<const R> M = MT_ + 3 * A * np;
-          const R jl = ldg<R>(JC_, pl_b), jr = ldg<R>(JC_, pr_b);
-          k1 = R(0.5) * (jl * ldg<R>(M, pl_b) + jr * ldg<R>(M, pr_b));
-          k2 = R(0.5) * (jl * ldg<R>(M + np, pl_b) + jr * ldg<R>(M + np, pr_b));
-          k3 = R(0.5) * (jl * ldg<R>(M + 2 * np, pl_b) + jr * ldg<R>(M + 2 * np, pr_b));
-          ausm_from_states<NS, R>(sp, ThermoConst<R>{sp}, S, pl, pr, k1, k2, k3, F);
-        }
-  #pragma unroll
-        for (int v = 0; v < NV; ++v) fbuf[v * NT + tid] = F[v];
-      }
-
-      // ---- viscous phase: F_visc subtracted from the slot as it is formed
-      if (S.viscous == 2) {
-        int bo[3];
-        bo[A] = o[A] - 1;
-        bo[Bx] = o[Bx] - 1;
-        bo[Cx] = o[Cx] - 1;
-        int l[3];
-        l[A] = t[A];
-        l[Bx] = t[Bx] + 1;
-        l[Cx] = t[Cx] + 1;
-        const int li = l[0] + BV::L0 * (l[1] + BV::L1 * l[2]);
-        constexpr int sa = BV::sa, sb = BV::sb, sc = BV::sc;
-        R mf[9];
-        if (face_ok) {
-          const bool q5 = (S.quirks & CFX_Q_METRIC_AVG) && DIR != 1;
-  #pragma unroll
-          for (int rc = 0; rc < 9; ++rc) {
-            const R a = ldg<R>(MT_ + rc * np, pl_b);
-            mf[rc] = (q5 && rc != 0) ? R(0.5) * (a + a) : R(0.5) * (a + ldg<R>(MT_ + rc * np, pr_b));
-          }
-        } else {
-  #pragma unroll
-          for (int rc = 0; rc < 9; ++rc) mf[rc] = R(0);
-        }
-        // computational-space derivatives of staged variable k at the face
-        auto grad = [&](int k, R gc[3]) {
-          const R *q = stg + k * BV::V + li;
+        R *const cur = n % 2 == 0 ? buf0 : buf1;
+        R *const nxt = n % 2 == 0 ? buf1 : buf0;
+        land();
+        if constexpr (nb2 + 1 < NVB) issue_vis(std::integral_constant<int, nb2 + 1>{}, nxt);
+        static_for<K>([&](auto kk) {
+          constexpr int k = decltype(kk)::value;
+          constexpr int j = j0 + k;
+          const R *q = cur + k * VPV + li;
+          R gc[3], d[3];
           gc[A] = q[sa] - q[0];
           gc[Bx] = R(0.25) * (q[sb] - q[-sb] + q[sa + sb] - q[sa - sb]);
           gc[Cx] = R(0.25) * (q[sc] - q[-sc] + q[sa + sc] - q[sa - sc]);
-        };
-        auto chain = [&](const R gc[3], R out[3]) {
-          out[0] = gc[0] * mf[0] + gc[1] * mf[3] + gc[2] * mf[6];
-          out[1] = gc[0] * mf[1] + gc[1] * mf[4] + gc[2] * mf[7];
-          out[2] = gc[0] * mf[2] + gc[1] * mf[5] + gc[2] * mf[8];
-        };
-        R gcv[3], du[3], dv[3], dw[3], dT[3], avg[4];
-        {
-          const gptr<const R> src[4] = {BV_ + np, BV_ + 2 * np, BV_ + 3 * np, BV_ + 5 * np};
-          stage(BV{}, std::integral_constant<int, 4>{}, src, bo);
-          grad(0, gcv);
-          chain(gcv, du);
-          grad(1, gcv);
-          chain(gcv, dv);
-          grad(2, gcv);
-          chain(gcv, dw);
-          grad(3, gcv);
-          chain(gcv, dT);
-  #pragma unroll
-          for (int q = 0; q < 4; ++q) avg[q] = R(0.5) * (stg[q * BV::V + li] + stg[q * BV::V + li + sa]);
-        }
-        {
-          const R vis = face_ok ? R(0.5) * (ldg<R>(PR_ + 2 * np, pl_b) + ldg<R>(PR_ + 2 * np, pr_b)) : R(0);
-          const R cond = face_ok ? R(0.5) * (ldg<R>(PR_ + 3 * np, pl_b) + ldg<R>(PR_ + 3 * np, pr_b)) : R(0);
-          const R third = R(1.0 / 3.0);
-          const R txx = vis * (R(4) * du[0] - R(2) * dv[1] - R(2) * dw[2]) * third;
-          const R tyy = vis * (R(4) * dv[1] - R(2) * du[0] - R(2) * dw[2]) * third;
-          const R tzz = vis * (R(4) * dw[2] - R(2) * du[0] - R(2) * dv[1]) * third;
-          const R txy = vis * (du[1] + dv[0]);
-          const R txz = vis * (du[2] + dw[0]);
-          const R tyz = vis * (dv[2] + dw[1]);
-          const R G1 = k1 * txx + k2 * txy + k3 * txz;
-          const R G2 = k1 * txy + k2 * tyy + k3 * tyz;
-          const R G3 = k1 * txz + k2 * tyz + k3 * tzz;
-          const R G4 = avg[0] * G1 + avg[1] * G2 + avg[2] * G3 + cond * (k1 * dT[0] + k2 * dT[1] + k3 * dT[2]);
-          fbuf[1 * NT + tid] -= G1;
-          fbuf[2 * NT + tid] -= G2;
-          fbuf[3 * NT + tid] -= G3;
-          fbuf[4 * NT + tid] -= G4;
-        }
-        if constexpr (NS > 0) {
-          const R tm = avg[3];
-          const unsigned pD_b = (DIR == 2 && (S.quirks & CFX_Q_HV_RHOD_J)) ? (unsigned)(pl_i + B.sy) * (unsigned)sizeof(R)
-                                                                            : pr_b;
-          R dp[NS], yb[NS];
-          R ucx = 0, ucy = 0, ucz = 0, e4 = 0;
-          static_for<(NS + P::KV - 1) / P::KV>([&](auto bi) {
-            constexpr int s0 = decltype(bi)::value * P::KV;
-            constexpr int K = NS - s0 < P::KV ? NS - s0 : P::KV;
-            gptr<const R> src[K];
-  #pragma unroll
-            for (int k = 0; k < K; ++k) src[k] = YK_ + (s0 + k) * np;
-            stage(BV{}, std::integral_constant<int, K>{}, src, bo);
-  #pragma unroll
-            for (int k = 0; k < K; ++k) {
-              const int s = s0 + k;
-              grad(k, gcv);
-              R dy[3];
-              chain(gcv, dy);
-              yb[s] = R(0.5) * (stg[k * BV::V + li] + stg[k * BV::V + li + sa]);
-              const R dif = face_ok ? R(0.5) * (ldg<R>(PR_ + (4 + s) * np, pl_b) + ldg<R>(PR_ + (4 + s) * np, pD_b)) : R(0);
-              R h;
-              if constexpr (P::F32) {
-                h = species_h<R>(sp, s, tm);
-              } else {  // FP64: the h-only form costs registers (spills) there
-                R cp;
-                species_h_cp<R>(sp, s, tm, h, cp);
-              }
-              const R proj = k1 * dy[0] + k2 * dy[1] + k3 * dy[2];
-              e4 += dif * h * proj;
-              ucx += dif * dy[0];
-              ucy += dif * dy[1];
-              ucz += dif * dy[2];
-              dp[s] = dif * proj;
+          chain(gc, d);
+          if constexpr (j < 4) {
+            dvel[j][0] = d[0], dvel[j][1] = d[1], dvel[j][2] = d[2];
+            avg[j] = R(0.5) * (q[0] + q[sa]);
+            if constexpr (j == 3) {
+              const R *du = dvel[0], *dv = dvel[1], *dw = dvel[2], *dT = dvel[3];
+              const R third = R(1.0 / 3.0);
+              const R txx = vis * (R(4) * du[0] - R(2) * dv[1] - R(2) * dw[2]) * third;
+              const R tyy = vis * (R(4) * dv[1] - R(2) * du[0] - R(2) * dw[2]) * third;
+              const R tzz = vis * (R(4) * dw[2] - R(2) * du[0] - R(2) * dv[1]) * third;
+              const R txy = vis * (du[1] + dv[0]);
+              const R txz = vis * (du[2] + dw[0]);
+              const R tyz = vis * (dv[2] + dw[1]);
+              const R G1 = k1 * txx + k2 * txy + k3 * txz;
+              const R G2 = k1 * txy + k2 * tyy + k3 * tyz;
+              const R G3 = k1 * txz + k2 * tyz + k3 * tzz;
+              const R G4 = avg[0] * G1 + avg[1] * G2 + avg[2] * G3 + cond * (k1 * dT[0] + k2 * dT[1] + k3 * dT[2]);
+              fbuf[1 * NT + tid] -= G1;
+              fbuf[2 * NT + tid] -= G2;
+              fbuf[3 * NT + tid] -= G3;
+              fbuf[4 * NT + tid] -= G4;
             }
-          });
-          const R uc = k1 * ucx + k2 * ucy + k3 * ucz;
-          fbuf[4 * NT + tid] -= e4;
-  #pragma unroll
-          for (int s = 0; s < NS; ++s) fbuf[(5 + s) * NT + tid] -= dp[s] - yb[s] * uc;
-        }
+          } else {
+            constexpr int s2 = j - 4;
+            yb[s2] = R(0.5) * (q[0] + q[sa]);
+            const R h = species_h<R>(sp, s2, avg[3]);
+            const R proj = k1 * d[0] + k2 * d[1] + k3 * d[2];
+            e4 += dif[s2] * h * proj;
+            ucx += dif[s2] * d[0];
+            ucy += dif[s2] * d[1];
+            ucz += dif[s2] * d[2];
+            dp[s2] = dif[s2] * proj;
+          }
+        });
+      });
+      if constexpr (NS > 0) {
+        const R uc = k1 * ucx + k2 * ucy + k3 * ucz;
+        fbuf[4 * NT + tid] -= e4;
+#pragma unroll
+        for (int s2 = 0; s2 < NS; ++s2) fbuf[(5 + s2) * NT + tid] -= dp[s2] - yb[s2] * uc;
       }
     }
 
@@ -596,7 +414,7 @@ __global__ void __launch_bounds__(TA *TB *TC) __attribute__((amdgpu_waves_per_eu
     rhs_pencil_kernel(const BlockView *__restrict__ views, const int *__restrict__ off, int nb, int padded,
                       const SpeciesDev *__restrict__ spg, SchemeDev S, int accumulate) {
   using P = PencilShape<NS, DIR, TA, TB, TC, R, WPE>;
-  __shared__ R stg[P::NBUF * P::STG];  // one or two batches of staged variables
+  __shared__ R stg[2 * P::STG];        // two batches of staged variables
   __shared__ R fbuf[P::NV * P::NT];    // each thread's face flux F_inv - F_visc
   const SpeciesPtrT<R> sp = const_table<R>(spg);
   if constexpr (PERSIST) {
