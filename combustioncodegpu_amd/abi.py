@@ -28,7 +28,7 @@ QUIRKS_REFERENCE = Q_METRIC_AVG | Q_HV_RHOD_J
 FP64, MIXED = 0, 1
 TRANSPORT_WILKE, TRANSPORT_MLP = 0, 1
 
-CV, BV, YK, PROPS, DQ, METRIC, DTLOCAL = range(7)
+CV, BV, YK, PROPS, DQ, METRIC, DTLOCAL, COORD = range(8)
 
 _dp = C.POINTER(C.c_double)
 

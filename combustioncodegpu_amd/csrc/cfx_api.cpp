@@ -619,6 +619,40 @@ extern "C" {
 
 int cfx_version(void) { return CFX_ABI_VERSION; }
 
+int cfx_ghost_grid(const cfx_block *blocks, int n_block, double *const *xyz) {
+  if (!blocks || n_block <= 0 || !xyz) return CFX_E_ARG;
+  std::vector<HostBlock> hb;
+  int dim = 0;
+  std::string err;
+  if (int rc = build_mesh(blocks, n_block, hb, dim, err)) return rc;
+  for (int b = 0; b < n_block; ++b) {
+    const HostBlock &B = hb[b];
+    if (!xyz[b]) return CFX_E_ARG;
+    for (int c = 0; c < 3; ++c)
+      for (int k = -kNgg; k < B.mz + kNgg; ++k)
+        for (int j = -kNgg; j < B.my + kNgg; ++j)
+          for (int i = -kNgg; i < B.mx + kNgg; ++i)
+            xyz[b][c * B.np + B.pidx(i, j, k)] = B.xyz[c][B.gidx(i, j, k)];
+  }
+  return 0;
+}
+
+int cfx_inflow_state(const cfx_params *params, const cfx_species *species, int index, double bv[6], double *yk) {
+  if (!params || !bv || index < 0 || index >= params->n_inflow || !params->inflow) return CFX_E_ARG;
+  SpeciesDev tables;
+  std::string err;
+  if (int rc = build_species(species, tables, err)) return rc;
+  SchemeDev S{};
+  S.gamma_air = params->gamma_air > 0 ? params->gamma_air : 1.4;
+  S.r_air = params->r_air > 0 ? params->r_air : kRuH / kMwAir;
+  InflowDev d;
+  if (int rc = build_inflow(params->inflow[index], tables.d, S, d, err)) return rc;
+  for (int v = 0; v < 6; ++v) bv[v] = d.st[v];
+  if (yk)
+    for (int l = 0; l < tables.d.ns; ++l) yk[l] = d.y[l];
+  return 0;
+}
+
 int cfx_create(const cfx_params *params, const cfx_species *species, const cfx_block *blocks, int n_block,
                cfx_ctx **out) {
   if (!params || !blocks || n_block <= 0 || !out) return CFX_E_ARG;
@@ -876,6 +910,14 @@ int cfx_download(cfx_ctx *ctx, int blk, int what, double *dst) {
       CFX_TRY((int)hipMemcpy(dst, V.metric, sizeof(double) * 9 * np, hipMemcpyDeviceToHost));
       CFX_TRY((int)hipMemcpy(dst + 9 * np, V.jac, sizeof(double) * np, hipMemcpyDeviceToHost));
       return 0;
+    case CFX_COORD: {
+      const HostBlock &B = ctx->hb[blk];
+      for (int c = 0; c < 3; ++c)
+        for (int k = -kNgg; k < B.mz + kNgg; ++k)
+          for (int j = -kNgg; j < B.my + kNgg; ++j)
+            for (int i = -kNgg; i < B.mx + kNgg; ++i) dst[c * np + B.pidx(i, j, k)] = B.xyz[c][B.gidx(i, j, k)];
+      return 0;
+    }
     default: return ctx->fail(CFX_E_ARG, "unknown download selector");
   }
   if (n) CFX_TRY((int)hipMemcpy(dst, src, sizeof(double) * n, hipMemcpyDeviceToHost));

@@ -1,0 +1,786 @@
+"""Reader (and writer) of the reference's input decks.
+
+A reference case directory holds ``input_files/``:
+
+  setup/0_global_control.txt ... 9_transport_property.txt
+      typed keys, one per line: ``int cfl_steps = 3``  (Parameter.cpp:41-71)
+  setup/5_boundary_condition.txt   ``wall name label`` lines (BoundCond.cu:316-392)
+  setup/6_inflow.txt, 7_wall.txt   conditions per label (BoundCond.cu:127-205,
+                                   BoundCond.cpp:10-141)
+  setup/8_initialization.txt       inflow groups and their boxes (Initialize.cu:24-104)
+  grid/grid   0.grd                blocks and vertex coordinates (Mesh.cpp:566-604;
+                                   ``grid{:>4}`` pads the rank with spaces)
+  boundary_condition/boundary   0.txt, inner   0.txt   (Mesh.cpp:606-670)
+  <mechanism_file>, <therm_file>, <transport_file>      CHEMKIN-style
+                                   (ChemData.cpp:10-61,106-261)
+
+``load_case(root)`` turns such a directory into a :class:`cases.Case` for the
+step library: the BC labels become condition indices, the species table is
+built from the element composition and the NASA-7 / Lennard-Jones records,
+and the initial field is filled group by group over the ghost-padded blocks.
+The ghost-grid coordinates the group boxes are tested against and the
+derived inflow states come from the step library's host-side setup
+(``cfx_ghost_grid``, ``cfx_inflow_state``), so the deck path and the step
+share one implementation of both.
+
+The reference's text helpers (``gxl::getline``, ``getline_to_stream``,
+``read_until``, ``to_stringstream``, ``trim_left``; gxl_lib, not vendored)
+are read as: one std::getline, then an optional case conversion of the whole
+line; ``read_until`` reads lines until one starts with the given word.
+
+Decisions where the reference's behaviour is undefined or silent:
+  * a label in 6_inflow.txt / 7_wall.txt that no boundary uses, or a used
+    label without a record, raises (the reference writes record 0 or leaves
+    device memory uninitialised);
+  * an element in the therm file that the mechanism does not declare, a
+    species without therm or transport data, or a therm line shorter than 75
+    columns raises;
+  * label 3 faces become the library's slip wall (the reference applies
+    nothing to them, BoundCond.cu:290);
+  * CR-LF line ends are read as LF.
+"""
+from __future__ import annotations
+
+import os
+import re
+
+import numpy as np
+
+from . import abi
+from .cases import BlockSpec, Case, NGG
+
+SETUP_FILES = ("0_global_control.txt", "1_grid_information.txt", "2_scheme.txt", "3_species_reactions.txt",
+               "4_laminar_turbulent.txt", "9_transport_property.txt")  # Parameter.h:55-62
+
+R_U = 8314.462618  # Constants.h:19
+MW_AIR = 28.9647   # Constants.h:28
+AVOGADRO = 6.02214076e26   # 1/kmol
+ELECTRON_MASS = 9.1093837015e-31
+
+# Standard atomic weights (IUPAC 2021 abridged values) of the elements a
+# combustion mechanism names, by upper-case symbol; D and E as isotope /
+# electron entries (Element.cpp:8-147 holds the same published values).
+ATOMIC_WEIGHT = {
+    "H": 1.008, "HE": 4.002602, "LI": 6.94, "BE": 9.0121831, "B": 10.81, "C": 12.011, "N": 14.007,
+    "O": 15.999, "F": 18.998403163, "NE": 20.1797, "NA": 22.98976928, "MG": 24.305, "AL": 26.9815384,
+    "SI": 28.085, "P": 30.973761998, "S": 32.06, "CL": 35.45, "AR": 39.95, "K": 39.0983, "CA": 40.078,
+    "FE": 55.845, "KR": 83.798, "XE": 131.293,
+    "D": 2.0141017781, "E": ELECTRON_MASS * AVOGADRO,
+}
+
+
+class DeckError(ValueError):
+    pass
+
+
+# ------------------------------------------------------------ text streams ---
+class _Stream:
+    """A text file read as the reference's std::ifstream is: whole lines
+    (std::getline) or whitespace-separated tokens (operator>>) from the
+    current position."""
+
+    def __init__(self, path):
+        self.path = path
+        try:
+            with open(path, "r", encoding="latin-1") as f:
+                self.s = f.read().replace("\r\n", "\n")
+        except OSError as e:
+            raise DeckError(f"cannot read {path}: {e}") from None
+        self.pos = 0
+
+    def getline(self, case=None):
+        if self.pos >= len(self.s):
+            return None
+        e = self.s.find("\n", self.pos)
+        if e < 0:
+            e = len(self.s)
+        line = self.s[self.pos:e]
+        self.pos = e + 1
+        return _case(line, case)
+
+    def token(self):
+        m = _TOKEN.match(self.s, self.pos)
+        if not m:
+            self.pos = len(self.s)
+            return None
+        self.pos = m.end()
+        return m.group(1)
+
+    def read_until(self, word, case=None):
+        while True:
+            line = self.getline(case)
+            if line is None or line.startswith(word):
+                return line
+
+
+_TOKEN = re.compile(r"\s*(\S+)")
+_INT = re.compile(r"[+-]?\d+")
+_FLOAT = re.compile(r"[+-]?(\d+\.?\d*|\.\d+)([eE][+-]?\d+)?")
+
+
+def _case(line, case):
+    return line if case is None else (line.upper() if case == "upper" else line.lower())
+
+
+def _int(tok):
+    """operator>> into an int: the leading integer of the token, 0 on failure."""
+    m = _INT.match(tok or "")
+    return int(m.group(0)) if m else 0
+
+
+def _float(tok):
+    """operator>> into a double: the leading number of the token, 0 on failure."""
+    m = _FLOAT.match(tok or "")
+    return float(m.group(0)) if m else 0.0
+
+
+def _stod(s):
+    """std::stod: leading blanks skipped, leading number parsed; raises if none."""
+    m = _FLOAT.match(s.lstrip())
+    if not m:
+        raise DeckError(f"not a number: {s!r}")
+    return float(m.group(0))
+
+
+def _setup(root, name):
+    return os.path.join(root, "input_files", "setup", name)
+
+
+# -------------------------------------------------------------- parameters ---
+def read_parameters(root):
+    """Typed keys of the six setup files, first definition wins (std::map
+    emplace), plus the derived ngg and n_var (Parameter.cpp:26-71)."""
+    par = {}
+    for name in SETUP_FILES:
+        path = _setup(root, name)
+        if not os.path.exists(path):
+            continue   # an unopened std::ifstream reads nothing
+        with open(path, "r", encoding="latin-1") as f:
+            for raw in f.read().replace("\r\n", "\n").split("\n"):
+                if raw.startswith("//") or raw.startswith("!") or raw == "":
+                    continue
+                tok = raw.split()
+                typ = tok[0] if tok else ""
+                key = tok[1] if len(tok) > 1 else ""
+                val = tok[3] if len(tok) > 3 else None
+                if typ == "int":
+                    v = _int(val)
+                elif typ == "double":
+                    v = _float(val)
+                elif typ == "bool":
+                    v = _int(val) != 0
+                elif typ == "string":
+                    v = val if val is not None else ""
+                else:
+                    continue
+                par.setdefault(key, v)
+    par.setdefault("ngg", 2)
+    if par.get("inviscid_scheme", 0) // 10 == 5:
+        par["ngg"] = 3
+    par["n_var"] = 5
+    return par
+
+
+# -------------------------------------------------------------------- grid ---
+def grid_path(root, myid=0):
+    return os.path.join(root, "input_files", "grid", f"grid{myid:>4}.grd")
+
+
+def read_grid(root, myid=0):
+    """[(mx, my, mz, x, y, z)] with coordinates flattened i fastest
+    (Mesh.cpp:566-604)."""
+    st = _Stream(grid_path(root, myid))
+    head = st.getline()
+    if head is None:
+        raise DeckError("empty grid file")
+    n_block = _int(head.split()[0] if head.split() else "")
+    dims = []
+    for _ in range(n_block):
+        t = (st.getline() or "").split()
+        if len(t) < 3:
+            raise DeckError("grid file: block extents missing")
+        dims.append(tuple(_int(v) for v in t[:3]))
+    need = sum(3 * a * b * c for a, b, c in dims)
+    vals = np.fromstring(st.s[st.pos:], dtype=np.float64, sep=" ")   # text mode: parsed in C
+    if vals.size < need:
+        raise DeckError(f"grid file holds {vals.size} coordinates, {need} needed")
+    out, o = [], 0
+    for mx, my, mz in dims:
+        n = mx * my * mz
+        xyz = [vals[o + q * n:o + (q + 1) * n].copy() for q in range(3)]
+        o += 3 * n
+        out.append((mx, my, mz, *xyz))
+    return out
+
+
+def read_boundary(root, n_block, myid=0):
+    """Per block the boundary lines (x1, x2, y1, y2, z1, z2, label)
+    (Mesh.cpp:606-626)."""
+    st = _Stream(os.path.join(root, "input_files", "boundary_condition", f"boundary{myid:>4}.txt"))
+    out = []
+    for _ in range(n_block):
+        line = st.getline()
+        if line is None:
+            raise DeckError("boundary file: block count line missing")
+        faces = []
+        for _ in range(_int(line.strip())):
+            t = (st.getline() or "").split()
+            if len(t) < 7:
+                raise DeckError("boundary file: short face line")
+            faces.append(tuple(_int(v) for v in t[:7]))
+        out.append(faces)
+    return out
+
+
+def read_inner(root, n_block, myid=0):
+    """Per block the inner faces (src[6], tar[6], target block 1-based):
+    a header line, n_block face counts, the rest of that line and one more
+    line, then two lines of 7 integers per face (Mesh.cpp:628-670)."""
+    path = os.path.join(root, "input_files", "boundary_condition", f"inner{myid:>4}.txt")
+    if not os.path.exists(path):
+        return [[] for _ in range(n_block)]
+    st = _Stream(path)
+    st.getline()
+    counts = [_int(st.token()) for _ in range(n_block)]
+    st.getline()
+    st.getline()
+    out = []
+    for b in range(n_block):
+        faces = []
+        for _ in range(counts[b]):
+            i = [_int(v) for v in (st.getline() or "").split()[:7]]
+            j = [_int(v) for v in (st.getline() or "").split()[:7]]
+            if len(i) < 6 or len(j) < 7:
+                raise DeckError("inner file: short face line")
+            faces.append((i[:6], j[:6], j[6]))
+        out.append(faces)
+    return out
+
+
+# --------------------------------------------------------------- chemistry ---
+def read_mechanism(path):
+    """(elements, species) in file order (ChemData.cpp:10-45)."""
+    st = _Stream(path)
+    st.getline()
+    st.getline()
+    st.getline()
+    st.getline()
+    elements = []
+    while True:
+        line = st.getline("upper")
+        if line is None:
+            break
+        t = line.split()
+        if t and t[0] == "SPECIES":
+            break
+        for e in t:
+            if e not in elements:
+                elements.append(e)
+    st.getline()
+    head = st.getline()
+    n_spec = _int((head or "").split()[0] if (head or "").split() else "")
+    st.getline()
+    species = []
+    while True:
+        line = st.getline("upper")
+        if line is None:
+            break
+        t = line.split()
+        if t and t[0] == "REACTION":
+            break
+        for s in t:
+            if len(species) >= n_spec:
+                break
+            if s not in species:
+                species.append(s)
+    if len(species) != n_spec:
+        raise DeckError(f"mechanism declares {n_spec} species, lists {len(species)}")
+    return elements, species
+
+
+def read_therm(path, species, elements):
+    """NASA-7 records in the fixed-column CHEMKIN layout: name in columns
+    1-18, element counts in four 5-column fields from column 25, T_low /
+    T_high / T_mid in 10-column fields from column 46; three lines of
+    15-column coefficients (high set a1..a7, then the low set).  Molecular
+    weights follow from the element composition (ChemData.cpp:106-228).
+    The global temperature line after THERMO does not change the per-species
+    defaults (300 / 1000 / 5000 K): the reference resizes vectors that
+    already have n_spec entries."""
+    ns = len(species)
+    index = {s: i for i, s in enumerate(species)}
+    t_low = np.full(ns, 300.0)
+    t_mid = np.full(ns, 1000.0)
+    t_high = np.full(ns, 5000.0)
+    lo = np.zeros((ns, 7))
+    hi = np.zeros((ns, 7))
+    comp = np.zeros((ns, len(elements)), dtype=np.int64)
+    seen = np.zeros(ns, dtype=bool)
+    st = _Stream(path)
+    if st.read_until("THERMO", "upper") is None:
+        raise DeckError("therm file without THERMO")
+    st.getline()
+    line = st.getline("upper")
+    n_read = 0
+    while line is not None and n_read < ns:
+        key = line.split()[0] if line.split() else ""
+        if key == "END":
+            break
+        if key == "!" or line.startswith("!") or line == "":
+            line = st.getline("upper")
+            continue
+        name = line[:18].split()[0] if line[:18].split() else ""
+        if name not in index:
+            st.getline()
+            st.getline()
+            st.getline()
+            line = st.getline("upper")
+            continue
+        if len(line) < 75:
+            raise DeckError(f"therm record of {name}: line shorter than 75 columns")
+        l = index[name]
+        t_low[l] = _stod(line[45:55])
+        t_high[l] = _stod(line[55:65])
+        if line[65:75].strip():
+            t_mid[l] = _stod(line[65:75])
+        for q in range(4):
+            f = line[24 + 5 * q:29 + 5 * q].lstrip()
+            if f == "" or f.startswith("0"):
+                break
+            t = f.split()
+            el = t[0].upper()
+            if el not in elements:
+                raise DeckError(f"therm record of {name}: element {el} not declared in the mechanism")
+            comp[l, elements.index(el)] = _int(t[1]) if len(t) > 1 else 0
+        rows = [st.getline() or "" for _ in range(3)]
+        c = [_stod(rows[r][15 * q:15 * q + 15]) for r in range(3) for q in range(5 if r < 2 else 4)]
+        hi[l, :] = c[0:7]
+        lo[l, :] = c[7:14]
+        seen[l] = True
+        n_read += 1
+        line = st.getline()
+    if not seen.all():
+        missing = [species[i] for i in range(ns) if not seen[i]]
+        raise DeckError(f"no therm data for {missing}")
+    for e in elements:
+        if e not in ATOMIC_WEIGHT:
+            raise DeckError(f"unknown element {e}")
+    mw = np.zeros(ns)
+    for i in range(ns):   # summed over the element map in its (sorted) key order
+        for e in sorted(elements):
+            mw[i] += ATOMIC_WEIGHT[e] * comp[i, elements.index(e)]
+    return dict(mw=mw, t_low=t_low, t_mid=t_mid, t_high=t_high, coeff_lo=lo, coeff_hi=hi, elem_comp=comp)
+
+
+def read_tran(path, species):
+    """Lennard-Jones well depth eps/k [K] and diameter sigma [A]: columns
+    name, geometry, eps/k, sigma of the transport file, up to END
+    (ChemData.cpp:230-255)."""
+    index = {s: i for i, s in enumerate(species)}
+    eps = np.zeros(len(species))
+    sig = np.zeros(len(species))
+    st = _Stream(path)
+    line = st.getline("upper")
+    while line is not None:
+        t = line.split()
+        key = t[0] if t else ""
+        if key.startswith("END"):
+            break
+        if key in index and len(t) >= 4:
+            eps[index[key]] = _float(t[2])
+            sig[index[key]] = _float(t[3])
+        line = st.getline("upper")
+    if not ((eps > 0) & (sig > 0)).all():
+        missing = [species[i] for i in range(len(species)) if not (eps[i] > 0 and sig[i] > 0)]
+        raise DeckError(f"no transport data for {missing}")
+    return eps, sig
+
+
+# --------------------------------------------------------------- conditions ---
+def read_bc_types(root, grid_labels):
+    """Labels per condition kind: the named lines of 5_boundary_condition.txt
+    for labels the boundary files use, then the defaults 2 (wall), 5 (inflow)
+    and 6 (outflow) for labels left over (BoundCond.cu:316-392)."""
+    left = list(grid_labels)
+    kinds = {"wall": [], "inflow": [], "outflow": []}
+    path = _setup(root, "5_boundary_condition.txt")
+    if os.path.exists(path):
+        st = _Stream(path)
+        while True:
+            line = st.getline("lower")
+            if line is None:
+                break
+            t = line.split()
+            typ = t[0] if t else ""
+            if line.startswith("//"):
+                continue
+            if typ == "end":
+                break
+            lab = _int(t[2]) if len(t) > 2 else 0
+            if lab in left:
+                left.remove(lab)
+                if typ in kinds:
+                    kinds[typ].append(lab)
+    for lab in left:
+        if lab == 2:
+            kinds["wall"].append(lab)
+        elif lab == 5:
+            kinds["inflow"].append(lab)
+        elif lab == 6:
+            kinds["outflow"].append(lab)
+    return kinds
+
+
+def _inflow_record(st, species):
+    """Keys of one inflow record up to the next ``label`` / ``end`` line
+    (BoundCond.cpp:10-41); returns (record, the line that ended it)."""
+    par, spec = {}, {}
+    while True:
+        line = st.getline("lower")
+        if line is None:
+            break
+        t = line.split()
+        key = t[0] if t else ""
+        if key == "double" and len(t) >= 4:
+            par.setdefault(t[1], _float(t[3]))
+            continue
+        if key == "species" and len(t) >= 4:
+            spec.setdefault(t[1].upper(), _float(t[3]))
+            continue
+        if key in ("label", "end"):
+            break
+    rec = {k: par[k] for k in ("mach", "pressure", "temperature", "velocity", "density", "u", "v", "w") if k in par}
+    if species is not None:
+        y = np.zeros(len(species))
+        for name, v in spec.items():
+            if name in species:
+                y[species.index(name)] = v
+        rec["yk"] = y
+    return rec, line
+
+
+def read_inflow_records(root, species):
+    """{label: record} of 6_inflow.txt, records in file order."""
+    st = _Stream(_setup(root, "6_inflow.txt"))
+    out = {}
+    line = st.read_until("label", "lower")
+    while line is not None and not line.startswith("end"):
+        t = line.split()
+        lab = _int(t[1]) if len(t) > 1 else 0
+        rec, line = _inflow_record(st, species)
+        if lab in out:
+            raise DeckError(f"6_inflow.txt: label {lab} given twice")
+        out[lab] = rec
+    return out
+
+
+def read_wall_records(root):
+    """{label: record} of 7_wall.txt (BoundCond.cpp:112-141): isothermal at
+    300 K unless ``option thermal_type = adiabatic`` or a temperature is
+    given."""
+    st = _Stream(_setup(root, "7_wall.txt"))
+    out = {}
+    line = st.read_until("label", "lower")
+    while line is not None and not line.startswith("end"):
+        t = line.split()
+        lab = _int(t[1]) if len(t) > 1 else 0
+        par, opt = {}, {}
+        while True:
+            line = st.getline("lower")
+            if line is None:
+                break
+            w = line.split()
+            key = w[0] if w else ""
+            if key == "double" and len(w) >= 4:
+                par.setdefault(w[1], _float(w[3]))
+            elif key == "option" and len(w) >= 4:
+                opt.setdefault(w[1], w[3])
+                key = w[3]
+            if key in ("label", "end"):
+                break
+        adiabatic = opt.get("thermal_type", "isothermal") != "isothermal"
+        if lab in out:
+            raise DeckError(f"7_wall.txt: label {lab} given twice")
+        out[lab] = dict(adiabatic=int(adiabatic), temperature=par.get("temperature", 300.0))
+    return out
+
+
+def read_initialization(root):
+    """Inflow labels of the initialisation groups and, for groups after the
+    first, their boxes ((xs, xe), (ys, ye), (zs, ze)) (Initialize.cu:24-104)."""
+    st = _Stream(_setup(root, "8_initialization.txt"))
+    n_group = 1
+    while True:
+        line = st.getline("lower")
+        if line is None:
+            break
+        t = line.split()
+        if t and t[0] == "//":
+            continue
+        if t and t[0] == "int":
+            n_group = _int(t[3]) if len(t) > 3 else 0
+            break
+    groups = []
+    line = st.read_until("label", "lower")
+    while len(groups) < n_group and line is not None and not line.startswith("end"):
+        t = line.split()
+        lab = _int(t[1]) if len(t) > 1 else 0
+        box = None
+        if groups:
+            box = []
+            for _ in range(3):
+                w = (st.getline() or "").split()
+                box.append((_float(w[3]) if len(w) > 3 else 0.0, _float(w[4]) if len(w) > 4 else 0.0))
+        groups.append((lab, box))
+        line = st.read_until("label", "lower")
+    return groups
+
+
+# ------------------------------------------------------------------- case ---
+def load_case(root, myid=0, precision=abi.FP64, transport=abi.TRANSPORT_WILKE, quirks=abi.QUIRKS_REFERENCE,
+              name=None):
+    """The reference case under ``root`` (the directory holding
+    ``input_files/``) as a Case for the step library."""
+    from . import lib   # host-side setup of the step library
+
+    par = read_parameters(root)
+    if par["ngg"] != NGG:
+        raise DeckError(f"inviscid_scheme {par.get('inviscid_scheme')} needs ngg {par['ngg']}; the library has {NGG}")
+    species = None
+    names = None
+    if "mechanism_file" in par:
+        inp = os.path.join(root, "input_files")
+        elements, names = read_mechanism(os.path.join(inp, par["mechanism_file"]))
+        th = read_therm(os.path.join(inp, par["therm_file"]), names, elements)
+        eps, sig = read_tran(os.path.join(inp, par["transport_file"]), names)
+        species = {k: th[k] for k in ("mw", "t_low", "t_mid", "t_high", "coeff_lo", "coeff_hi")}
+        species.update(eps_k=eps, sigma=sig, names=names)
+    grid = read_grid(root, myid)
+    scale = par.get("gridScale", 1.0)
+    if abs(scale - 1.0) > 1e-15:   # Mesh.cpp:534-536, 704-716
+        grid = [(mx, my, mz, x * scale, y * scale, z * scale) for mx, my, mz, x, y, z in grid]
+    n_block = len(grid)
+    bnd = read_boundary(root, n_block, myid)
+    inner = read_inner(root, n_block, myid)
+    labels = []
+    for faces in bnd:
+        for f in faces:
+            if f[6] not in labels:
+                labels.append(f[6])
+    kinds = read_bc_types(root, labels)
+    init = read_initialization(root) if os.path.exists(_setup(root, "8_initialization.txt")) else []
+    needs_inflow = kinds["inflow"] or init
+    in_rec = read_inflow_records(root, names) if needs_inflow else {}
+    wall_rec = read_wall_records(root) if kinds["wall"] else {}
+    inflows = []
+    for lab in kinds["inflow"]:
+        if lab not in in_rec:
+            raise DeckError(f"inflow label {lab} has no record in 6_inflow.txt")
+        inflows.append(in_rec[lab])
+    walls = []
+    for lab in kinds["wall"]:
+        if lab not in wall_rec:
+            raise DeckError(f"wall label {lab} has no record in 7_wall.txt")
+        walls.append(wall_rec[lab])
+    cond = {}
+    for typ, kind in ((abi.BC_WALL, "wall"), (abi.BC_INFLOW, "inflow"), (abi.BC_OUTFLOW, "outflow")):
+        for c, lab in enumerate(kinds[kind]):
+            cond[lab] = (typ, c)
+    blocks = []
+    for b, (mx, my, mz, x, y, z) in enumerate(grid):
+        bc = []
+        for x1, x2, y1, y2, z1, z2, lab in bnd[b]:
+            if lab == 3:
+                typ, c = abi.BC_SYMMETRY, 0
+            elif lab in cond:
+                typ, c = cond[lab]
+            else:
+                raise DeckError(f"block {b + 1}: boundary label {lab} has no condition")
+            bc.append(((x1, y1, z1), (x2, y2, z2), typ, c))
+        blocks.append(BlockSpec(mx, my, mz, x, y, z, bc=bc, inner=[tuple(f) for f in inner[b]]))
+    params = dict(inviscid_scheme=par.get("inviscid_scheme", 3), reconstruction=par.get("reconstruction", 2),
+                  limiter=par.get("limiter", 0), viscous_order=par.get("viscous_order", 2),
+                  temporal_scheme=par.get("temporal_scheme", 0), rk_stages=par.get("rk_stages", 1),
+                  cfl=par.get("cfl", 0.5), prandtl=par.get("prandtl_number", 0.72),
+                  schmidt=par.get("schmidt_number", 0.9), gamma_air=1.4, r_air=R_U / MW_AIR,
+                  precision=precision, transport=transport, quirks=quirks)
+    case = Case(name or os.path.basename(os.path.abspath(root)), params, species, blocks, inflows, walls,
+                [None] * n_block, [None] * n_block)
+    case.deck = par
+    if not init:
+        raise DeckError("8_initialization.txt: no initialisation group")
+    # the group states, derived as inflow records are (Initialize.cu:52-72)
+    state = Case(case.name, params, species, blocks, [in_rec[lab] if lab in in_rec else _missing(lab)
+                                                      for lab, _ in init], [], [], [])
+    st = [lib.inflow_state(state, g) for g in range(len(init))]
+    coords = lib.ghost_grid(case) if len(init) > 1 else None
+    ns = case.n_spec
+    for b, blk in enumerate(blocks):
+        shp = blk.padded_shape
+        sel = np.zeros(shp, dtype=np.int64)
+        if coords is not None:   # first box that holds the point, else group 0 (Field.cu:48-61)
+            X, Y, Z = coords[b]
+            for g in range(len(init) - 1, 0, -1):
+                (xs, xe), (ys, ye), (zs, ze) = init[g][1]
+                inside = (X >= xs) & (X <= xe) & (Y >= ys) & (Y <= ye) & (Z >= zs) & (Z <= ze)
+                sel[inside] = g
+        bv = np.empty((6,) + shp)
+        for v in range(6):
+            bv[v] = np.array([s[0][v] for s in st])[sel]
+        case.bv0[b] = bv
+        if ns:
+            yk = np.empty((ns,) + shp)
+            for l in range(ns):
+                yk[l] = np.array([s[1][l] for s in st])[sel]
+            case.yk0[b] = yk
+    return case
+
+
+def _missing(lab):
+    raise DeckError(f"initialisation label {lab} has no record in 6_inflow.txt")
+
+
+# ------------------------------------------------------------------ writer ---
+def _e15(v):
+    return f"{v:15.8E}"
+
+
+def write_deck(root, case, *, species_names=None, elements=None, elem_comp=None, labels=None, init=None,
+               extra_params=None):
+    """Write ``case`` as a reference deck under ``root``.
+
+    species_names / elements / elem_comp (n_spec x n_elem integers) describe
+    the species; the molecular weights the reader derives from them replace
+    case.species['mw'].  labels maps (type, cond) to the file label
+    (default: wall 2 / inflow 5 / outflow 6 for condition 0, then 20+,
+    50+, 60+).  init lists the initialisation groups as (inflow record,
+    box or None).  Coefficients are written in the 15-column E format
+    (9 significant digits)."""
+    inp = os.path.join(root, "input_files")
+    for d in ("setup", "grid", "boundary_condition"):
+        os.makedirs(os.path.join(inp, d), exist_ok=True)
+    p = case.params
+    lines = [
+        "// written by combustioncodegpu_amd.deck",
+        f"int inviscid_scheme = {p.get('inviscid_scheme', 3)}",
+        f"int reconstruction = {p.get('reconstruction', 2)}",
+        f"int limiter = {p.get('limiter', 0)}",
+        f"int viscous_order = {p.get('viscous_order', 2)}",
+        f"int temporal_scheme = {p.get('temporal_scheme', 0)}",
+        f"int rk_stages = {p.get('rk_stages', 1)}",
+        f"double cfl = {p.get('cfl', 0.5)!r}",
+        f"double prandtl_number = {p.get('prandtl', 0.72)!r}",
+        f"double schmidt_number = {p.get('schmidt', 0.9)!r}",
+        "double gridScale = 1.0",
+        f"int dimension = {2 if case.blocks[0].mz == 1 else 3}",
+        "int initial = 0",
+    ]
+    for k, v in (extra_params or {}).items():
+        typ = "int" if isinstance(v, int) else ("double" if isinstance(v, float) else "string")
+        lines.append(f"{typ} {k} = {v}")
+    sp = case.species
+    if sp is not None:
+        lines += ["string mechanism_file = chemistry/mechanism.inp", "string therm_file = chemistry/therm.dat",
+                  "string transport_file = chemistry/tran.dat"]
+    with open(_setup(root, "0_global_control.txt"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    # grid
+    with open(grid_path(root), "w") as f:
+        f.write(f"{len(case.blocks)}\n")
+        for b in case.blocks:
+            f.write(f"{b.mx} {b.my} {b.mz}\n")
+        for b in case.blocks:
+            for a in (b.x, b.y, b.z):
+                np.ascontiguousarray(a, dtype=np.float64).tofile(f, sep="\n", format="%.17g")
+                f.write("\n")
+    # conditions and labels
+    lab = dict(labels or {})
+    base = {abi.BC_WALL: (2, 20), abi.BC_INFLOW: (5, 50), abi.BC_OUTFLOW: (6, 60)}
+    for b in case.blocks:
+        for _, _, typ, c in b.bc:
+            if typ == abi.BC_SYMMETRY or (typ, c) in lab:
+                continue
+            lab[(typ, c)] = base[typ][0] if c == 0 else base[typ][1] + c
+    with open(os.path.join(inp, "boundary_condition", f"boundary{0:>4}.txt"), "w") as f:
+        for b in case.blocks:
+            f.write(f"{len(b.bc)}\n")
+            for rs, re_, typ, c in b.bc:
+                L = 3 if typ == abi.BC_SYMMETRY else lab[(typ, c)]
+                f.write(f"{rs[0]} {re_[0]} {rs[1]} {re_[1]} {rs[2]} {re_[2]} {L}\n")
+    with open(os.path.join(inp, "boundary_condition", f"inner{0:>4}.txt"), "w") as f:
+        f.write(f"{len(case.blocks)}\n")
+        f.write(" ".join(str(len(b.inner)) for b in case.blocks) + "\n")
+        f.write("src / target\n")
+        for bi, b in enumerate(case.blocks):
+            for src, tar, tb in b.inner:
+                f.write(" ".join(str(v) for v in src) + f" {bi + 1}\n")
+                f.write(" ".join(str(v) for v in tar) + f" {tb}\n")
+    kinds = {abi.BC_WALL: "wall", abi.BC_INFLOW: "inflow", abi.BC_OUTFLOW: "outflow"}
+    with open(_setup(root, "5_boundary_condition.txt"), "w") as f:
+        for (typ, c), L in sorted(lab.items(), key=lambda kv: kv[1]):
+            f.write(f"{kinds[typ]} {kinds[typ]}_{c} {L}\n")
+        f.write("end\n")
+
+    def inflow_lines(rec):
+        out = []
+        for k in ("mach", "pressure", "temperature", "velocity", "density", "u", "v", "w"):
+            if k in rec:
+                out.append(f"double {k} = {float(rec[k])!r}")
+        if rec.get("yk") is not None:
+            for n, y in zip(species_names, rec["yk"]):
+                if y != 0:
+                    out.append(f"species {n} = {float(y)!r}")
+        return out
+
+    init = list(init or [])
+    with open(_setup(root, "6_inflow.txt"), "w") as f:
+        for c, rec in enumerate(case.inflow):
+            f.write(f"label {lab.get((abi.BC_INFLOW, c), 50 + c)}\n" + "\n".join(inflow_lines(rec)) + "\n")
+        for g, (rec, _) in enumerate(init):
+            f.write(f"label {900 + g}\n" + "\n".join(inflow_lines(rec)) + "\n")
+        f.write("end\n")
+    with open(_setup(root, "7_wall.txt"), "w") as f:
+        for c, w in enumerate(case.wall):
+            f.write(f"label {lab.get((abi.BC_WALL, c), 20 + c)}\n")
+            f.write(f"option thermal_type = {'adiabatic' if w.get('adiabatic') else 'isothermal'}\n")
+            f.write(f"double temperature = {float(w.get('temperature', 300.0))!r}\n")
+        f.write("end\n")
+    with open(_setup(root, "8_initialization.txt"), "w") as f:
+        f.write(f"int groups = {len(init)}\n")
+        for g, (_, box) in enumerate(init):
+            f.write(f"label {900 + g}\n")
+            if g:
+                for ax, (a, b_) in zip("xyz", box):
+                    f.write(f"double {ax}_range = {a!r} {b_!r}\n")
+        f.write("end\n")
+    if sp is None:
+        return
+    names = list(species_names)
+    ch = os.path.join(inp, "chemistry")
+    os.makedirs(ch, exist_ok=True)
+    with open(os.path.join(ch, "mechanism.inp"), "w") as f:
+        f.write("Elements\n--------\n" + f"{len(elements)} elements\n--------\n")
+        f.write(" ".join(elements) + "\n")
+        f.write("Species\n--------\n" + f"{len(names)} species\n--------\n")
+        f.write(" ".join(names) + "\n")
+        f.write("Reactions\n")
+    with open(os.path.join(ch, "therm.dat"), "w") as f:
+        f.write("THERMO\n   300.000  1000.000  5000.000\n")
+        for l, n in enumerate(names):
+            comp = ""
+            for q, e in enumerate(elements):
+                if elem_comp[l][q] and len(comp) < 20:
+                    comp += f"{e:<2}{int(elem_comp[l][q]):>3}"
+            comp = f"{comp:<20}"
+            head = f"{n:<18}{'':6}{comp}G{sp['t_low'][l]:10.3f}{sp['t_high'][l]:10.3f}{sp['t_mid'][l]:8.2f}"
+            f.write(f"{head:<79}1\n")
+            hi, lo = sp["coeff_hi"][l], sp["coeff_lo"][l]
+            f.write("".join(_e15(v) for v in hi[:5]) + "    2\n")
+            f.write("".join(_e15(v) for v in (hi[5], hi[6], lo[0], lo[1], lo[2])) + "    3\n")
+            f.write("".join(_e15(v) for v in lo[3:7]) + "                   4\n")
+        f.write("END\n")
+    with open(os.path.join(ch, "tran.dat"), "w") as f:
+        for l, n in enumerate(names):
+            f.write(f"{n:<16}{1:>4}{sp['eps_k'][l]:>10.3f}{sp['sigma'][l]:>10.3f}{0.0:>10.3f}{0.0:>10.3f}"
+                    f"{1.0:>10.3f}\n")
+        f.write("END\n")

@@ -93,7 +93,7 @@ class StepContext:
         ns, nv = self.case.n_spec, self.case.n_var
         return {
             abi.CV: (nv,) + pad, abi.BV: (6,) + pad, abi.YK: (ns,) + pad, abi.PROPS: (4 + ns,) + pad,
-            abi.DQ: (nv,) + inner, abi.METRIC: (10,) + pad, abi.DTLOCAL: inner,
+            abi.DQ: (nv,) + inner, abi.METRIC: (10,) + pad, abi.DTLOCAL: inner, abi.COORD: (3,) + pad,
         }[what]
 
     def download(self, blk, what):

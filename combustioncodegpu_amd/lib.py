@@ -79,8 +79,38 @@ def load():
     lib.cfx_set_transport.restype = C.c_int
     lib.cfx_mlp_stats.argtypes = [vp, C.POINTER(C.c_longlong)]
     lib.cfx_mlp_stats.restype = C.c_int
+    lib.cfx_ghost_grid.argtypes = [C.POINTER(abi.CfxBlock), C.c_int, C.POINTER(dp)]
+    lib.cfx_ghost_grid.restype = C.c_int
+    lib.cfx_inflow_state.argtypes = [C.POINTER(abi.CfxParams), C.POINTER(abi.CfxSpecies), C.c_int, dp, dp]
+    lib.cfx_inflow_state.restype = C.c_int
     _lib = lib
     return lib
+
+
+def ghost_grid(case):
+    """Ghost-padded vertex coordinates of every block, (3, pz, py, px) each,
+    from the library's host-side mesh setup (no device is used)."""
+    lib = load()
+    m = case.marshal()
+    out = [np.empty((3,) + b.padded_shape, dtype=np.float64) for b in case.blocks]
+    ptrs = (C.POINTER(C.c_double) * len(out))(*[abi.dptr(a) for a in out])
+    rc = lib.cfx_ghost_grid(m.blocks, m.n_block, ptrs)
+    if rc:
+        raise RuntimeError(f"cfx_ghost_grid failed ({rc})")
+    return out
+
+
+def inflow_state(case, index):
+    """(rho, u, v, w, p, T) and mass fractions of inflow record `index` as
+    the library derives them (host only)."""
+    lib = load()
+    m = case.marshal()
+    bv = np.empty(6, dtype=np.float64)
+    yk = np.zeros(max(1, case.n_spec), dtype=np.float64)
+    rc = lib.cfx_inflow_state(C.byref(m.params), C.byref(m.species), index, abi.dptr(bv), abi.dptr(yk))
+    if rc:
+        raise RuntimeError(f"cfx_inflow_state failed ({rc})")
+    return bv, yk[:case.n_spec]
 
 
 class Solver(StepContext):

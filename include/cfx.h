@@ -80,6 +80,7 @@ extern "C" {
 #define CFX_DQ     4 /* n_var x interior (mx*my*mz, i fastest) */
 #define CFX_METRIC 5 /* 10 x padded: xi_x,xi_y,xi_z,eta_x..,zeta_z, jacobian */
 #define CFX_DTLOCAL 6 /* interior: local time step of the last cfx_time_step */
+#define CFX_COORD  7 /* 3 x padded: x, y, z vertex coordinates with the ghost grid */
 
 /* One physical boundary face, as a line of the reference's boundary file
  * (Mesh.cpp:606-615, Boundary ctor Mesh.cpp:10-12): 0-based vertex ranges,
@@ -163,6 +164,23 @@ typedef struct cfx_ctx cfx_ctx;
 
 /* Library version / capability query. */
 int cfx_version(void);
+
+/* Host-only helpers (no device, no context): what the reference's setup
+ * computes before the step loop, for callers that build the initial field
+ * themselves (an input-deck reader).
+ *
+ * cfx_ghost_grid: the ghost-padded vertex coordinates of every block after
+ * the ghost-grid construction (Mesh::init_ghost_grid, Mesh.cpp:718-922) and
+ * the trim to ngg layers (Block::trim_abundant_ghost_mesh, Mesh.cpp:501-515):
+ * xyz[b] receives 3 x n_padded doubles (x, then y, then z) in the padded
+ * layout above.  Blocks are described as for cfx_create.
+ * cfx_inflow_state: the primitive state (rho, u, v, w, p, T) and mass
+ * fractions of inflow record `index` as the reference derives it
+ * (Inflow::register_boundary_condition, BoundCond.cpp:10-105; var_info,
+ * BoundCond.cpp:108-110); yk may be NULL. */
+int cfx_ghost_grid(const cfx_block *blocks, int n_block, double *const *xyz);
+int cfx_inflow_state(const cfx_params *params, const cfx_species *species, int index, double bv[6],
+                     double *yk);
 
 /* Driver ctor + Field::setup_device_memory + DParameter + ghost grid and
  * metrics (Driver.cu:71-111, Field.cu:182-240, DParameter.cu:118-159,

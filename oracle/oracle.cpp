@@ -1385,6 +1385,15 @@ int orc_download(orc_ctx *C, int blk, int what, double *dst) {
       std::memcpy(dst + 9 * np, B.jac.data(), sizeof(double) * np);
       break;
     case CFX_DTLOCAL: std::memcpy(dst, B.dt_local.data(), sizeof(double) * B.ni); break;
+    case CFX_COORD:
+      for (int k = -B.ngg; k < B.mz + B.ngg; ++k)
+        for (int j = -B.ngg; j < B.my + B.ngg; ++j)
+          for (int i = -B.ngg; i < B.mx + B.ngg; ++i) {
+            dst[B.id(i, j, k)] = B.X(i, j, k);
+            dst[np + B.id(i, j, k)] = B.Y(i, j, k);
+            dst[2 * np + B.id(i, j, k)] = B.Z(i, j, k);
+          }
+      break;
     default: return CFX_E_ARG;
   }
   return 0;
