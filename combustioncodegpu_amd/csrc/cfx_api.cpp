@@ -617,6 +617,47 @@ int run_steps(cfx_ctx *ctx, int n_steps) {
 
 extern "C" {
 
+namespace cfx {
+namespace {
+
+// Metrics of one block computed on the device from its ghost-grid
+// coordinates; a block with a non-positive Jacobian anywhere is recomputed on
+// the host with the reference's fallback rules and uploaded.
+int device_metrics(cfx_ctx *ctx, HostBlock &H, const BlockView &V) {
+  const size_t ng = H.xyz[0].size(), np = (size_t)H.np;
+  double *xyz = nullptr;
+  int *bad = nullptr;
+  int rc = (int)hipMalloc(&xyz, sizeof(double) * 3 * ng);
+  if (!rc) rc = (int)hipMalloc(&bad, sizeof(int));
+  for (int c = 0; c < 3 && !rc; ++c)
+    rc = (int)hipMemcpy(xyz + c * ng, H.xyz[c].data(), sizeof(double) * ng, hipMemcpyHostToDevice);
+  if (!rc) rc = (int)hipMemset(bad, 0, sizeof(int));
+  if (!rc) rc = launch_metrics(xyz, H.mx, H.my, H.mz, V.metric, V.jac, V.mixed ? V.metric32 : nullptr,
+                               V.mixed ? V.jac32 : nullptr, bad, nullptr);
+  int n_bad = 0;
+  if (!rc) rc = (int)hipMemcpy(&n_bad, bad, sizeof(int), hipMemcpyDeviceToHost);
+  (void)hipFree(xyz);
+  (void)hipFree(bad);
+  if (rc) return ctx->fail(rc, "device metrics");
+  if (n_bad == 0) return 0;
+  if ((rc = compute_metrics(H, ctx->err))) return rc;
+  rc = (int)hipMemcpy(V.metric, H.metric.data(), sizeof(double) * 9 * np, hipMemcpyHostToDevice);
+  if (!rc) rc = (int)hipMemcpy(V.jac, H.jac.data(), sizeof(double) * np, hipMemcpyHostToDevice);
+  if (!rc && V.mixed) {
+    std::vector<float> m32(9 * np), j32(np);
+    for (size_t i = 0; i < 9 * np; ++i) m32[i] = (float)H.metric[i];
+    for (size_t i = 0; i < np; ++i) j32[i] = (float)H.jac[i];
+    rc = (int)hipMemcpy(V.metric32, m32.data(), sizeof(float) * 9 * np, hipMemcpyHostToDevice);
+    if (!rc) rc = (int)hipMemcpy(V.jac32, j32.data(), sizeof(float) * np, hipMemcpyHostToDevice);
+  }
+  std::vector<double>().swap(H.metric);
+  std::vector<double>().swap(H.jac);
+  return rc ? ctx->fail(rc, "metric upload") : 0;
+}
+
+}  // namespace
+}  // namespace cfx
+
 int cfx_version(void) { return CFX_ABI_VERSION; }
 
 int cfx_ghost_grid(const cfx_block *blocks, int n_block, double *const *xyz) {
@@ -624,7 +665,7 @@ int cfx_ghost_grid(const cfx_block *blocks, int n_block, double *const *xyz) {
   std::vector<HostBlock> hb;
   int dim = 0;
   std::string err;
-  if (int rc = build_mesh(blocks, n_block, hb, dim, err)) return rc;
+  if (int rc = build_mesh(blocks, n_block, hb, dim, err, /*metrics=*/false)) return rc;
   for (int b = 0; b < n_block; ++b) {
     const HostBlock &B = hb[b];
     if (!xyz[b]) return CFX_E_ARG;
@@ -685,7 +726,7 @@ int cfx_create(const cfx_params *params, const cfx_species *species, const cfx_b
     ctx->err = "no FP32 kernels built for " + std::to_string(ctx->ns) + " species";
     return bail(CFX_E_UNSUPPORTED);
   }
-  rc = build_mesh(blocks, n_block, ctx->hb, ctx->dim, ctx->err);
+  rc = build_mesh(blocks, n_block, ctx->hb, ctx->dim, ctx->err, /*metrics=*/false);
   if (rc) return bail(rc);
   SchemeDev &S = ctx->S;
   S.reconstruction = params->reconstruction;
@@ -781,8 +822,6 @@ int cfx_create(const cfx_params *params, const cfx_species *species, const cfx_b
     V.dt_local = ctx->dalloc<double>(ni, rc);
     V.bv_last = ctx->dalloc<double>(4 * ni, rc);
     if (rc) return bail(ctx->fail(rc, "hipMalloc block arrays"));
-    (void)hipMemcpy(V.metric, H.metric.data(), sizeof(double) * 9 * np, hipMemcpyHostToDevice);
-    (void)hipMemcpy(V.jac, H.jac.data(), sizeof(double) * np, hipMemcpyHostToDevice);
     V.mixed = ctx->mixed ? 1 : 0;
     if (ctx->mixed) {
       V.bv32 = ctx->dalloc<float>(6 * np, rc);
@@ -792,14 +831,10 @@ int cfx_create(const cfx_params *params, const cfx_species *species, const cfx_b
       V.jac32 = ctx->dalloc<float>(np, rc);
       V.dq32 = ctx->dalloc<float>(ctx->nv * ni, rc);
       if (rc) return bail(ctx->fail(rc, "hipMalloc FP32 arrays"));
-      std::vector<float> m32(9 * np), j32(np);
-      for (size_t i = 0; i < 9 * np; ++i) m32[i] = (float)H.metric[i];
-      for (size_t i = 0; i < np; ++i) j32[i] = (float)H.jac[i];
-      (void)hipMemcpy(V.metric32, m32.data(), sizeof(float) * 9 * np, hipMemcpyHostToDevice);
-      (void)hipMemcpy(V.jac32, j32.data(), sizeof(float) * np, hipMemcpyHostToDevice);
       (void)hipMemset(V.props32, 0, sizeof(float) * (4 + ctx->ns) * np);
       (void)hipMemset(V.dq32, 0, sizeof(float) * ctx->nv * ni);
     }
+    if ((rc = device_metrics(ctx, ctx->hb[b], V))) return bail(rc);
     (void)hipMemset(V.props, 0, sizeof(double) * (4 + ctx->ns) * np);
     (void)hipMemset(V.dq, 0, sizeof(double) * ctx->nv * ni);
     ctx->views.push_back(V);

@@ -103,6 +103,11 @@ int launch_dt_reset(unsigned long long *dt_bits, hipStream_t st);
 int launch_dt(const Batch &b, const SchemeDev &S, int ns, int store_local, unsigned long long *dt_bits,
               hipStream_t st);
 int launch_store_last(const Batch &b, hipStream_t st);
+// metrics and Jacobian of one block from its coordinates over ngg+1 ghost
+// layers (3 planes of (mx+2ngg+2)(my+2ngg+2)(mz+2ngg+2)); FP32 copies when
+// metric32 is non-null; *bad counts points with a non-positive Jacobian
+int launch_metrics(const double *xyz, int mx, int my, int mz, double *metric, double *jac, float *metric32,
+                   float *jac32, int *bad, hipStream_t st);
 // partial sums of workgroup w go to partial[4 * (w)] with w counted from 0 of
 // the full prefix, so a per-block launch and a batched one fill the same slots
 int launch_residual_partial(const Batch &b, double *partial, hipStream_t st);

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
+#include "mesh.h"
 
 namespace cfx {
 
@@ -210,6 +211,52 @@ int launch_dt(const Batch &b, const SchemeDev &S, int ns, int store_local, unsig
               hipStream_t st) {
   if (b.total <= 0) return 0;
   dt_kernel<<<b.total, kTpb, 0, st>>>(b.views, b.off, b.nb, S, ns, store_local, dt_bits);
+  return (int)hipGetLastError();
+}
+
+// Jacobian and metrics of one block (Block::compute_jac_metric,
+// Mesh.cpp:226-488): one thread per point of the ngg-padded range, reading the
+// vertex coordinates over ngg+1 ghost layers (x, y, z planes of gx*gy*gz).
+// FP32 copies are written when metric32 is set.  A point whose Jacobian is
+// not positive is counted in *bad and left unwritten.
+__global__ void __launch_bounds__(kTpb) metric_kernel(const double *__restrict__ xyz, int mx, int my, int mz,
+                                                      double *__restrict__ metric, double *__restrict__ jac,
+                                                      float *__restrict__ metric32, float *__restrict__ jac32,
+                                                      int *bad) {
+  const int px = mx + 2 * kNgg, py = my + 2 * kNgg, pz = mz + 2 * kNgg;
+  const long long np = (long long)px * py * pz;
+  const long long p = (long long)blockIdx.x * kTpb + threadIdx.x;
+  if (p >= np) return;
+  const int i = (int)(p % px) - kNgg;
+  const long long r = p / px;
+  const int j = (int)(r % py) - kNgg, k = (int)(r / py) - kNgg;
+  const int G = kNgg + 1, gx = mx + 2 * G, gy = my + 2 * G;
+  const long long ng = (long long)gx * gy * (mz + 2 * G);
+  // i + di in [-ngg-1, m+ngg]: inside the ngg+1 ghost layers
+  const auto X = [&](int cc, int di, int dj, int dk) {
+    return xyz[cc * ng + ((long long)(k + dk + G) * gy + (j + dj + G)) * gx + (i + di + G)];
+  };
+  double J, m[9];
+  if (!metric_point(X, J, m)) {
+    atomicAdd(bad, 1);
+    return;
+  }
+  jac[p] = J;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) metric[c * np + p] = m[c];
+  if (metric32) {
+    jac32[p] = (float)J;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) metric32[c * np + p] = (float)m[c];
+  }
+}
+
+int launch_metrics(const double *xyz, int mx, int my, int mz, double *metric, double *jac, float *metric32,
+                   float *jac32, int *bad, hipStream_t st) {
+  const long long np = (long long)(mx + 2 * kNgg) * (my + 2 * kNgg) * (mz + 2 * kNgg);
+  const long long nblk = (np + kTpb - 1) / kTpb;
+  if (nblk <= 0 || nblk > 0x7fffffffLL) return (int)hipErrorInvalidValue;
+  metric_kernel<<<(unsigned)nblk, kTpb, 0, st>>>(xyz, mx, my, mz, metric, jac, metric32, jac32, bad);
   return (int)hipGetLastError();
 }
 

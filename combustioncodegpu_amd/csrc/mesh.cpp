@@ -282,29 +282,11 @@ void inner_ghost_grid(std::vector<HostBlock> &blocks, const cfx_block *desc, int
 }
 
 bool metric_at(HostBlock &B, int i, int j, int k, double &J, double m[9]) {
-  double d[3][3];  // d[axis][coord]: d(coord)/d(axis) by central differences
-  for (int cc = 0; cc < 3; ++cc) {
-    d[0][cc] = (B.coord(cc, i + 1, j, k) - B.coord(cc, i - 1, j, k)) * 0.5;
-    d[1][cc] = (B.coord(cc, i, j + 1, k) - B.coord(cc, i, j - 1, k)) * 0.5;
-    d[2][cc] = (B.coord(cc, i, j, k + 1) - B.coord(cc, i, j, k - 1)) * 0.5;
-  }
-  const double xa = d[0][0], ya = d[0][1], za = d[0][2];
-  const double xb = d[1][0], yb = d[1][1], zb = d[1][2];
-  const double xc = d[2][0], yc = d[2][1], zc = d[2][2];
-  J = xa * yb * zc + xb * yc * za + xc * ya * zb - xa * yc * zb - xb * ya * zc - xc * yb * za;
-  if (J <= 0) return false;
-  // grad(xi) = (dr/deta x dr/dzeta)/J and cyclic
-  m[0] = (yb * zc - yc * zb) / J;
-  m[1] = (xc * zb - xb * zc) / J;
-  m[2] = (xb * yc - xc * yb) / J;
-  m[3] = (yc * za - ya * zc) / J;
-  m[4] = (xa * zc - xc * za) / J;
-  m[5] = (xc * ya - xa * yc) / J;
-  m[6] = (ya * zb - yb * za) / J;
-  m[7] = (xb * za - xa * zb) / J;
-  m[8] = (xa * yb - xb * ya) / J;
-  return true;
+  const auto X = [&](int cc, int di, int dj, int dk) { return B.coord(cc, i + di, j + dj, k + dk); };
+  return metric_point(X, J, m);
 }
+
+}  // namespace
 
 int compute_metrics(HostBlock &B, std::string &err) {
   const int mx = B.mx, my = B.my, mz = B.mz, g = kNgg;
@@ -348,9 +330,8 @@ int compute_metrics(HostBlock &B, std::string &err) {
   return 0;
 }
 
-}  // namespace
-
-int build_mesh(const cfx_block *blocks, int n_block, std::vector<HostBlock> &out, int &dim, std::string &err) {
+int build_mesh(const cfx_block *blocks, int n_block, std::vector<HostBlock> &out, int &dim, std::string &err,
+               bool metrics) {
   if (!blocks || n_block <= 0) {
     err = "no blocks";
     return CFX_E_ARG;
@@ -408,10 +389,11 @@ int build_mesh(const cfx_block *blocks, int n_block, std::vector<HostBlock> &out
   for (auto &B : out) extrapolate_ghost_grid(B, dim);
   inner_ghost_grid(out, blocks, dim);
   for (auto &B : out) corner_ghost_grid(B);
-  for (auto &B : out) {
-    const int rc = compute_metrics(B, err);
-    if (rc) return rc;
-  }
+  if (metrics)
+    for (auto &B : out) {
+      const int rc = compute_metrics(B, err);
+      if (rc) return rc;
+    }
   return 0;
 }
 
