@@ -29,6 +29,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# MI355X vector peaks: FP64 78.6 TFLOP/s; FP32 157.3 TFLOP/s (the mixed RHS computes in FP32)
+VALU_PEAK_TFLOPS = {"fp64": 78.6, "mixed": 157.3}
 # Algorithmic HBM bytes per cell-stage of the FP64 SSP-RK3 stage:
 #   B = 8 * [(8/3) n_c + n_g + 2] with n_c = 14 conserved, n_g = 10 geometric
 BYTES_PER_CELL_STAGE_FP64 = 8.0 * ((8.0 / 3.0) * 14 + 10 + 2)
@@ -176,6 +178,18 @@ def measure_group(case, args, specs, rounds=5):
                 "kernel": "rhs_pencil_kernel (LDS-staged RHS, one launch per face direction)",
                 "kernel_avg_ms_per_stage": round(rhs_stage_s * 1e3, 4),
                 "launches_per_stage": round(rhs_n / stages, 2), "bytes_per_cell_stage": round(bpc, 2)}
+            # the same kernel against the vector ALU: FP64 / FP32 flops per stage
+            # counted with rocprofv3 VALU instruction counters on the default tiles
+            fpath = os.path.join(ROOT, "profiles", f"rhs_c1_{precision}_flops.json")
+            if os.path.exists(fpath) and "tiles" not in info and case.n_cells == 256 * 128 * 128:
+                with open(fpath) as f:
+                    flops = json.load(f)["flops_per_stage"]
+                peak = VALU_PEAK_TFLOPS[precision]
+                tf = flops / rhs_stage_s / 1e12
+                out["roofline_valu"] = {
+                    "bound": "valu", "achieved": round(tf, 3), "peak": peak, "unit": "TFLOP/s",
+                    "frac": round(tf / peak, 5), "flops_per_cell_stage": round(flops / case.n_cells, 1),
+                    "source": os.path.relpath(fpath, ROOT)}
         out["dt"] = g.get_dt()
         out["residual"] = [float(x) for x in g.residual()]
         g.close()
@@ -376,10 +390,12 @@ def main():
                                "AUSM+, viscous + Wilke, SSP-RK3 global CFL 0.5",
                    "cells": case.n_cells, "event_ms_per_step": head["event_ms_per_step"]},
         "roofline": head["roofline"],
+        "roofline_valu": head.get("roofline_valu"),
         "kernel_ms_per_step": head["kernel_ms_per_step"],
         "variants": {
             **{k: {kk: v[kk] for kk in ("value", "ms_per_step", "kernel_ms_per_step") + tuple(
-                x for x in ("rhs_tiles", "mlp") if x in v)} for k, v in forms.items() if k != "fp64"},
+                x for x in ("rhs_tiles", "mlp", "roofline", "roofline_valu") if x in v)} for k, v in forms.items()
+                if k != "fp64"},
             "changed_form_speedup_vs_fp64": {k: round(forms[k]["value"] / head["value"], 3)
                                              for k in forms if k != "fp64"},
             "best_changed_form": best,
