@@ -188,20 +188,40 @@ def grid_path(root, myid=0):
 
 def read_grid(root, myid=0):
     """[(mx, my, mz, x, y, z)] with coordinates flattened i fastest
-    (Mesh.cpp:566-604)."""
-    st = _Stream(grid_path(root, myid))
-    head = st.getline()
-    if head is None:
+    (Mesh.cpp:566-604).  The header lines are read as lines, the coordinates
+    as whitespace-separated numbers by the library's parser."""
+    from . import lib   # the library's multi-threaded number parser
+    path = grid_path(root, myid)
+    try:
+        with open(path, "rb") as f:
+            data = f.read()
+    except OSError as e:
+        raise DeckError(f"cannot read {path}: {e}") from None
+    pos = 0
+
+    def line():
+        nonlocal pos
+        e = data.find(b"\n", pos)
+        e = len(data) if e < 0 else e
+        out = data[pos:e].decode("latin-1")
+        pos = e + 1
+        return out
+
+    head = line().split()
+    if not head:
         raise DeckError("empty grid file")
-    n_block = _int(head.split()[0] if head.split() else "")
+    n_block = _int(head[0])
     dims = []
     for _ in range(n_block):
-        t = (st.getline() or "").split()
+        t = line().split()
         if len(t) < 3:
             raise DeckError("grid file: block extents missing")
         dims.append(tuple(_int(v) for v in t[:3]))
     need = sum(3 * a * b * c for a, b, c in dims)
-    vals = np.fromstring(st.s[st.pos:], dtype=np.float64, sep=" ")   # text mode: parsed in C
+    try:
+        vals = lib.parse_numbers(data[pos:], need)
+    except ValueError as e:
+        raise DeckError(f"grid file: {e}") from None
     if vals.size < need:
         raise DeckError(f"grid file holds {vals.size} coordinates, {need} needed")
     out, o = [], 0

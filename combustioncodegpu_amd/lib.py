@@ -83,6 +83,8 @@ def load():
     lib.cfx_ghost_grid.restype = C.c_int
     lib.cfx_inflow_state.argtypes = [C.POINTER(abi.CfxParams), C.POINTER(abi.CfxSpecies), C.c_int, dp, dp]
     lib.cfx_inflow_state.restype = C.c_int
+    lib.cfx_parse_numbers.argtypes = [C.c_char_p, C.c_longlong, dp, C.c_longlong, C.POINTER(C.c_longlong)]
+    lib.cfx_parse_numbers.restype = C.c_int
     _lib = lib
     return lib
 
@@ -238,3 +240,15 @@ class Solver(StepContext):
             self.lib.cfx_kernel_times(self.h, c, C.byref(ms), C.byref(n))
             out[name] = (ms.value, n.value)
         return out
+
+
+def parse_numbers(text: bytes, n_max: int):
+    """Up to n_max whitespace-separated numbers of text (host only); raises
+    ValueError naming the first token that is not a number."""
+    lib = load()
+    out = np.empty(max(0, n_max), dtype=np.float64)
+    n = C.c_longlong(0)
+    rc = lib.cfx_parse_numbers(text, len(text), abi.dptr(out), n_max, C.byref(n))
+    if rc:
+        raise ValueError(f"token {n.value} is not a number")
+    return out[:n.value]

@@ -181,6 +181,11 @@ int cfx_version(void);
 int cfx_ghost_grid(const cfx_block *blocks, int n_block, double *const *xyz);
 int cfx_inflow_state(const cfx_params *params, const cfx_species *species, int index, double bv[6],
                      double *yk);
+/* cfx_parse_numbers: the whitespace-separated decimal numbers of text[0..len)
+ * as operator>> reads the grid file's coordinates (Mesh.cpp:586-603), up to
+ * n_max of them, into out; *n_out = the count.  A token that is not a whole
+ * number gives CFX_E_ARG with *n_out = its index.  Host only, multi-threaded. */
+int cfx_parse_numbers(const char *text, long long len, double *out, long long n_max, long long *n_out);
 
 /* Driver ctor + Field::setup_device_memory + DParameter + ghost grid and
  * metrics (Driver.cu:71-111, Field.cu:182-240, DParameter.cu:118-159,

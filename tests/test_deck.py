@@ -263,3 +263,20 @@ def test_missing_records_raise(oracle_lib, tmp_path):
     f.write_text("label 2\noption thermal_type = isothermal\nend\n")
     with pytest.raises(deck.DeckError, match="wall label 21"):
         deck.load_case(str(tmp_path))
+
+
+def test_number_parser():
+    """The library's grid-coordinate parser against Python's float() (both
+    correctly rounded): signs, exponents, mixed whitespace, a cap on the
+    count, a malformed token, and a text long enough to be cut into chunks
+    for several threads."""
+    got = lib.parse_numbers(b" 1 +2.5\n-3e-3\t4.0E+02 \r\n.5 ", 10)
+    assert got.tolist() == [1.0, 2.5, -3e-3, 400.0, 0.5]
+    assert lib.parse_numbers(b"1 2 3 4 5", 3).tolist() == [1.0, 2.0, 3.0]
+    with pytest.raises(ValueError, match="token 2"):
+        lib.parse_numbers(b"1 2 1.0D+00 4", 10)
+    rng = np.random.default_rng(5)
+    v = rng.standard_normal(2_000_000) * 10.0 ** rng.integers(-12, 12, 2_000_000)
+    text = " \n".join(f"{x:.17g}" for x in v).encode()
+    assert len(text) > 4 * (1 << 22)
+    assert np.array_equal(lib.parse_numbers(text, v.size), v)
