@@ -38,6 +38,18 @@ def test_ghost_grid_matches_oracle(oracle_lib, make):
         assert np.array_equal(g[b][n], want)
 
 
+def test_oracle_metric_fallback_on_folded_ghosts(oracle_lib):
+    """The grid of the GPU fallback test does fold its ghosts at i = -2: the
+    oracle's metrics there are copies of i = -1 (Mesh.cpp:226-488)."""
+    from tests.test_gpu_parity import folded_ghost_case
+    case = folded_ghost_case()
+    X, Y, Z = lib.ghost_grid(case)[0]
+    dy = (Y[2:-2, 4:-2, :] - Y[2:-2, 2:-4, :]) * 0.5    # d y / d j at j = 1..my-2, interior k
+    assert (dy[:, :, 0] < 0).all() and (dy[:, :, 2:] > 0).all()
+    m = _oracle(case).download(0, abi.METRIC)
+    assert np.array_equal(m[:, 2:-2, 3:-3, 0], m[:, 2:-2, 3:-3, 1]) and (m[9] > 0).all()
+
+
 def test_ghost_grid_cartesian_continues_linearly(oracle_lib):
     """On a uniform Cartesian block the ghost vertices continue the grid."""
     case = cases.c1(9, 8, 7, dx=1e-3)

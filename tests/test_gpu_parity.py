@@ -138,3 +138,28 @@ def test_stage_pieces_parity():
             c.eliminate_k_gradient()
             c.update_properties()
     compare_state(case, o, g, TOL_STEP, "pieces")
+
+
+def folded_ghost_case():
+    """A block whose j-lines spread with i (y = j dy (1 + 0.6 i)): the linear
+    ghost extrapolation folds at i = -2 (J <= 0 there), where the reference
+    copies the metrics of i = -1."""
+    c = cases.c1(12, 10, 8)
+    k, j, i = np.meshgrid(np.arange(8), np.arange(10), np.arange(12), indexing="ij")
+    c.blocks[0].y = np.ascontiguousarray((j * 1e-3 * (1 + 0.6 * i)).ravel(), dtype=np.float64)
+    return c
+
+
+def test_metric_fallback_on_folded_ghosts():
+    """The device metric pass finds the non-positive Jacobians and the host
+    recomputes the block with the fallback: metrics as the oracle's."""
+    from combustioncodegpu_amd.lib import Solver
+    from oracle.pyoracle import Oracle
+    case = folded_ghost_case()
+    print("[parity] folded ghost grid 12x10x8", flush=True)
+    mo = Oracle(case).download(0, abi.METRIC)
+    g = Solver(case)
+    mg = g.download(0, abi.METRIC)
+    g.close()
+    assert np.array_equal(mg[:, 2:-2, 3:-3, 0], mg[:, 2:-2, 3:-3, 1])   # i = -2 took i = -1's values
+    assert_close(mg, mo, 1e-14, "metrics with fallback")
