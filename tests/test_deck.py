@@ -292,3 +292,23 @@ def test_number_parser():
     text = " \n".join(f"{x:.17g}" for x in v).encode()
     assert len(text) > 4 * (1 << 22)
     assert np.array_equal(lib.parse_numbers(text, v.size), v)
+
+
+def test_air_2d_deck(oracle_lib, tmp_path):
+    """The 2-D air workload (inflow, outflow, slip walls; no species files)
+    through the deck: same blocks and conditions, the initial field is the
+    inflow state, and the oracle steps it."""
+    case = cases.c0(mx=24, my=12)
+    deck.write_deck(str(tmp_path), case, init=[(case.inflow[0], None)])
+    assert not (tmp_path / "input_files" / "chemistry").exists()
+    got = deck.load_case(str(tmp_path))
+    _same_blocks(case, got)
+    assert got.species is None and got.n_spec == 0 and got.yk0 == [None]
+    st, _ = lib.inflow_state(got, 0)
+    assert np.array_equal(got.bv0[0], np.broadcast_to(st[:, None, None, None], got.bv0[0].shape))
+    assert abs(st[1] - 2.0 * np.sqrt(1.4 * deck.R_U / deck.MW_AIR * 300.0)) < 1e-9   # Mach 2 at 300 K
+    o = _oracle(got)
+    o.upload_initial()
+    o.init_conservative()
+    o.step(3)
+    assert np.isfinite(o.download(0, abi.CV)).all()
