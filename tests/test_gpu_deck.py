@@ -29,3 +29,17 @@ def test_deck_case_parity(tmp_path, which, steps):
     compare_state(case, o, g, 1e-12 * steps, f"{steps} steps")
     rc, nfail = g.check()
     assert rc == 0 and nfail == 0
+
+
+def test_context_coordinates_match_host_ghost_grid():
+    """CFX_COORD of a context (the ghost grid its metrics were computed from)
+    equals the host-only cfx_ghost_grid, block by block."""
+    import numpy as np
+    from combustioncodegpu_amd import abi, cases, lib
+    case = cases.lattice([[7, 6], [6, 5], [5, 6]])
+    print("[deck] coordinates of a 2x2x2 lattice", flush=True)
+    host = lib.ghost_grid(case)
+    g = lib.Solver(case)
+    for b in range(len(case.blocks)):
+        assert np.array_equal(g.download(b, abi.COORD), host[b])
+    g.close()
