@@ -312,3 +312,20 @@ def test_air_2d_deck(oracle_lib, tmp_path):
     o.init_conservative()
     o.step(3)
     assert np.isfinite(o.download(0, abi.CV)).all()
+
+
+def test_grid_file_errors(tmp_path):
+    with pytest.raises(deck.DeckError, match="cannot read"):
+        deck.read_grid(str(tmp_path))
+    g = tmp_path / "input_files" / "grid"
+    g.mkdir(parents=True)
+    path = g / "grid   0.grd"
+    path.write_text("1\n3 3 1\n" + " ".join(["0.5"] * 26) + "\n")
+    with pytest.raises(deck.DeckError, match="holds 26 coordinates, 27 needed"):
+        deck.read_grid(str(tmp_path))
+    path.write_text("1\n3 3 1\n" + " ".join(["0.5"] * 5 + ["1.0D+00"] + ["0.5"] * 21) + "\n")
+    with pytest.raises(deck.DeckError, match="token 5"):
+        deck.read_grid(str(tmp_path))
+    path.write_text("1\n3 3 1\n" + " ".join(str(float(v)) for v in range(27)) + "\n")
+    (mx, my, mz, x, y, z), = deck.read_grid(str(tmp_path))
+    assert (mx, my, mz) == (3, 3, 1) and x.tolist() == list(range(9)) and z.tolist() == list(range(18, 27))
