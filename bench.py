@@ -78,9 +78,12 @@ def prepare(case, args, name, precision="fp64", batched=True, graph=False, autot
     from combustioncodegpu_amd import abi
     from combustioncodegpu_amd.lib import Solver
     case.params["precision"] = abi.MIXED if precision == "mixed" else abi.FP64
-    g = Solver(case)
+    t0 = time.perf_counter()
+    g = Solver(case)   # cfx_create: ghost grid, device metrics, tables, schedules
+    g.sync()
+    create_ms = 1e3 * (time.perf_counter() - t0)
     case.params["precision"] = abi.FP64
-    info = {}
+    info = {"create_ms": round(create_ms, 1)}
     if mlp:
         log(f"[bench] {name}: training the transport surrogate")
         t0 = time.perf_counter()
@@ -156,6 +159,7 @@ def measure_group(case, args, specs, rounds=5):
                           "blocks": len(case.blocks), "ghost_fill_ops_per_stage": n_ops,
                           "ghost_fill_levels_per_stage": n_lev},
         }
+        out["create_ms"] = info["create_ms"]
         if "tiles" in info:
             out["rhs_tiles"] = info["tiles"]
         if "mlp_report" in info:
@@ -335,7 +339,7 @@ def main():
                         "Wilke vs surrogate transport",
             "cells": c4.n_cells,
             "forms": {k: {kk: v[kk] for kk in ("value", "ms_per_step", "kernel_ms_per_step") + tuple(
-                x for x in ("rhs_tiles", "mlp") if x in v)} for k, v in c4_forms.items()},
+                x for x in ("rhs_tiles", "mlp", "create_ms") if x in v)} for k, v in c4_forms.items()},
             "surrogate_speedup_vs_wilke": round(c4_forms["mixed_graph_tuned_mlp"]["value"]
                                                 / c4_forms["mixed_graph_tuned"]["value"], 3),
             "surrogate_props_ms_vs_wilke": [c4_forms["mixed_graph_tuned_mlp"]["kernel_ms_per_step"]["props"],
@@ -363,7 +367,7 @@ def main():
                         f"(smallest {sizes[0]}, largest {sizes[-1]}), {c3.n_cells} points, 9 species",
             "cells": c3.n_cells,
             "forms": {k: {kk: v[kk] for kk in ("value", "ms_per_step", "kernel_ms_per_step", "launches_per_step",
-                                               "execution") + (("rhs_tiles",) if "rhs_tiles" in v else ())}
+                                               "execution", "create_ms") + (("rhs_tiles",) if "rhs_tiles" in v else ())}
                       for k, v in c3_forms.items()},
             "batched_speedup_vs_per_block_fp64": round(c3_forms["fp64_batched_graph_tuned"]["value"]
                                                        / c3_forms["fp64_per_block"]["value"], 3),
@@ -392,9 +396,11 @@ def main():
         "roofline": head["roofline"],
         "roofline_valu": head.get("roofline_valu"),
         "kernel_ms_per_step": head["kernel_ms_per_step"],
+        "create_ms": head.get("create_ms"),
         "variants": {
             **{k: {kk: v[kk] for kk in ("value", "ms_per_step", "kernel_ms_per_step") + tuple(
-                x for x in ("rhs_tiles", "mlp", "roofline", "roofline_valu") if x in v)} for k, v in forms.items()
+                x for x in ("rhs_tiles", "mlp", "roofline", "roofline_valu", "create_ms") if x in v)}
+                for k, v in forms.items()
                 if k != "fp64"},
             "changed_form_speedup_vs_fp64": {k: round(forms[k]["value"] / head["value"], 3)
                                              for k in forms if k != "fp64"},
