@@ -329,3 +329,14 @@ def test_grid_file_errors(tmp_path):
     path.write_text("1\n3 3 1\n" + " ".join(str(float(v)) for v in range(27)) + "\n")
     (mx, my, mz, x, y, z), = deck.read_grid(str(tmp_path))
     assert (mx, my, mz) == (3, 3, 1) and x.tolist() == list(range(9)) and z.tolist() == list(range(18, 27))
+
+
+def test_grid_scale(oracle_lib, tmp_path):
+    """`double gridScale` multiplies every coordinate (Mesh.cpp:534-536,
+    704-716) unless it is 1 within 1e-15."""
+    case, init = deck_cases.lattice_deck(str(tmp_path), sizes=((6, 5), (5,), (5,)))
+    ctl = tmp_path / "input_files" / "setup" / "0_global_control.txt"
+    ctl.write_text(ctl.read_text().replace("double gridScale = 1.0", "double gridScale = 0.001"))
+    got = deck.load_case(str(tmp_path))
+    for p, q in zip(case.blocks, got.blocks):
+        assert np.array_equal(q.x, np.asarray(p.x) * 0.001) and np.array_equal(q.z, np.asarray(p.z) * 0.001)
