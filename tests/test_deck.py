@@ -340,3 +340,18 @@ def test_grid_scale(oracle_lib, tmp_path):
     got = deck.load_case(str(tmp_path))
     for p, q in zip(case.blocks, got.blocks):
         assert np.array_equal(q.x, np.asarray(p.x) * 0.001) and np.array_equal(q.z, np.asarray(p.z) * 0.001)
+
+
+def test_reference_time_scheme_without_rk_key(oracle_lib, tmp_path):
+    """A deck without the (library-only) rk_stages key runs the reference's
+    explicit Euler with local time steps."""
+    deck_cases.walls_deck(str(tmp_path), box=False)
+    ctl = tmp_path / "input_files" / "setup" / "0_global_control.txt"
+    ctl.write_text("\n".join(l for l in ctl.read_text().splitlines() if "rk_stages" not in l) + "\n")
+    got = deck.load_case(str(tmp_path))
+    assert got.params["rk_stages"] == 1
+    o = _oracle(got)
+    o.upload_initial()
+    o.init_conservative()
+    o.step(2)
+    assert np.isfinite(o.download(0, abi.CV)).all()
