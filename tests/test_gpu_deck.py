@@ -43,3 +43,28 @@ def test_context_coordinates_match_host_ghost_grid():
     for b in range(len(case.blocks)):
         assert np.array_equal(g.download(b, abi.COORD), host[b])
     g.close()
+
+
+def test_deck_case_mixed_graph(tmp_path):
+    """The deck-loaded walls case (two initial groups) in mixed precision,
+    batched and graph-captured, against the FP64 oracle after 5 steps: the
+    mixed-precision error bar (normwise 1e-5 per quantity here)."""
+    from combustioncodegpu_amd import abi
+    from combustioncodegpu_amd.lib import Solver
+    from oracle.pyoracle import Oracle
+    from tests.helpers import assert_close_sel, interior
+    deck_cases.walls_deck(str(tmp_path))
+    ref = deck.load_case(str(tmp_path))
+    mixed = deck.load_case(str(tmp_path), precision=abi.MIXED)
+    print("[deck] walls case, mixed + graph, 5 steps", flush=True)
+    o, g = Oracle(ref), Solver(mixed)
+    g.set_execution(batched=True, graph=True)
+    for c in (o, g):
+        c.upload_initial()
+        c.init_conservative()
+        c.step(5)
+    for sel in (abi.CV, abi.BV):
+        assert_close_sel(sel, interior(g.download(0, sel)), interior(o.download(0, sel)), 1e-5, f"mixed {sel}")
+    rc, nfail = g.check()
+    assert rc == 0 and nfail == 0
+    g.close()
