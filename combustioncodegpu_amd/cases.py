@@ -145,19 +145,33 @@ def padded_index_coords(mx, my, mz):
 
 
 # ------------------------------------------------------------------ species ---
-def synthetic_species(n_spec=9, seed=2024):
-    """n_spec synthetic species: W ~ U[2,44] kg/kmol; NASA-7 with one
-    coefficient set for both ranges (t_low 200, t_mid 1000, t_high 3000 K)
-    such that cp/R in [2.5, 6] on [200, 3000] K (checked on 2801 samples);
-    a5 ~ U[-3e4, 3e4] K; Lennard-Jones sigma ~ U[2.5,4] A, eps/k ~ U[30,600] K."""
+def synthetic_species(n_spec=9, seed=2024, two_range=True, t_low=200.0):
+    """n_spec synthetic species: W ~ U[2,44] kg/kmol; NASA-7 with t_low
+    (200 K by default), t_mid 1000 K, t_high 3000 K and cp/R in [2.5, 6] on
+    [200, 3000] K (checked on 2801 samples of each set); a5 ~ U[-3e4, 3e4] K;
+    Lennard-Jones sigma ~ U[2.5,4] A, eps/k ~ U[30,600] K.
+
+    two_range=True gives the high set its own coefficients, as real CHEMKIN
+    data has: cp_hi(T) = cp_lo(T) + (T - t_mid)(b1 + b2 (T - t_mid)) with
+    seeded b1, b2, and a5 of the high set chosen so that h is continuous at
+    t_mid (cp is continuous there too, its slope is not).  two_range=False
+    uses the low set for both ranges."""
     u = uniform(seed, 1, 8 * n_spec)
     mw = 2.0 + 42.0 * u[:n_spec]
     sigma = 2.5 + 1.5 * u[n_spec:2 * n_spec]
     eps_k = 30.0 + 570.0 * u[2 * n_spec:3 * n_spec]
     a5 = -3e4 + 6e4 * u[3 * n_spec:4 * n_spec]
-    T0, D = 200.0, 2800.0
+    T0, D, TM = 200.0, 2800.0, 1000.0
     tt = np.linspace(200.0, 3000.0, 2801)
+
+    def cp_of(a, t):
+        return a[0] + t * (a[1] + t * (a[2] + t * (a[3] + t * a[4])))
+
+    def h_of(a, t):  # h/R
+        return t * (a[0] + t * (a[1] / 2 + t * (a[2] / 3 + t * (a[3] / 4 + t * a[4] / 5)))) + a[5]
+
     coeff = np.zeros((n_spec, 7))
+    coeff_hi = np.zeros((n_spec, 7))
     for l in range(n_spec):
         attempt = 0
         while True:
@@ -176,9 +190,26 @@ def synthetic_species(n_spec=9, seed=2024):
         coeff[l, :5] = a
         coeff[l, 5] = a5[l]
         coeff[l, 6] = 0.0
+        coeff_hi[l] = coeff[l]
+        if two_range:
+            attempt = 0
+            while True:
+                r = uniform(seed, 500 + 31 * l + attempt, 2)
+                b1 = -6e-4 + 1.2e-3 * r[0]     # per K
+                b2 = -2e-7 + 4e-7 * r[1]       # per K^2
+                hi = coeff[l].copy()
+                # (T - TM)(b1 + b2 (T - TM)) in powers of T
+                hi[0] += -b1 * TM + b2 * TM * TM
+                hi[1] += b1 - 2 * b2 * TM
+                hi[2] += b2
+                hi[5] += h_of(coeff[l], TM) - h_of(hi, TM)
+                if 2.5 <= cp_of(hi, tt).min() and cp_of(hi, tt).max() <= 6.0:
+                    break
+                attempt += 1
+            coeff_hi[l] = hi
     return {
-        "mw": mw, "t_low": np.full(n_spec, 200.0), "t_mid": np.full(n_spec, 1000.0),
-        "t_high": np.full(n_spec, 3000.0), "coeff_lo": coeff.copy(), "coeff_hi": coeff.copy(),
+        "mw": mw, "t_low": np.full(n_spec, float(t_low)), "t_mid": np.full(n_spec, TM),
+        "t_high": np.full(n_spec, 3000.0), "coeff_lo": coeff, "coeff_hi": coeff_hi,
         "eps_k": eps_k, "sigma": sigma,
     }
 
@@ -290,16 +321,16 @@ def _smooth_fields(seed, n_spec, sp, idx, L):
 
 
 def c1(mx=256, my=128, mz=128, n_spec=9, seed=11, dx=1e-3, perturb=0.0, cfl=0.5, rk_stages=3, limiter=1,
-       bc="outflow", species_seed=2024, viscous=2, quirks=abi.QUIRKS_REFERENCE):
+       bc="outflow", species_seed=2024, viscous=2, quirks=abi.QUIRKS_REFERENCE, t_low=200.0, reconstruction=2):
     """3-D block with n_spec synthetic species and smooth seeded fields:
     T = 1400 + 1100 S_T, p = 101325 (1 + 0.1 S_p), u_d = 200/sqrt(3) S_ud,
     Y smoothed Dirichlet(1); outflow on all faces."""
-    sp = synthetic_species(n_spec, species_seed) if n_spec > 0 else None
+    sp = synthetic_species(n_spec, species_seed, t_low=t_low) if n_spec > 0 else None
     x, y, z = cartesian_grid(mx, my, mz, dx, perturb=perturb, seed=seed)
     L = (max(mx - 1, 1), max(my - 1, 1), max(mz - 1, 1))
     bv, Y = _smooth_fields(seed, n_spec, sp, padded_index_coords(mx, my, mz), L)
     blk = BlockSpec(mx, my, mz, x, y, z, bc=outflow_all_faces(mx, my, mz))
-    params = dict(inviscid_scheme=3, reconstruction=2, limiter=limiter, viscous_order=viscous,
+    params = dict(inviscid_scheme=3, reconstruction=reconstruction, limiter=limiter, viscous_order=viscous,
                   rk_stages=rk_stages, cfl=cfl, prandtl=0.72, schmidt=0.9, quirks=quirks)
     return Case("c1", params, sp, [blk], [], [], [np.ascontiguousarray(bv)],
                 [None if Y is None else np.ascontiguousarray(Y)])
@@ -390,13 +421,14 @@ def lattice(sizes, n_spec=9, seed=11, dx=1e-3, cfl=0.5, rk_stages=3, limiter=1, 
     return Case(name, params, sp, blocks, [], [], bvs, yks)
 
 
-def c1_walls(mx=24, my=20, mz=16, n_spec=9, seed=11, dx=1e-3, rk_stages=3, limiter=1):
+def c1_walls(mx=24, my=20, mz=16, n_spec=9, seed=11, dx=1e-3, rk_stages=3, limiter=1, t_low=500.0):
     """The C1 fields and species on one block with every boundary kind:
     inflow at i-min (T = 1400 K, p = 101325 Pa, 150 m/s along x, the mean
     composition of the field), outflow at i-max, an isothermal no-slip wall
     (600 K) at j-min, an adiabatic no-slip wall at j-max, slip walls at k-min
-    and k-max."""
-    c = c1(mx, my, mz, n_spec=n_spec, seed=seed, dx=dx, rk_stages=rk_stages, limiter=limiter)
+    and k-max.  t_low defaults to 500 K so that the field (300-2500 K) also
+    reaches the NASA-7 extension below t_low."""
+    c = c1(mx, my, mz, n_spec=n_spec, seed=seed, dx=dx, rk_stages=rk_stages, limiter=limiter, t_low=t_low)
     blk = c.blocks[0]
     X, Y, Z = mx - 1, my - 1, mz - 1
     blk.bc = [

@@ -1556,4 +1556,17 @@ int orc_inviscid_face(const cfx_species *sp, const cfx_params *prm, const double
   return 0;
 }
 
+int orc_face_flux(orc_ctx *C, int blk, int dir, int i, int j, int k, int pass, double *out) {
+  if (!C || blk < 0 || blk >= (int)C->blocks.size() || dir < 0 || dir >= C->dim || !out) return CFX_E_ARG;
+  const Block &B = C->blocks[blk];
+  const int m[3] = {B.mx, B.my, B.mz}, p[3] = {i, j, k};
+  for (int q = 0; q < 3; ++q) {
+    const int lo = q == dir ? -1 : 0;
+    if (q < C->dim && (p[q] < lo || p[q] >= m[q])) return CFX_E_ARG;
+  }
+  if (pass == 0) inviscid_face_at(*C, B, dir, B.id(i, j, k), out);
+  else viscous_face_at(*C, B, dir, B.id(i, j, k), out);
+  return 0;
+}
+
 }  // extern "C"

@@ -66,6 +66,12 @@ int orc_inviscid_face(const cfx_species *sp, const cfx_params *prm,
                       const double *pv, const double *metric, const double *jac,
                       double *flux);
 
+/* The face flux the step differences (inviscid_face_at / viscous_face_at):
+ * the face between interior point (i,j,k) and (i,j,k) + e_dir of block blk
+ * (i may be -1 along dir), pass 0 = AUSM+ flux, 1 = viscous flux; out has
+ * 5+ns entries.  Uses the context's current bv, yk, metric and properties. */
+int orc_face_flux(orc_ctx *ctx, int blk, int dir, int i, int j, int k, int pass, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,6 +46,8 @@ def lib():
         _lib.orc_sutherland.argtypes = [C.c_double]
         _lib.orc_sutherland.restype = C.c_double
         _lib.orc_inviscid_face.argtypes = [C.POINTER(abi.CfxSpecies), C.POINTER(abi.CfxParams), dp, dp, dp, dp]
+        _lib.orc_face_flux.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp]
+        _lib.orc_face_flux.restype = C.c_int
         fp = C.POINTER(C.c_float)
         _lib.orc_mlp_forward.argtypes = [fp, C.c_int, C.c_int, fp, C.c_int, C.c_int, dp]
         _lib.orc_mlp_forward.restype = C.c_int
@@ -64,6 +66,14 @@ class Oracle(StepContext):
 
     def newton_failures(self):
         return self.lib.orc_newton_failures(self.h)
+
+    def face_flux(self, blk, d, i, j, k, viscous):
+        """Face flux between interior point (i,j,k) and (i,j,k)+e_d."""
+        out = np.zeros(5 + self.case.n_spec)
+        rc = self.lib.orc_face_flux(self.h, blk, d, i, j, k, 1 if viscous else 0, abi.dptr(out))
+        if rc:
+            raise RuntimeError(f"orc_face_flux: {rc}")
+        return out
 
 
 class SpeciesHolder:

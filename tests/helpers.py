@@ -6,8 +6,9 @@ The scale is per physical quantity, not per array row:
     are scaled by the largest vector magnitude; a component that is nearly
     zero (the y-momentum of the 2-D air case is ~1e-9 of |rho V|) is a
     cancellation of O(|rho V|) fluxes and carries round-off of that size;
-  - partial densities rho*Y_l are scaled by max rho, mass fractions by 1;
-  - every other quantity by its own maximum magnitude.
+  - mass fractions are scaled by 1;
+  - every other quantity, partial densities rho*Y_l and their dq included,
+    by its own maximum magnitude.
 """
 import numpy as np
 
@@ -26,13 +27,10 @@ def scales(sel, ref):
         vmag = np.sqrt((r[1:4] ** 2).sum(axis=0)).max() if sel == abi.BV else np.sqrt(
             (r[1:4] ** 2).sum(axis=0)).max()
         s[1:4] = vmag
-        if sel == abi.CV:
-            s[5:] = mag[0]
     if sel == abi.YK:
         s[:] = 1.0
     if sel == abi.DQ and r.shape[0] >= 4:
         s[1:4] = mag[1:4].max()
-        s[5:] = np.maximum(mag[5:], mag[0]) if r.shape[0] > 5 else s[5:]
     return np.where(s == 0, 1.0, s)
 
 

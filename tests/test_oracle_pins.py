@@ -1,0 +1,420 @@
+"""Closed-form pins of the oracle functions that the invariants of
+test_oracle.py leave open (no GPU).
+
+Each check states the result the mathematics fixes and computes it
+independently of the oracle's code path (analytic gradients instead of
+finite differences, physical spacings and rotation angles instead of
+metric terms, NASA-7 polynomials evaluated here from the raw coefficients),
+so that a dropped term, a wrong sign or index, or a transposed metric
+operand fails:
+
+  central viscous flux           quadratic u, v, w, T, Y on a Cartesian
+  (ViscousScheme.cu:11-342)      grid with three different spacings: the
+                                 2-point normal and 4-point tangential
+                                 differences are exact at the face centre,
+                                 so every face flux equals the Stokes stress
+                                 (lambda = -2/3 mu) dotted with the face area,
+                                 the Fourier and species-enthalpy heat flux,
+                                 and the Fick flux minus the correction
+                                 velocity term, in all three directions and
+                                 with either averaging index of rho*D
+  local time step                cfl / (sum |u.e_d|/D_d + c sum 1/D_d
+  (TemporalScheme.cu:10-43)      + max(gamma, 4/3) mu/(rho Pr) sum 1/D_d^2)
+                                 on a rotated Cartesian grid, 3-D
+                                 multispecies and 2-D air
+  wall / outflow / slip ghosts   the identities of BoundCond.cu:455-487 and
+  (BoundCond.cu:455-629)         :529-629 on the block with every BC kind
+  total energy, cv from bv       constant-cp species: rho E = rho |V|^2 / 2
+  (Thermo.cu:66-86,              + sum rho Y_l R_l (a0_l T + a5_l) - p;
+   Field.cu:150-176)             air: rho |V|^2 / 2 + p / (gamma - 1)
+  gamma, c, kappa of the air     gamma = cp / (cp - R), c = sqrt(gamma R T);
+  path (Field.cu:178-210)        air kappa = mu gamma R / ((gamma - 1) Pr)
+  2-D k copy (Field.cu:268-292)  every k plane of bv and yk equals k = 0
+  NASA-7 range select            the high set above t_mid and the low set
+  (Thermo.cu:7-63)               below it, from the raw polynomials
+"""
+import math
+
+import numpy as np
+import pytest
+
+from combustioncodegpu_amd import abi, cases
+
+RU = 8314.462618
+NGG = 2
+
+
+def nasa7(sp, t):
+    """h [J/kg] and cp [J/(kg K)] of every species at scalar t from the raw
+    coefficients: low set below t_mid, high set from t_mid on, and below
+    t_low the low set's cp(t_low) frozen with h extended linearly."""
+    ns = len(sp["mw"])
+    h = np.empty(ns)
+    cp = np.empty(ns)
+    for l in range(ns):
+        r = RU / sp["mw"][l]
+        tl = sp["t_low"][l]
+        te = max(t, tl)
+        a = sp["coeff_hi"][l] if te >= sp["t_mid"][l] else sp["coeff_lo"][l]
+        c = sum(a[n] * te ** n for n in range(5))
+        hh = sum(a[n] * te ** (n + 1) / (n + 1) for n in range(5)) + a[5]
+        cp[l] = r * c
+        h[l] = r * hh + (r * c * (t - tl) if t < tl else 0.0)
+    return h, cp
+
+
+# ----------------------------------------------------------- NASA-7 select ---
+def test_nasa7_range_select(oracle_lib):
+    sp = cases.synthetic_species(9, 2024)
+    assert np.abs(sp["coeff_hi"] - sp["coeff_lo"]).max() > 1e-6  # two distinct sets
+    o = oracle_lib
+    for t in (250.0, 600.0, 999.0, 1000.0, 1001.0, 1800.0, 2900.0):
+        h, cp = o.enthalpy_cp(sp, t)
+        he, cpe = nasa7(sp, t)
+        np.testing.assert_allclose(cp, cpe, rtol=1e-13)
+        np.testing.assert_allclose(h, he, rtol=1e-11, atol=1e-9 * np.abs(he).max())
+    # the two sets differ away from t_mid, so a swapped select would fail above
+    lo = dict(sp, coeff_hi=sp["coeff_lo"])
+    _, cp_lo = o.enthalpy_cp(lo, 2500.0)
+    _, cp = o.enthalpy_cp(sp, 2500.0)
+    assert np.abs(cp - cp_lo).max() > 1e-2 * np.abs(cp).max()
+    # continuity at t_mid (the generator's construction) and below t_low
+    spl = cases.synthetic_species(9, 2024, t_low=500.0)
+    h1, cp1 = o.enthalpy_cp(spl, 1000.0 - 1e-9)
+    h2, cp2 = o.enthalpy_cp(spl, 1000.0)
+    np.testing.assert_allclose(h1, h2, rtol=1e-9, atol=1e-6)
+    np.testing.assert_allclose(cp1, cp2, rtol=1e-9)
+    h, cp = o.enthalpy_cp(spl, 350.0)
+    he, cpe = nasa7(spl, 350.0)
+    np.testing.assert_allclose(cp, cpe, rtol=1e-13)
+    np.testing.assert_allclose(h, he, rtol=1e-11, atol=1e-9 * np.abs(he).max())
+
+
+# ---------------------------------------------------------- quadratic fields ---
+class Quad:
+    """f(x) = c0 + g.x + x^T Q x with symmetric Q: value and exact gradient."""
+
+    def __init__(self, c0, g, q):
+        self.c0 = float(c0)
+        self.g = np.asarray(g, dtype=float)
+        q = np.asarray(q, dtype=float)
+        self.q = 0.5 * (q + q.T)
+
+    def __call__(self, X):  # X: (..., 3)
+        return self.c0 + X @ self.g + np.einsum("...i,ij,...j->...", X, self.q, X)
+
+    def grad(self, X):
+        return self.g + 2.0 * X @ self.q
+
+
+def cartesian(mx, my, mz, d, theta=0.0):
+    """Vertex and padded coordinates of a Cartesian block with spacings d,
+    rotated by theta about z; padded point (i,j,k) of the ghost grid lies on
+    the same lattice (the ghost grid of a Cartesian block extends it)."""
+    c, s = math.cos(theta), math.sin(theta)
+
+    def xyz(i, j, k):
+        x0, y0, z0 = i * d[0], j * d[1], k * d[2]
+        return np.stack([c * x0 - s * y0, s * x0 + c * y0, z0], axis=-1)
+
+    k, j, i = np.meshgrid(np.arange(mz), np.arange(my), np.arange(mx), indexing="ij")
+    v = xyz(i, j, k)
+    pk, pj, pi = np.meshgrid(np.arange(-NGG, mz + NGG), np.arange(-NGG, my + NGG), np.arange(-NGG, mx + NGG),
+                             indexing="ij")
+    P = xyz(pi, pj, pk)
+    return [np.ascontiguousarray(v[..., a]).ravel() for a in range(3)], P
+
+
+def three_species():
+    sp = cases.synthetic_species(3, 77)
+    return sp
+
+
+def quad_case(quirks, d=(1.0e-3, 1.6e-3, 0.7e-3), shape=(9, 8, 7)):
+    mx, my, mz = shape
+    c = cases.c1(mx, my, mz, n_spec=3, quirks=quirks)
+    c.species = three_species()
+    xs, P = cartesian(mx, my, mz, d)
+    b = c.blocks[0]
+    b.x, b.y, b.z = xs
+    L = np.array([mx * d[0], my * d[1], mz * d[2]])
+    # coefficients for coordinates normalised by the block size, scaled below
+    fU = Quad(60.0, [90.0, -50.0, 30.0], [[40.0, 25.0, -10.0], [25.0, -30.0, 15.0], [-10.0, 15.0, 20.0]])
+    fV = Quad(-20.0, [-40.0, 70.0, 10.0], [[-15.0, 10.0, 30.0], [10.0, 25.0, -20.0], [30.0, -20.0, 10.0]])
+    fW = Quad(35.0, [20.0, 30.0, -80.0], [[10.0, -25.0, 5.0], [-25.0, 15.0, 35.0], [5.0, 35.0, -30.0]])
+    fT = Quad(1050.0, [400.0, -300.0, 250.0], [[150.0, 80.0, -60.0], [80.0, -120.0, 90.0], [-60.0, 90.0, 110.0]])
+    fY0 = Quad(0.30, [0.05, -0.03, 0.02], [[0.02, 0.01, 0.0], [0.01, -0.02, 0.01], [0.0, 0.01, 0.015]])
+    fY1 = Quad(0.25, [-0.02, 0.04, -0.03], [[-0.01, 0.02, 0.01], [0.02, 0.01, -0.01], [0.01, -0.01, 0.02]])
+    def phys(f):
+        return Quad(f.c0, f.g / L, f.q / np.outer(L, L))
+    fU, fV, fW, fT, fY0, fY1 = (phys(f) for f in (fU, fV, fW, fT, fY0, fY1))
+    fY2 = Quad(1.0 - fY0.c0 - fY1.c0, -fY0.g - fY1.g, -fY0.q - fY1.q)
+    fields = dict(u=fU, v=fV, w=fW, T=fT, Y=[fY0, fY1, fY2])
+    bv = np.empty((6,) + P.shape[:3])
+    bv[1], bv[2], bv[3], bv[5] = fU(P), fV(P), fW(P), fT(P)
+    Y = np.stack([f(P) for f in fields["Y"]])
+    assert Y.min() > 0.05 and bv[5].min() > 300.0
+    bv[4] = 101325.0
+    rmix = RU * np.einsum("l...,l->...", Y, 1.0 / c.species["mw"])
+    bv[0] = bv[4] / (rmix * bv[5])
+    c.bv0 = [np.ascontiguousarray(bv)]
+    c.yk0 = [np.ascontiguousarray(Y)]
+    return c, P, fields, np.array(d)
+
+
+@pytest.mark.parametrize("quirks", [0, abi.QUIRKS_REFERENCE])
+def test_viscous_flux_closed_form(oracle_lib, quirks):
+    from oracle.pyoracle import Oracle
+    c, P, F, d = quad_case(quirks)
+    o = Oracle(c)
+    o.upload_initial()
+    o.update_properties()  # the uploaded quadratic field everywhere, ghosts included
+    props = o.download(0, abi.PROPS)
+    mu, kap, rhoD = props[2], props[3], props[4:]
+    sp = c.species
+    B = c.blocks[0]
+    ext = (B.mx, B.my, B.mz)
+    J = d.prod()
+    area = np.array([J / d[0], J / d[1], J / d[2]])  # |J grad xi_d| of a Cartesian cell
+    for dr in range(3):
+        S = np.zeros(3)
+        S[dr] = area[dr]
+        e = np.zeros(3, dtype=int)
+        e[dr] = 1
+        worst = np.zeros(8)
+        got_all, exp_all = [], []
+        rng = np.random.default_rng(dr)
+        faces = [(i, j, k) for k in range(ext[2]) for j in range(ext[1]) for i in range(ext[0])]
+        faces = [f for f in faces if f[dr] < ext[dr] - 1] + [tuple(f[q] - e[q] for q in range(3))
+                                                           for f in faces if f[dr] == 0]
+        for (i, j, k) in [faces[n] for n in rng.choice(len(faces), 60, replace=False)]:
+            pl = (k + NGG, j + NGG, i + NGG)
+            pr = (k + NGG + e[2], j + NGG + e[1], i + NGG + e[0])
+            xl, xr = P[pl], P[pr]
+            xm = 0.5 * (xl + xr)
+            gu, gv, gw, gT = (F[n].grad(xm) for n in ("u", "v", "w", "T"))
+            gY = [f.grad(xm) for f in F["Y"]]
+            G = np.array([gu, gv, gw])  # G[a, b] = d u_a / d x_b
+            mu_f = 0.5 * (mu[pl] + mu[pr])
+            kap_f = 0.5 * (kap[pl] + kap[pr])
+            pD = pr
+            if dr == 2 and (quirks & abi.Q_HV_RHOD_J):
+                pD = (pl[0], pl[1] + 1, pl[2])
+            D_f = np.array([0.5 * (rhoD[l][pl] + rhoD[l][pD]) for l in range(3)])
+            tau = mu_f * (G + G.T - (2.0 / 3.0) * np.trace(G) * np.eye(3))
+            mom = S @ tau
+            vm = 0.5 * np.array([F[n](xl) + F[n](xr) for n in ("u", "v", "w")])
+            Tm = 0.5 * (F["T"](xl) + F["T"](xr))
+            Ym = np.array([0.5 * (f(xl) + f(xr)) for f in F["Y"]])
+            h, _ = nasa7(sp, Tm)
+            fick = np.array([D_f[l] * (S @ gY[l]) for l in range(3)])
+            corr = sum(fick)
+            energy = vm @ mom + kap_f * (S @ gT) + sum(h[l] * fick[l] for l in range(3))
+            exp = np.concatenate([[0.0], mom, [energy], fick - Ym * corr])
+            got = o.face_flux(0, dr, i, j, k, viscous=True)
+            got_all.append(got)
+            exp_all.append(exp)
+        got_all, exp_all = np.array(got_all), np.array(exp_all)
+        scale = np.abs(exp_all).max(axis=0)
+        scale[1:4] = scale[1:4].max()
+        scale[5:] = scale[5:].max()
+        err = np.abs(got_all - exp_all).max(axis=0) / np.where(scale == 0, 1, scale)
+        assert got_all[:, 0].max() == 0.0
+        assert err.max() <= 1e-10, (dr, err)
+
+
+# ------------------------------------------------------------- local dt ---
+def _dt_expected(bv, props, d, theta, cfl, pr, gamma=None, dim=3):
+    c_, s_ = math.cos(theta), math.sin(theta)
+    e = [np.array([c_, s_, 0.0]), np.array([-s_, c_, 0.0]), np.array([0.0, 0.0, 1.0])]
+    vel = np.stack([bv[1], bv[2], bv[3]], axis=-1)
+    lam = np.zeros(bv.shape[1:])
+    for a in range(dim):
+        lam += np.abs(vel @ e[a]) / d[a] + props[1] / d[a]
+    g = props[0] if gamma is None else gamma
+    vis = np.maximum(g, 4.0 / 3.0) * props[2] / (bv[0] * pr) * sum(1.0 / d[a] ** 2 for a in range(dim))
+    return cfl / (lam + vis)
+
+
+def test_local_dt_closed_form_3d(oracle_lib):
+    from oracle.pyoracle import Oracle
+    d, theta = (1.0e-3, 1.7e-3, 0.6e-3), 0.37
+    c = cases.c1(10, 9, 8, n_spec=3, cfl=0.7, rk_stages=1)
+    c.blocks[0].x, c.blocks[0].y, c.blocks[0].z = cartesian(10, 9, 8, d, theta)[0]
+    o = Oracle(c)
+    o.upload_initial()
+    o.init_conservative()
+    o.time_step(False)
+    got = o.download(0, abi.DTLOCAL)
+    bv = o.download(0, abi.BV)[:, 2:-2, 2:-2, 2:-2]
+    props = o.download(0, abi.PROPS)[:, 2:-2, 2:-2, 2:-2]
+    exp = _dt_expected(bv, props, d, theta, 0.7, 0.72)
+    np.testing.assert_allclose(got, exp, rtol=1e-12)
+
+
+def test_local_dt_closed_form_2d_air(oracle_lib):
+    from oracle.pyoracle import Oracle
+    d, theta = (1.0e-3, 0.55e-3, 1.0e-3), -0.8
+    c = cases.c0(20, 14, cfl=0.45, rk_stages=1)
+    c.blocks[0].x, c.blocks[0].y, c.blocks[0].z = cartesian(20, 14, 1, d, theta)[0]
+    c.bv0[0][2] = 35.0 * np.sin(np.arange(c.bv0[0].shape[3]) / 3.0)[None, None, :]
+    o = Oracle(c)
+    o.upload_initial()
+    o.init_conservative()
+    o.time_step(False)
+    got = o.download(0, abi.DTLOCAL)[0]
+    bv = o.download(0, abi.BV)[:, 2:3, 2:-2, 2:-2][:, 0]
+    props = o.download(0, abi.PROPS)[:, 2:3, 2:-2, 2:-2][:, 0]
+    exp = _dt_expected(bv, props, d, theta, 0.45, 0.72, gamma=1.4, dim=2)
+    np.testing.assert_allclose(got, exp, rtol=1e-12)
+
+
+# ------------------------------------------------------- ghost identities ---
+def test_wall_outflow_slip_ghost_identities(oracle_lib):
+    """c1_walls: isothermal wall (600 K) at j-min, adiabatic wall at j-max,
+    outflow at i-max, slip walls at k-min / k-max, applied in the order
+    outflow, inflow, slip, wall.  Rows that a later condition rewrites are
+    left out of each check."""
+    from oracle.pyoracle import Oracle
+    c = cases.c1_walls(14, 12, 10, n_spec=3)
+    o = Oracle(c)
+    o.upload_initial()
+    o.init_conservative()
+    bv, yk, cv = o.download(0, abi.BV), o.download(0, abi.YK), o.download(0, abi.CV)
+    W = c.species["mw"]
+    mx, my, mz = 14, 12, 10
+    J0, K0 = NGG, NGG  # padded index of j = 0, k = 0
+    ii = slice(NGG, NGG + mx)  # wall range: the face's own vertex range
+    kk = slice(NGG, NGG + mz)
+
+    def mw(y):
+        return 1.0 / np.einsum("l...,l->...", y, 1.0 / W)
+
+    for (jw, sgn, tw) in ((J0, -1, 600.0), (J0 + my - 1, +1, None)):
+        jin = jw - sgn
+        p_in = bv[4][kk, jin, ii]
+        t_wall = bv[5][kk, jin, ii] if tw is None else np.full_like(p_in, tw)
+        # wall point (BoundCond.cu:553-575)
+        np.testing.assert_array_equal(yk[:, kk, jw, ii], yk[:, kk, jin, ii])
+        np.testing.assert_allclose(bv[0][kk, jw, ii], p_in * mw(yk[:, kk, jin, ii]) / (t_wall * RU), rtol=1e-15)
+        assert np.all(bv[1:4][:, kk, jw, ii] == 0) and np.all(cv[1:4][:, kk, jw, ii] == 0)
+        np.testing.assert_array_equal(bv[4][kk, jw, ii], p_in)
+        np.testing.assert_array_equal(bv[5][kk, jw, ii], t_wall)
+        for g in (1, 2):
+            jg, ji = jw + sgn * g, jw - sgn * g
+            ti = bv[5][kk, ji, ii]
+            if tw is None:
+                tg = ti
+            else:
+                tg = 2 * tw - ti
+                tg = np.where(tg <= 0.1 * tw, tw, tg)
+            np.testing.assert_array_equal(yk[:, kk, jg, ii], yk[:, kk, ji, ii])
+            np.testing.assert_array_equal(bv[1:4][:, kk, jg, ii], -bv[1:4][:, kk, ji, ii])
+            np.testing.assert_array_equal(bv[4][kk, jg, ii], bv[4][kk, ji, ii])
+            np.testing.assert_allclose(bv[5][kk, jg, ii], tg, rtol=1e-15)
+            rho_g = bv[4][kk, ji, ii] * mw(yk[:, kk, ji, ii]) / (tg * RU)
+            np.testing.assert_allclose(bv[0][kk, jg, ii], rho_g, rtol=1e-14)
+            np.testing.assert_allclose(cv[1:4][:, kk, jg, ii], -rho_g * bv[1:4][:, kk, ji, ii], rtol=1e-14)
+            np.testing.assert_allclose(cv[5:][:, kk, jg, ii], rho_g * yk[:, kk, ji, ii], rtol=1e-14)
+        if tw is not None:  # both branches of the 0.1 T_w clamp are present
+            t1 = bv[5][kk, jw - sgn, ii]
+            assert np.any(2 * tw - t1 <= 0.1 * tw) and np.any(2 * tw - t1 > 0.1 * tw)
+    # outflow at i-max: every ghost layer equals the face layer (away from
+    # the rows the walls and slip walls rewrite afterwards)
+    jj, kq = slice(NGG + 1, NGG + my - 1), slice(NGG + 1, NGG + mz - 1)
+    xf = NGG + mx - 1
+    for g in (1, 2):
+        for arr in (bv, yk, cv):
+            np.testing.assert_array_equal(arr[:, kq, jj, xf + g], arr[:, kq, jj, xf])
+    # slip wall at k-min on a Cartesian grid: normal = z; ghosts mirror the
+    # interior with w negated, the face point loses w
+    ir = slice(NGG + 1, NGG + mx - 1)
+    assert np.all(bv[3][K0, jj, ir] == 0)
+    for g in (1, 2):
+        a, b_ = bv[:, K0 - g, jj, ir], bv[:, K0 + g, jj, ir]
+        np.testing.assert_array_equal(a[[0, 1, 2, 4, 5]], b_[[0, 1, 2, 4, 5]])
+        np.testing.assert_allclose(a[3], -b_[3], rtol=0, atol=1e-12 * np.abs(bv[3]).max())
+
+
+# ------------------------------------------------- energy, gamma, c, kappa ---
+def test_constant_cp_energy_gamma_c(oracle_lib):
+    """Two species with constant cp (a1..a4 = 0), T crossing t_mid and
+    t_low: h_l = R_l (a0_l T + a5_l) on both sides of both."""
+    from oracle.pyoracle import Oracle
+    c = cases.c1(9, 8, 7, n_spec=2)
+    a_lo = np.zeros((2, 7))
+    a_lo[0, 0], a_lo[0, 5] = 3.5, -1200.0
+    a_lo[1, 0], a_lo[1, 5] = 4.6, 2500.0
+    c.species = {"mw": np.array([28.0, 4.0]), "t_low": np.array([700.0, 600.0]),
+                 "t_mid": np.array([1000.0, 1500.0]), "t_high": np.array([3000.0, 3000.0]),
+                 "coeff_lo": a_lo, "coeff_hi": a_lo.copy(), "eps_k": np.array([98.0, 10.2]),
+                 "sigma": np.array([3.6, 2.6])}
+    Y = c.yk0[0][:2] / c.yk0[0][:2].sum(axis=0)
+    c.yk0[0] = np.ascontiguousarray(Y)
+    bv = c.bv0[0]
+    t = bv[5]
+    bv[5] = 400.0 + 1700.0 * (t - t.min()) / (t.max() - t.min())
+    bv[0] = bv[4] / (RU * np.einsum("l...,l->...", Y, 1.0 / c.species["mw"]) * bv[5])
+    o = Oracle(c)
+    o.upload_initial()
+    o.init_conservative()
+    bv, yk, cv = o.download(0, abi.BV), o.download(0, abi.YK), o.download(0, abi.CV)
+    props = o.download(0, abi.PROPS)
+    T = bv[5]
+    assert T.min() < 600 and T.max() > 1500
+    R = RU / c.species["mw"]
+    rhoE = 0.5 * bv[0] * (bv[1] ** 2 + bv[2] ** 2 + bv[3] ** 2) - bv[4]
+    for l in range(2):
+        rhoE = rhoE + bv[0] * yk[l] * R[l] * (a_lo[l, 0] * T + a_lo[l, 5])
+    np.testing.assert_allclose(cv[4], rhoE, rtol=1e-12, atol=1e-12 * np.abs(rhoE).max())
+    np.testing.assert_allclose(cv[0], bv[0], rtol=1e-15)
+    for a in range(3):
+        np.testing.assert_allclose(cv[1 + a], bv[0] * bv[1 + a], rtol=1e-15)
+    for l in range(2):
+        np.testing.assert_allclose(cv[5 + l], bv[0] * yk[l], rtol=1e-15)
+    cp = sum(yk[l] * R[l] * a_lo[l, 0] for l in range(2))
+    rmix = sum(yk[l] * R[l] for l in range(2))
+    gam = cp / (cp - rmix)
+    np.testing.assert_allclose(props[0], gam, rtol=1e-13)
+    np.testing.assert_allclose(props[1], np.sqrt(gam * rmix * T), rtol=1e-13)
+
+
+def test_air_energy_gamma_c_kappa(oracle_lib):
+    from oracle.pyoracle import Oracle
+    c = cases.c0(16, 12)
+    c.bv0[0][2] = 25.0 * np.cos(np.arange(c.bv0[0].shape[2]) / 2.0)[None, :, None]
+    o = Oracle(c)
+    o.upload_initial()
+    o.init_conservative()
+    bv, cv, props = o.download(0, abi.BV), o.download(0, abi.CV), o.download(0, abi.PROPS)
+    g, R, Pr = 1.4, 287.0, 0.72
+    rhoE = 0.5 * bv[0] * (bv[1] ** 2 + bv[2] ** 2 + bv[3] ** 2) + bv[4] / (g - 1)
+    np.testing.assert_allclose(cv[4], rhoE, rtol=1e-13)
+    np.testing.assert_allclose(props[1], np.sqrt(g * R * bv[5]), rtol=1e-14)
+    mu = 1.716e-5 * (bv[5] / 273) ** 1.5 * 384 / (bv[5] + 111)
+    np.testing.assert_allclose(props[2], mu, rtol=1e-13)
+    np.testing.assert_allclose(props[3], mu * g * R / ((g - 1) * Pr), rtol=1e-13)
+
+
+# ------------------------------------------------------------- 2-D k copy ---
+@pytest.mark.parametrize("n_spec", [0, 2])
+def test_eliminate_k_gradient_copies_plane(oracle_lib, n_spec):
+    from oracle.pyoracle import Oracle
+    c = cases.c0(16, 12) if n_spec == 0 else cases.c1(16, 12, 1, n_spec=2)
+    rng = np.random.default_rng(5)
+    c.bv0[0] = np.ascontiguousarray(c.bv0[0] * (1 + 0.01 * rng.standard_normal(c.bv0[0].shape)))
+    if n_spec:
+        y = c.yk0[0] * (1 + 0.05 * rng.random(c.yk0[0].shape))
+        c.yk0[0] = np.ascontiguousarray(y / y.sum(axis=0))
+    o = Oracle(c)
+    o.upload_initial()
+    before = o.download(0, abi.BV)
+    assert np.abs(before[:, 0] - before[:, 2]).max() > 0  # the planes differ before
+    o.eliminate_k_gradient()
+    bv = o.download(0, abi.BV)
+    for k in range(5):
+        np.testing.assert_array_equal(bv[:, k], before[:, 2])
+    if n_spec:
+        yk = o.download(0, abi.YK)
+        for k in range(5):
+            np.testing.assert_array_equal(yk[:, k], c.yk0[0][:, 2])
