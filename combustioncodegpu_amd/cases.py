@@ -1,12 +1,12 @@
 """Seeded synthetic workloads (grids, species, initial fields, BCs).
 
-The reference initialises from input decks (Initialize.cu:9-110,
-Field.cu:118-169) that are not shipped; these generators produce the same
+The reference initialises from input decks (Initialize.cu:9-111,
+Field.cu:26-77) that are not shipped; these generators produce the same
 kind of data deterministically so that the HIP path and the CPU oracle see
 identical bytes.  Random numbers come from a counter-based SplitMix64 hash
 (Steele, Lea & Flood 2014), which is easy to reproduce bit-exactly in any
 language.  Every field is filled over the ghost-padded range, as the
-reference's initialisation does (Field.cu:139-168).
+reference's initialisation does (Field.cu:48-75).
 
 Workloads:
   c0   2-D 128x64 air, Mach-2 inflow with a Gaussian density bump,
@@ -29,7 +29,7 @@ import numpy as np
 
 from . import abi
 
-R_U = 8314.462618  # Constants.h:64
+R_U = 8314.462618  # Constants.h:19
 NGG = 2
 
 _GOLDEN = np.uint64(0x9E3779B97F4A7C15)

@@ -1,5 +1,5 @@
 // cfx_api.cpp — the C ABI of include/cfx.h: context, device memory, the
-// reference's Driver step loop (Driver.cu:167-246) over the HIP kernels.
+// reference's Driver step loop (Driver.cu:113-192) over the HIP kernels.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -94,7 +94,7 @@ int build_species(const cfx_species *sp, SpeciesDev &tables, std::string &err) {
       }
       s.hc[r][l][5] = a[5] * rs;
     }
-    // enthalpy and cp at t_low with the low set (Thermo.cu:29-36)
+    // enthalpy and cp at t_low with the low set (Thermo.cu:30-37)
     const double tt = sp->t_low[l];
     const double *c = s.cpc[0][l], *e = s.hc[0][l];
     s.cp_tlow[l] = c[0] + tt * (c[1] + tt * (c[2] + tt * (c[3] + tt * c[4])));
@@ -157,7 +157,7 @@ int build_inflow(const cfx_inflow &in, const SpeciesT<double> &s, const SchemeDe
   out.st[3] = in.w * vel;
   out.st[4] = p;
   out.st[5] = T;
-  // total energy (Thermo.cu:85-105)
+  // total energy (Thermo.cu:66-86)
   double e = 0.5 * rho * (out.st[1] * out.st[1] + out.st[2] * out.st[2] + out.st[3] * out.st[3]);
   if (ns > 0) {
     for (int l = 0; l < ns; ++l) {
@@ -562,7 +562,7 @@ int do_store_last(cfx_ctx *ctx) {
   return 0;
 }
 
-// Driver::steady_simulation loop body (Driver.cu:189-244): with rk_stages 1
+// Driver::steady_simulation loop body (Driver.cu:135-190): with rk_stages 1
 // the reference's explicit Euler with local dt; with 3 the SSP-RK3 stages
 // with the global CFL dt taken at stage 0.
 int do_step(cfx_ctx *ctx) {

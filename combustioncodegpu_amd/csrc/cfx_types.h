@@ -12,8 +12,8 @@
 
 namespace cfx {
 
-constexpr double kRu = 8314.462618;  // universal gas constant, J/(kmol K) (Constants.h:64)
-constexpr double kMwAir = 28.9647;   // Constants.h:73
+constexpr double kRu = 8314.462618;  // universal gas constant, J/(kmol K) (Constants.h:19)
+constexpr double kMwAir = 28.9647;   // Constants.h:28
 constexpr int kNgg = 2;              // ghost layers for MUSCL (Parameter.cpp:32)
 constexpr int kMaxSp = CFX_MAX_SPEC;
 
@@ -22,7 +22,7 @@ constexpr int kMaxSp = CFX_MAX_SPEC;
 //   cp_l(T) = sum_k cpc[r][l][k] T^k,   h_l(T) = T*sum_k hc[r][l][k] T^k + hc[r][l][5]
 // with r = 0 for T < t_mid (low set) and r = 1 otherwise, already scaled by
 // R_u/W_l; below t_low the enthalpy is extended linearly with the frozen cp
-// (h_tlow, cp_tlow), as Thermo.cu:29-36 does.  Held in the arithmetic type
+// (h_tlow, cp_tlow), as Thermo.cu:30-37 does.  Held in the arithmetic type
 // of the kernel that reads it, so that every constant is a scalar operand.
 // One species' NASA-7 thermo as one contiguous, 64-byte aligned record of
 // 32 values, read by the point loops as a single vector (two or four wide
@@ -46,7 +46,7 @@ struct SpeciesT {
   T hc[2][kMaxSp][6];
   T h_tlow[kMaxSp], cp_tlow[kMaxSp];
   T lj_inv[kMaxSp], log_lj_inv[kMaxSp], vis_coeff[kMaxSp];
-  T cond_add[kMaxSp];        // 1.25 R_u / W_l (Transport.cu:222)
+  T cond_add[kMaxSp];        // 1.25 R_u / W_l (Transport.cu:59)
   T wj4[kMaxSp][kMaxSp];     // (W_n/W_m)^(1/4)           (ChemData.cpp:257)
   T sqw8[kMaxSp][kMaxSp];    // 1/sqrt(8(1+W_m/W_n))      (ChemData.cpp:258)
 };

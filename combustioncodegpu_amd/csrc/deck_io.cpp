@@ -1,6 +1,6 @@
 // deck_io.cpp — host-side text parsing for the reference's input decks: the
 // grid file is whitespace-separated decimal coordinates read with
-// operator>> (Mesh.cpp:586-603); a 256x128x128 block is 12.6M numbers.  The
+// operator>> (Mesh.cpp:587-607); a 256x128x128 block is 12.6M numbers.  The
 // text is cut at whitespace into one chunk per thread; each thread counts its
 // tokens, then parses them with std::from_chars (correctly rounded, like
 // strtod) into its slice of the output.

@@ -71,7 +71,7 @@ template <>
 __device__ __forceinline__ float flog<float>(float a) { return __builtin_amdgcn_logf(a) * 0.6931471805599453f; }
 
 // ---------------------------------------------------------------- thermo ---
-// h_l(T) and cp_l(T) of one species (Thermo.cu:46-66): NASA-7 polynomial in
+// h_l(T) and cp_l(T) of one species (Thermo.cu:27-47): NASA-7 polynomial in
 // Horner form with the low/high set chosen at t_mid and the linear enthalpy
 // extension with frozen cp below t_low.  The coefficients are wave-uniform
 // (one record per species); T varies per lane.  Both ranges are evaluated
@@ -125,7 +125,7 @@ __device__ __forceinline__ R species_h(SpeciesPtrT<R> sp, int l, R t) {
   return t < tl ? R(c[kThHtlow]) + R(c[kThCptlow]) * (t - tl) : hp;
 }
 
-// cp_l(T) only, frozen below t_low (Thermo.cu:68-82).  Loops over species
+// cp_l(T) only, frozen below t_low (Thermo.cu:49-63).  Loops over species
 // that need only cp read the per-array layout: the cp coefficients of all
 // species are contiguous there and arrive in a few wide loads.
 template <typename R>
@@ -156,7 +156,7 @@ struct ThermoShared {
 };
 
 // -------------------------------------------------------------- limiters ---
-// minmod (Limiter.cuh:135-138) and Van Albada (tag 1)
+// minmod (Limiter.cuh:13-16) and Van Albada (tag 1)
 template <typename R>
 __device__ __forceinline__ R limiter(int tag, R a, R b) {
   if (tag == 1) {
@@ -167,7 +167,7 @@ __device__ __forceinline__ R limiter(int tag, R a, R b) {
   return (a > R(0) && b > R(0)) ? m : ((a < R(0) && b < R(0)) ? -m : R(0));
 }
 
-// MUSCL kappa = 1/3 on one variable (Reconstruction.cu:59-77), or first
+// MUSCL kappa = 1/3 on one variable (Reconstruction.cu:26-44), or first
 // order.  Both limiters are symmetric, lim(a,b) == lim(b,a) bit for bit, so
 // the reference's four calls per face reduce to the two limited slopes of
 // the two cells: (1-kappa)/4 + (1+kappa)/4 = 1/2.
@@ -184,10 +184,10 @@ __device__ __forceinline__ void muscl(const SchemeDev &S, R q0, R q1, R q2, R q3
   }
 }
 
-// AUSM+ flux (InviscidScheme.cu:235-304) from the reconstructed face states
+// AUSM+ flux (InviscidScheme.cu:39-106) from the reconstructed face states
 // pl/pr = (rho,u,v,w,p,Y..) and the face area vector k = 0.5(J_L grad_L +
 // J_R grad_R) of the face direction; the face temperature, energy and gamma
-// follow Reconstruction.cu:79-122.
+// follow Reconstruction.cu:46-89.
 template <int NS, typename R, typename TH>
 __device__ __forceinline__ void ausm_from_states(SpeciesPtrT<R> sp, const TH &th, const SchemeDev &S, const R *pl,
                                                  const R *pr, R k1, R k2, R k3, R *F) {

@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(kTpb) dt_kernel(const BlockView *__restrict__ 
 
 __global__ void set_u64_kernel(unsigned long long *dst, unsigned long long v) { *dst = v; }
 
-// store_last_step (TimeAdvanceFunc.cu:59-70)
+// store_last_step (TimeAdvanceFunc.cu:13-24)
 __global__ void __launch_bounds__(kTpb) store_last_kernel(const BlockView *__restrict__ views,
                                                           const int *__restrict__ off, int nb) {
   const int w = blockIdx.x + off[0];
@@ -113,8 +113,8 @@ __global__ void __launch_bounds__(kTpb) store_last_kernel(const BlockView *__res
   }
 }
 
-// compute_square_of_dbv + reduction_of_dbv_squared (TimeAdvanceFunc.cu:338-349,
-// Driver.cu:383-420): one partial sum per workgroup, summed in a fixed order.
+// compute_square_of_dbv (TimeAdvanceFunc.cu:292-303) + reduction_of_dbv_squared
+// (Driver.cu:329-366): one partial sum per workgroup, summed in a fixed order.
 __global__ void __launch_bounds__(kTpb) residual_partial_kernel(const BlockView *__restrict__ views,
                                                                 const int *__restrict__ off, int nb,
                                                                 double *partial) {
@@ -171,7 +171,7 @@ __global__ void __launch_bounds__(kTpb) residual_final_kernel(const double *part
   }
 }
 
-// eliminate_k_gradient (Field.cu:360-384), 2-D only
+// eliminate_k_gradient (Field.cu:268-292), 2-D only
 __global__ void eliminate_k_kernel(BlockView view, int ns) {
   const GView B = gview(view);
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) - kNgg;

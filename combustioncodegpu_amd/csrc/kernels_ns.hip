@@ -61,7 +61,7 @@ __device__ __forceinline__ PointWork point_work(const int *__restrict__ off, int
 }
 
 // ------------------------------------------------------------ properties ---
-// update_physical_properties (Field.cu:270-302): mixture cp, gamma and speed
+// update_physical_properties (Field.cu:178-210): mixture cp, gamma and speed
 // of sound at temperature t and mass fractions Y; cp_l and sum Y_l/W_l are
 // returned for the transport.
 template <typename R>
@@ -80,14 +80,14 @@ __device__ __forceinline__ void mixture_gamma_c(SpeciesPtrT<R> sp, R t, const R 
   out[1] = fsqrt<R>(gam * Ru * t * winv);
 }
 
-// compute_transport_property (Transport.cu:206-259): out = mu, kappa,
+// compute_transport_property (Transport.cu:43-96): out = mu, kappa,
 // rho*D_1..ns.  The Wilke sums use s_l = sqrt(mu_l) so that sqrt(mu_m/mu_n)
 // = s_m/s_n (NS square roots instead of NS^2) and T*^-0.145 =
 // exp(-0.145 log T*) with one shared log T.
 template <typename R>
 __device__ __forceinline__ void wilke_point(SpeciesPtrT<R> sp, const SchemeDev &S, R t, const R *Y, const R *cp,
                                             R winv, R *out) {
-  // Chapman-Enskog species viscosity (Transport.cu:215-220)
+  // Chapman-Enskog species viscosity (Transport.cu:52-57)
   const R lt = flog<R>(t), st = fsqrt<R>(t);
   R mu_s[NSA], X[NSA], s[NSA], rs[NSA];
   const R imw = frcp<R>(winv);
@@ -101,7 +101,7 @@ __device__ __forceinline__ void wilke_point(SpeciesPtrT<R> sp, const SchemeDev &
     s[l] = fsqrt<R>(mu_s[l]);
     rs[l] = frcp<R>(s[l]);
   }
-  // Wilke mixing and Mason-Saxena conductivity (Transport.cu:221-246)
+  // Wilke mixing and Mason-Saxena conductivity (Transport.cu:58-83)
   R mu = 0, kappa = 0;
 #pragma unroll
   for (int m = 0; m < NS; ++m) {
@@ -118,7 +118,7 @@ __device__ __forceinline__ void wilke_point(SpeciesPtrT<R> sp, const SchemeDev &
   }
   out[0] = mu;
   out[1] = kappa;
-  // constant-Schmidt diffusivity (Transport.cu:249-255, X[l] in the test)
+  // constant-Schmidt diffusivity (Transport.cu:85-92, X[l] in the test)
   const R mus = mu * R(S.inv_sc);
 #pragma unroll
   for (int l = 0; l < NS; ++l)
@@ -245,7 +245,7 @@ void register_rhs(Set &k) {
 
 #if CFX_PREC == 64
 // ---------------------------------------------------------- total energy ---
-// compute_total_energy (Thermo.cu:85-105) from bv/yk/cv at padded point p.
+// compute_total_energy (Thermo.cu:66-86) from bv/yk/cv at padded point p.
 __device__ __forceinline__ void total_energy(const GView &B, SpeciesPtrT<double> sp, const SchemeDev &S,
                                              long long p) {
   const long long np = B.np;
@@ -267,9 +267,9 @@ __device__ __forceinline__ void total_energy(const GView &B, SpeciesPtrT<double>
 }
 
 // --------------------------------------------------------------- update ---
-// update_cv_and_bv (TimeAdvanceFunc.cu:301-336) with the RK stage
+// update_cv_and_bv (TimeAdvanceFunc.cu:255-290) with the RK stage
 // combination q = a*qn + b*(q + dt*dq/J) and the Newton inversion of T
-// (Thermo.cu:108-144, stop |dT| <= 1e-10 K unless CFX_Q_NEWTON_REL).
+// (Thermo.cu:89-125, stop |dT| <= 1e-10 K unless CFX_Q_NEWTON_REL).
 // MIXED selects the FP32 dq.  The loads are issued before the first store:
 // the stores go through pointers the compiler must assume alias the loaded
 // arrays.
@@ -398,7 +398,7 @@ __global__ void __launch_bounds__(kTpb) update_kernel(const BlockView *__restric
   }
 }
 
-// compute_cv_from_bv (Field.cu:242-268) over the padded range
+// compute_cv_from_bv (Field.cu:150-176) over the padded range
 __global__ void __launch_bounds__(kTpb) cv_from_bv_kernel(const BlockView *__restrict__ views,
                                                           const int *__restrict__ off, int nb,
                                                           const SpeciesDev *__restrict__ spg, SchemeDev S) {
@@ -580,7 +580,7 @@ __device__ __forceinline__ void bc_symmetry_point(const GView &B, const FaceRang
   }
 }
 
-// inner_communication (Field.cu:305-358) with the face loop of
+// inner_communication (Field.cu:213-266) with the face loop of
 // Mesh.cpp:905-917 at loop point n: shared-face points are averaged between
 // the two blocks, ghost layers take the neighbour's values.
 __device__ __forceinline__ void exchange_point(const GView &B, const GView &T, const InnerDev &f,

@@ -2,7 +2,7 @@
 // (T, log p, Y_1..Y_ns) to (log mu, log kappa, log rho*D_1..ns), normalised
 // on both sides, evaluated with BF16 MFMA (v_mfma_f32_16x16x32_bf16) and
 // FP32 accumulation.  No counterpart in the reference, whose transport is
-// Transport.cu:206-259; that routine generates the training samples and is
+// Transport.cu:43-96; that routine generates the training samples and is
 // the fallback.
 //
 // Layout.  A wavefront evaluates 16 points per MFMA in the transposed form

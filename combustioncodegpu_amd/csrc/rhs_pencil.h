@@ -6,7 +6,7 @@
 // order).  Consecutive faces along a share a cell, so TA faces difference
 // into TA-1 cells and neighbouring tiles overlap by one face.
 //
-// Inviscid phase (InviscidScheme.cu:220-304 via Reconstruction.cu:45-122):
+// Inviscid phase (InviscidScheme.cu:24-108 via Reconstruction.cu:12-89):
 // the 5+NS reconstructed variables are staged over the box (TA+3) x TB x TC
 // (the 4-point MUSCL stencil along a), KI variables per batch.  Batches go
 // straight into two alternating LDS buffers with global-to-LDS loads, the

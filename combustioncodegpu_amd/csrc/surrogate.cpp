@@ -1,7 +1,7 @@
 // surrogate.cpp — the learned transport surrogate on the host side: training
 // set generation, normalisation, training envelope, training on the GPU,
 // validation, loading and inspection.  No counterpart in the reference; its
-// transport routine (Transport.cu:206-259) produces the samples.
+// transport routine (Transport.cu:43-96) produces the samples.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

@@ -4,13 +4,13 @@ A reference case directory holds ``input_files/``:
 
   setup/0_global_control.txt ... 9_transport_property.txt
       typed keys, one per line: ``int cfl_steps = 3``  (Parameter.cpp:41-71)
-  setup/5_boundary_condition.txt   ``wall name label`` lines (BoundCond.cu:316-392)
-  setup/6_inflow.txt, 7_wall.txt   conditions per label (BoundCond.cu:127-205,
+  setup/5_boundary_condition.txt   ``wall name label`` lines (BoundCond.cu:315-409)
+  setup/6_inflow.txt, 7_wall.txt   conditions per label (BoundCond.cu:107-207,
                                    BoundCond.cpp:10-141)
-  setup/8_initialization.txt       inflow groups and their boxes (Initialize.cu:24-104)
-  grid/grid   0.grd                blocks and vertex coordinates (Mesh.cpp:566-604;
+  setup/8_initialization.txt       inflow groups and their boxes (Initialize.cu:25-111)
+  grid/grid   0.grd                blocks and vertex coordinates (Mesh.cpp:566-614;
                                    ``grid{:>4}`` pads the rank with spaces)
-  boundary_condition/boundary   0.txt, inner   0.txt   (Mesh.cpp:606-670)
+  boundary_condition/boundary   0.txt, inner   0.txt   (Mesh.cpp:616-670)
   <mechanism_file>, <therm_file>, <transport_file>      CHEMKIN-style
                                    (ChemData.cpp:10-61,106-261)
 
@@ -59,7 +59,7 @@ ELECTRON_MASS = 9.1093837015e-31
 
 # Standard atomic weights (IUPAC 2021 abridged values) of the elements a
 # combustion mechanism names, by upper-case symbol; D and E as isotope /
-# electron entries (Element.cpp:8-147 holds the same published values).
+# electron entries (Element.cpp:13-145 holds the same published values).
 ATOMIC_WEIGHT = {
     "H": 1.008, "HE": 4.002602, "LI": 6.94, "BE": 9.0121831, "B": 10.81, "C": 12.011, "N": 14.007,
     "O": 15.999, "F": 18.998403163, "NE": 20.1797, "NA": 22.98976928, "MG": 24.305, "AL": 26.9815384,
@@ -149,7 +149,7 @@ def _setup(root, name):
 # -------------------------------------------------------------- parameters ---
 def read_parameters(root):
     """Typed keys of the six setup files, first definition wins (std::map
-    emplace), plus the derived ngg and n_var (Parameter.cpp:26-71)."""
+    emplace), plus the derived ngg and n_var (Parameter.cpp:25-71)."""
     par = {}
     for name in SETUP_FILES:
         path = _setup(root, name)
@@ -188,7 +188,7 @@ def grid_path(root, myid=0):
 
 def read_grid(root, myid=0):
     """[(mx, my, mz, x, y, z)] with coordinates flattened i fastest
-    (Mesh.cpp:566-604).  The header lines are read as lines, the coordinates
+    (Mesh.cpp:566-614).  The header lines are read as lines, the coordinates
     as whitespace-separated numbers by the library's parser."""
     from . import lib   # the library's multi-threaded number parser
     path = grid_path(root, myid)
@@ -235,7 +235,7 @@ def read_grid(root, myid=0):
 
 def read_boundary(root, n_block, myid=0):
     """Per block the boundary lines (x1, x2, y1, y2, z1, z2, label)
-    (Mesh.cpp:606-626)."""
+    (Mesh.cpp:616-636)."""
     st = _Stream(os.path.join(root, "input_files", "boundary_condition", f"boundary{myid:>4}.txt"))
     out = []
     for _ in range(n_block):
@@ -255,7 +255,7 @@ def read_boundary(root, n_block, myid=0):
 def read_inner(root, n_block, myid=0):
     """Per block the inner faces (src[6], tar[6], target block 1-based):
     a header line, n_block face counts, the rest of that line and one more
-    line, then two lines of 7 integers per face (Mesh.cpp:628-670)."""
+    line, then two lines of 7 integers per face (Mesh.cpp:638-670)."""
     path = os.path.join(root, "input_files", "boundary_condition", f"inner{myid:>4}.txt")
     if not os.path.exists(path):
         return [[] for _ in range(n_block)]
@@ -420,7 +420,7 @@ def read_tran(path, species):
 def read_bc_types(root, grid_labels):
     """Labels per condition kind: the named lines of 5_boundary_condition.txt
     for labels the boundary files use, then the defaults 2 (wall), 5 (inflow)
-    and 6 (outflow) for labels left over (BoundCond.cu:316-392)."""
+    and 6 (outflow) for labels left over (BoundCond.cu:315-409)."""
     left = list(grid_labels)
     kinds = {"wall": [], "inflow": [], "outflow": []}
     path = _setup(root, "5_boundary_condition.txt")
@@ -453,7 +453,7 @@ def read_bc_types(root, grid_labels):
 
 def _inflow_record(st, species):
     """Keys of one inflow record up to the next ``label`` / ``end`` line
-    (BoundCond.cpp:10-41); returns (record, the line that ended it)."""
+    (BoundCond.cpp:10-34); returns (record, the line that ended it)."""
     par, spec = {}, {}
     while True:
         line = st.getline("lower")
@@ -527,7 +527,7 @@ def read_wall_records(root):
 
 def read_initialization(root):
     """Inflow labels of the initialisation groups and, for groups after the
-    first, their boxes ((xs, xe), (ys, ye), (zs, ze)) (Initialize.cu:24-104)."""
+    first, their boxes ((xs, xe), (ys, ye), (zs, ze)) (Initialize.cu:25-111)."""
     st = _Stream(_setup(root, "8_initialization.txt"))
     n_group = 1
     while True:
@@ -577,7 +577,7 @@ def load_case(root, myid=0, precision=abi.FP64, transport=abi.TRANSPORT_WILKE, q
         species.update(eps_k=eps, sigma=sig, names=names)
     grid = read_grid(root, myid)
     scale = par.get("gridScale", 1.0)
-    if abs(scale - 1.0) > 1e-15:   # Mesh.cpp:534-536, 704-716
+    if abs(scale - 1.0) > 1e-15:   # Mesh.cpp:537-543, 704-716
         grid = [(mx, my, mz, x * scale, y * scale, z * scale) for mx, my, mz, x, y, z in grid]
     n_block = len(grid)
     bnd = read_boundary(root, n_block, myid)

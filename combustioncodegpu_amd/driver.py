@@ -2,12 +2,12 @@
 
 `StepContext` wraps one context of a library that exports the step API of
 include/cfx.h under a symbol prefix (``cfx_`` for the HIP library).  Its
-methods are the pieces of Driver::steady_simulation (Driver.cu:167-246):
+methods are the pieces of Driver::steady_simulation (Driver.cu:113-192):
 store_last_step, rhs, time_step, update, apply_bc, exchange,
 eliminate_k_gradient, update_properties and residual, plus `step` for the
 whole loop body, so callers read like the reference's own step loop.
 
-`Driver` adds the reference's residual bookkeeping (Driver.cu:273-331): the
+`Driver` adds the reference's residual bookkeeping (Driver.cu:219-277): the
 four RMS residuals are normalised by their value at the first output step
 and the maximum is reported.
 """

@@ -3,9 +3,9 @@
  * compressible multispecies Navier–Stokes solver.
  *
  * The reference has no C ABI: its step path sits behind host free functions
- * launched from Driver::steady_simulation (Driver.cu:167-246) and device
+ * launched from Driver::steady_simulation (Driver.cu:113-192) and device
  * "scheme" objects selected by integer tags in setup_schemes
- * (Driver.cu:347-381).  Each entry point below names the reference routine(s)
+ * (Driver.cu:293-327).  Each entry point below names the reference routine(s)
  * it replaces.  A caller keeps the reference's Driver loop shape: create a
  * context from Parameter/Mesh/ChemData-equivalent descriptors, upload the
  * initial primitive field, then call the step pieces (or cfx_step) per
@@ -57,7 +57,7 @@ extern "C" {
 /* quirk bits: 1 = reproduce the reference text, 0 = corrected form */
 #define CFX_Q_METRIC_AVG   0x1 /* Q5: fv/hv average only xi_x over the face (ViscousScheme.cu:16-24,238-246) */
 #define CFX_Q_HV_RHOD_J    0x2 /* Q6: hv averages rho*D with (i,j+1,k) (ViscousScheme.cu:317) */
-#define CFX_Q_NEWTON_REL   0x4 /* Q8: Newton stops at |1-T1/T|<=1e-3 (Thermo.cu:123,138) instead of |dT|<=1e-10 K */
+#define CFX_Q_NEWTON_REL   0x4 /* Q8: Newton stops at |1-T1/T|<=1e-3 (Thermo.cu:104,119) instead of |dT|<=1e-10 K */
 #define CFX_QUIRKS_REFERENCE (CFX_Q_METRIC_AVG | CFX_Q_HV_RHOD_J)
 
 /* precision modes */
@@ -83,7 +83,7 @@ extern "C" {
 #define CFX_COORD  7 /* 3 x padded: x, y, z vertex coordinates with the ghost grid */
 
 /* One physical boundary face, as a line of the reference's boundary file
- * (Mesh.cpp:606-615, Boundary ctor Mesh.cpp:10-12): 0-based vertex ranges,
+ * (Mesh.cpp:616-636, Boundary ctor Mesh.cpp:10-12): 0-based vertex ranges,
  * start == end in the face-normal direction. */
 typedef struct {
   int range_start[3];
@@ -93,7 +93,7 @@ typedef struct {
 } cfx_bc;
 
 /* One block-to-block interface, as two lines of the reference's inner file
- * (Mesh.cpp:640-665, InnerFace ctor Mesh.cpp:34-42): 1-based, signed ranges
+ * (Mesh.cpp:649-664, InnerFace ctor Mesh.cpp:34-42): 1-based, signed ranges
  * in the Plot3D/Gridgen convention, target block 1-based. */
 typedef struct {
   int src[6];       /* x1 x2 y1 y2 z1 z2 of this block            */
@@ -176,19 +176,19 @@ int cfx_version(void);
  * layout above.  Blocks are described as for cfx_create.
  * cfx_inflow_state: the primitive state (rho, u, v, w, p, T) and mass
  * fractions of inflow record `index` as the reference derives it
- * (Inflow::register_boundary_condition, BoundCond.cpp:10-105; var_info,
+ * (Inflow::register_boundary_condition, BoundCond.cpp:10-106; var_info,
  * BoundCond.cpp:108-110); yk may be NULL. */
 int cfx_ghost_grid(const cfx_block *blocks, int n_block, double *const *xyz);
 int cfx_inflow_state(const cfx_params *params, const cfx_species *species, int index, double bv[6],
                      double *yk);
 /* cfx_parse_numbers: the whitespace-separated decimal numbers of text[0..len)
- * as operator>> reads the grid file's coordinates (Mesh.cpp:586-603), up to
+ * as operator>> reads the grid file's coordinates (Mesh.cpp:587-607), up to
  * n_max of them, into out; *n_out = the count.  A token that is not a whole
  * number gives CFX_E_ARG with *n_out = its index.  Host only, multi-threaded. */
 int cfx_parse_numbers(const char *text, long long len, double *out, long long n_max, long long *n_out);
 
 /* Driver ctor + Field::setup_device_memory + DParameter + ghost grid and
- * metrics (Driver.cu:71-111, Field.cu:182-240, DParameter.cu:118-159,
+ * metrics (Driver.cu:17-57, Field.cu:90-148, DParameter.cu:4-45,
  * Mesh.cpp:718-872 and 226-488). */
 int cfx_create(const cfx_params *params, const cfx_species *species,
                const cfx_block *blocks, int n_block, cfx_ctx **out);
@@ -196,7 +196,7 @@ void cfx_destroy(cfx_ctx *ctx);
 const char *cfx_last_error(const cfx_ctx *ctx);
 
 /* Upload the primitive state of one block over the padded range (the
- * reference fills ghosts too: Field.cu:139-168). bv is 6 x padded, yk is
+ * reference fills ghosts too: Field.cu:48-75). bv is 6 x padded, yk is
  * n_spec x padded (ignored on the air path). */
 int cfx_upload_primitive(cfx_ctx *ctx, int blk, const double *bv, const double *yk);
 /* Upload the conservative field directly (checkpoint restore), n_var x padded,
@@ -204,37 +204,37 @@ int cfx_upload_primitive(cfx_ctx *ctx, int blk, const double *bv, const double *
 int cfx_upload_state(cfx_ctx *ctx, int blk, const double *cv, const double *bv, const double *yk);
 int cfx_download(cfx_ctx *ctx, int blk, int what, double *dst);
 
-/* Driver::initialize_computation (Driver.cu:113-151): cv from bv, boundary
+/* Driver::initialize_computation (Driver.cu:59-97): cv from bv, boundary
  * conditions, inter-block exchange, physical properties. */
 int cfx_init_conservative(cfx_ctx *ctx);
 
-/* store_last_step (TimeAdvanceFunc.cu:59-70) for every block. */
+/* store_last_step (TimeAdvanceFunc.cu:13-24) for every block. */
 int cfx_store_last_step(cfx_ctx *ctx);
-/* compute_inviscid_flux + compute_viscous_flux (TimeAdvanceFunc.cu:84-289)
+/* compute_inviscid_flux + compute_viscous_flux (TimeAdvanceFunc.cu:38-243)
  * for one block: dq = -(dF_inv) + (dF_visc), overwritten (set_dq_to_0 is
  * folded in).  blk < 0 means every block. */
 int cfx_rhs(cfx_ctx *ctx, int blk);
-/* local_time_step (TimeAdvanceFunc.cu:291-299, TemporalScheme.cu:10-43) for
+/* local_time_step (TimeAdvanceFunc.cu:245-253, TemporalScheme.cu:10-43) for
  * every block; global != 0 also reduces min(dt_local) over all blocks into the
  * device-resident dt used by cfx_update.  dt_out (may be NULL) receives it. */
 int cfx_time_step(cfx_ctx *ctx, int global, double *dt_out);
-/* update_cv_and_bv (TimeAdvanceFunc.cu:301-336) for every block.  With
+/* update_cv_and_bv (TimeAdvanceFunc.cu:255-290) for every block.  With
  * rk_stages == 1 the local dt is used (reference); with rk_stages == 3,
  * stage = 0,1,2 applies the SSP-RK3 combination with the device dt. */
 int cfx_update(cfx_ctx *ctx, int stage);
 /* DBoundCond::apply_boundary_conditions (BoundCond.cu:251-313) + slip wall. */
 int cfx_apply_bc(cfx_ctx *ctx);
-/* Driver::data_communication (Driver.cu:248-271), inner faces on device. */
+/* Driver::data_communication (Driver.cu:194-217), inner faces on device. */
 int cfx_exchange(cfx_ctx *ctx);
-/* eliminate_k_gradient (Field.cu:360-384), 2-D only. */
+/* eliminate_k_gradient (Field.cu:268-292), 2-D only. */
 int cfx_eliminate_k_gradient(cfx_ctx *ctx);
-/* update_physical_properties (Field.cu:270-302) over the padded range. */
+/* update_physical_properties (Field.cu:178-210) over the padded range. */
 int cfx_update_properties(cfx_ctx *ctx);
-/* compute_residual (Driver.cu:273-331) without the normalisation:
+/* compute_residual (Driver.cu:219-277) without the normalisation:
  * res[0..3] = sqrt(sum over all blocks of (d rho, d|V|, d p, d T)^2 / n_grid_total). */
 int cfx_residual(cfx_ctx *ctx, double res[4]);
-/* The whole step loop body of Driver::steady_simulation (Driver.cu:189-244),
- * n_steps times, without residual output. */
+/* The loop body (Driver.cu:135-190) of Driver::steady_simulation
+ * (Driver.cu:113-192), n_steps times, without residual output. */
 int cfx_step(cfx_ctx *ctx, int n_steps);
 /* Read back the device dt of the last global reduction. */
 int cfx_get_dt(cfx_ctx *ctx, double *dt);
@@ -259,7 +259,7 @@ int cfx_kernel_times(cfx_ctx *ctx, int cls, double *ms, long long *launches);
 int cfx_set_profiling(cfx_ctx *ctx, int on);
 
 /* Execution of the step (no counterpart in the reference, whose Driver
- * launches every kernel block by block, Driver.cu:189-244).
+ * launches every kernel block by block, Driver.cu:135-190).
  *   batched = 0: one launch per block and kernel, one launch per boundary
  *                face and per inner face, in the reference's order;
  *   batched = 1 (default): one launch per kernel over all blocks, and the
@@ -294,7 +294,7 @@ int cfx_get_rhs_tiles(cfx_ctx *ctx, int variant[3], int persist[3]);
 int cfx_autotune(cfx_ctx *ctx, int reps, double *best_ms);
 
 /* ------------------------------------------------ transport surrogate ---
- * No counterpart in the reference (its transport is Transport.cu:206-259,
+ * No counterpart in the reference (its transport is Transport.cu:43-96,
  * which generates the training samples and stays the fallback).  A 3 x 64
  * SiLU MLP maps (T, log p, Y_1..Y_ns) to log(mu, kappa, rho*D_1..ns), BF16
  * weights on MFMA with FP32 accumulation, trained on the GPU.  Runs on the

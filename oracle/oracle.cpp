@@ -6,10 +6,10 @@
 // routine follows the reference source text it cites (file:line); the only
 // departures are the ones the reference leaves undefined or that the step
 // adds, each marked "corrected" or "new":
-//   - Transport.cu:251 indexes X with the grid index i (out of range); the
+//   - Transport.cu:88 indexes X with the grid index i (out of range); the
 //     oracle uses X[l].                                          (corrected)
-//   - Field.cu:308 (n[1] = blockDim.y + blockIdx.y + ...) and the host array
-//     passed to the device at Driver.cu:268; the oracle loops the face range
+//   - Field.cu:216 (n[1] = blockDim.y + blockIdx.y + ...) and the host array
+//     passed to the device at Driver.cu:214; the oracle loops the face range
 //     with the index map of Mesh.cpp:905-917.                     (corrected)
 //   - Newton stop |dT| <= 1e-10 K unless CFX_Q_NEWTON_REL.         (corrected)
 //   - Van Albada limiter (tag 1), slip wall (type 3), SSP-RK3 with a global
@@ -28,8 +28,8 @@
 
 namespace {
 
-constexpr double kRu = 8314.462618;   // Constants.h:64
-constexpr double kMwAir = 28.9647;    // Constants.h:73
+constexpr double kRu = 8314.462618;   // Constants.h:19
+constexpr double kMwAir = 28.9647;    // Constants.h:28
 constexpr int kMaxSp = CFX_MAX_SPEC;
 constexpr int kMaxVar = 5 + kMaxSp + 2;
 
@@ -66,7 +66,7 @@ inline int isgn(int a) { return (a > 0) - (a < 0); }
 inline double dsgn(double a) { return (a > 0) - (a < 0); }
 
 // ---------------------------------------------------------------- thermo ---
-// Thermo.cu:46-66 compute_enthalpy_and_cp
+// Thermo.cu:27-47 compute_enthalpy_and_cp
 void enthalpy_and_cp(const Species &s, double t, double *h, double *cp) {
   const double t2{t * t}, t3{t2 * t}, t4{t3 * t}, t5{t4 * t};
   for (int i = 0; i < s.ns; ++i) {
@@ -87,13 +87,13 @@ void enthalpy_and_cp(const Species &s, double t, double *h, double *cp) {
   }
 }
 
-// Thermo.cu:26-44 compute_enthalpy (same h as above)
+// Thermo.cu:7-25 compute_enthalpy (same h as above)
 void enthalpy(const Species &s, double t, double *h) {
   double cp[kMaxSp];
   enthalpy_and_cp(s, t, h, cp);
 }
 
-// Thermo.cu:68-82 compute_cp (cp frozen below t_low)
+// Thermo.cu:49-63 compute_cp (cp frozen below t_low)
 void cp_only(const Species &s, double t, double *cp) {
   const double t2{t * t}, t3{t2 * t}, t4{t3 * t};
   for (int i = 0; i < s.ns; ++i) {
@@ -110,7 +110,7 @@ void cp_only(const Species &s, double t, double *cp) {
   }
 }
 
-// Thermo.cu:108-144 compute_temperature.  Returns the iteration count; the
+// Thermo.cu:89-125 compute_temperature.  Returns the iteration count; the
 // stop rule is |dT| <= 1e-10 K unless quirk CFX_Q_NEWTON_REL (|1-T1/T|<=1e-3).
 int newton_t(const Species &s, int quirks, const double *Y, double e, double t, double *t_out,
              double *gas_const_out) {
@@ -142,10 +142,10 @@ int newton_t(const Species &s, int quirks, const double *Y, double e, double t, 
   return iter;
 }
 
-// Transport.cu:170-173
+// Transport.cu:7-10
 double sutherland(double t) { return 1.716e-5 * std::pow(t / 273, 1.5) * (273 + 111) / (t + 111); }
 
-// Transport.cu:206-259 compute_transport_property (X[l] in the rho*D branch)
+// Transport.cu:43-96 compute_transport_property (X[l] in the rho*D branch)
 void transport(const Species &s, double sc, double t, double mw_total, const double *cp, const double *Y,
                double *mu_out, double *kappa_out, double *rhoD) {
   const int ns = s.ns;
@@ -185,7 +185,7 @@ void transport(const Species &s, double sc, double t, double mw_total, const dou
   }
 }
 
-// Limiter.cuh:135-138 (minmod, tags 0 and 1 in the reference); tag 1 here is
+// Limiter.cuh:13-16 (minmod, tags 0 and 1 in the reference); tag 1 here is
 // Van Albada: ab(a+b)/(a^2+b^2) for ab > 0, else 0 (new).
 double limiter(int tag, double a, double b) {
   if (tag == 1) {
@@ -195,8 +195,8 @@ double limiter(int tag, double a, double b) {
   return 0.5 * (dsgn(a) + dsgn(b)) * std::min(std::abs(a), std::abs(b));
 }
 
-// Reconstruction.cu:45-77 (first order / MUSCL kappa=1/3) + 79-122 (face
-// thermo) and InviscidScheme.cu:220-304 (AUSM+).  pv4 holds 4 points of
+// Reconstruction.cu:12-44 (first order / MUSCL kappa=1/3) + 46-89 (face
+// thermo) and InviscidScheme.cu:24-108 (AUSM+).  pv4 holds 4 points of
 // (rho,u,v,w,p,Y..) with stride nrec; metr is the metric row of the face
 // direction at the 2 face points, jac the two jacobians.
 void ausm_face(const Species &s, const Params &P, const double *pv4, int nrec, const double *metr,
@@ -253,7 +253,7 @@ void ausm_face(const Species &s, const Params &P, const double *pv4, int nrec, c
     pv_r[5] = er * pv_r[0] + pv_r[4] / (P.gamma_air - 1);
     gam_l = gam_r = P.gamma_air;
   }
-  // InviscidScheme.cu:235-304
+  // InviscidScheme.cu:39-106
   constexpr double alpha{3 / 16.0};
   const double *metr_l = metr, *metr_r = metr + 3;
   const double jac_l = jac[0], jac_r = jac[1];
@@ -671,7 +671,7 @@ void init_ghost_grid(orc_ctx &C) {
   }
 }
 
-// Thermo.cu:85-105 compute_total_energy at padded index p
+// Thermo.cu:66-86 compute_total_energy at padded index p
 void total_energy(const orc_ctx &C, Block &B, long p) {
   const long np = B.np;
   const double u = B.bv[1 * np + p], v = B.bv[2 * np + p], w = B.bv[3 * np + p];
@@ -688,7 +688,7 @@ void total_energy(const orc_ctx &C, Block &B, long p) {
   B.cv[4 * np + p] = e;
 }
 
-// Field.cu:242-268 compute_cv_from_bv over the padded range
+// Field.cu:150-176 compute_cv_from_bv over the padded range
 void cv_from_bv(const orc_ctx &C, Block &B) {
   const long np = B.np;
   const int ns = C.sp.ns;
@@ -704,7 +704,7 @@ void cv_from_bv(const orc_ctx &C, Block &B) {
   }
 }
 
-// Field.cu:270-302 update_physical_properties over the padded range
+// Field.cu:178-210 update_physical_properties over the padded range
 void properties(const orc_ctx &C, Block &B) {
   const long np = B.np;
   const int ns = C.sp.ns;
@@ -844,13 +844,13 @@ void viscous_face_at(const orc_ctx &C, const Block &B, int d, long p, double *G)
   }
 }
 
-// compute_inviscid_flux + compute_viscous_flux (TimeAdvanceFunc.cu:84-289):
+// compute_inviscid_flux + compute_viscous_flux (TimeAdvanceFunc.cu:38-243):
 // dq(i) = -sum_d (F(i+1/2)-F(i-1/2)) + sum_d (G(i+1/2)-G(i-1/2)).
 void rhs(orc_ctx &C, Block &B) {
   const int nv = C.nv;
   const int mx = B.mx, my = B.my, mz = B.mz;
   const long ni = B.ni;
-  std::fill(B.dq.begin(), B.dq.end(), 0.0);  // set_dq_to_0 (TimeAdvanceFunc.cu:72-82)
+  std::fill(B.dq.begin(), B.dq.end(), 0.0);  // set_dq_to_0 (TimeAdvanceFunc.cu:26-36)
   const int dim = C.dim;
   for (int pass = 0; pass < 2; ++pass) {  // 0 inviscid, 1 viscous
     if (pass == 1 && C.P.viscous != 2) break;
@@ -919,7 +919,7 @@ void local_dt(const orc_ctx &C, Block &B) {
       }
 }
 
-// update_cv_and_bv (TimeAdvanceFunc.cu:301-336) with the RK stage combination
+// update_cv_and_bv (TimeAdvanceFunc.cu:255-290) with the RK stage combination
 // q = a*qn + b*(q + dt*dq/J) (a=0,b=1 for the reference's Euler step).
 void update(orc_ctx &C, Block &B, int stage) {
   const long np = B.np, ni = B.ni;
@@ -959,7 +959,7 @@ void update(orc_ctx &C, Block &B, int stage) {
             B.yk[l * np + p] = cv[(5 + l) * np + p] * density_inv;
             Y[l] = B.yk[l * np + p];
           }
-          // Thermo.cu:108-144
+          // Thermo.cu:89-125
           const double e = cv[4 * np + p] / cv[p] - 0.5 * velocity;
           double t, gas_const;
           const int it = newton_t(C.sp, C.P.quirks, Y, e, bv[5 * np + p], &t, &gas_const);
@@ -1148,8 +1148,8 @@ void apply_bc(orc_ctx &C) {
   }
 }
 
-// Driver::data_communication + inner_communication (Driver.cu:248-271,
-// Field.cu:305-358) with the face range looped as in Mesh.cpp:905-917.
+// Driver::data_communication + inner_communication (Driver.cu:194-217,
+// Field.cu:213-266) with the face range looped as in Mesh.cpp:905-917.
 void exchange(orc_ctx &C) {
   const int ns = C.sp.ns, nv = C.nv;
   for (auto &B : C.blocks) {
@@ -1203,7 +1203,7 @@ void exchange(orc_ctx &C) {
   }
 }
 
-// eliminate_k_gradient (Field.cu:360-384)
+// eliminate_k_gradient (Field.cu:268-292)
 void eliminate_k(orc_ctx &C, Block &B) {
   const long np = B.np;
   for (int j = -B.ngg; j < B.my + B.ngg; ++j)
@@ -1407,7 +1407,7 @@ int orc_init_conservative(orc_ctx *C) {
   return 0;
 }
 
-// store_last_step (TimeAdvanceFunc.cu:59-70)
+// store_last_step (TimeAdvanceFunc.cu:13-24)
 int orc_store_last_step(orc_ctx *C) {
   for (auto &B : C->blocks) {
     const long np = B.np, ni = B.ni;
@@ -1464,7 +1464,7 @@ int orc_update_properties(orc_ctx *C) {
   return 0;
 }
 
-// compute_square_of_dbv + reduction_of_dbv_squared + the root (Driver.cu:273-311)
+// compute_square_of_dbv + reduction_of_dbv_squared + the root (Driver.cu:219-257)
 int orc_residual(orc_ctx *C, double res[4]) {
   long double acc[4] = {0, 0, 0, 0};
   for (auto &B : C->blocks) {
@@ -1485,7 +1485,7 @@ int orc_residual(orc_ctx *C, double res[4]) {
   return 0;
 }
 
-// Driver::steady_simulation loop body (Driver.cu:189-244); with rk_stages == 3
+// Driver::steady_simulation loop body (Driver.cu:135-190); with rk_stages == 3
 // the body runs once per SSP-RK3 stage with the global dt of stage 0.
 int orc_step(orc_ctx *C, int n_steps) {
   for (int s = 0; s < n_steps; ++s) {

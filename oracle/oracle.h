@@ -45,20 +45,20 @@ int orc_set_dt(orc_ctx *ctx, double dt);
 long long orc_newton_failures(orc_ctx *ctx);
 
 /* Point-wise building blocks, exposed for known-answer tests. */
-/* Thermo.cu:46-66 — h [J/kg] and cp [J/(kg K)] of every species at t. */
+/* Thermo.cu:27-47 — h [J/kg] and cp [J/(kg K)] of every species at t. */
 int orc_enthalpy_cp(const cfx_species *sp, double t, double *h, double *cp);
-/* Transport.cu:206-259 — mu, kappa, rho*D (n_spec) at (t, Y) with cp from
+/* Transport.cu:43-96 — mu, kappa, rho*D (n_spec) at (t, Y) with cp from
  * compute_cp; returns mixture mw through *mw_out. */
 int orc_transport(const cfx_species *sp, double schmidt, double t, const double *Y,
                   double *mu, double *kappa, double *rhoD);
-/* Thermo.cu:108-144 — Newton inversion of T from e; returns iterations used. */
+/* Thermo.cu:89-125 — Newton inversion of T from e; returns iterations used. */
 int orc_newton_temperature(const cfx_species *sp, int quirks, const double *Y,
                            double e, double t_guess, double *t_out);
-/* Limiter.cuh:135-138 (tag 0) and the Van Albada limiter (tag 1). */
+/* Limiter.cuh:13-16 (tag 0) and the Van Albada limiter (tag 1). */
 double orc_limiter(int tag, double a, double b);
-/* Transport.cu:170-173 */
+/* Transport.cu:7-10 */
 double orc_sutherland(double t);
-/* One AUSM+ face flux (InviscidScheme.cu:220-304, Reconstruction.cu:79-122)
+/* One AUSM+ face flux (InviscidScheme.cu:24-108, Reconstruction.cu:46-89)
  * from four points of primitive data pv[4][5+ns] (rho,u,v,w,p,Y...), the
  * metric row k of the face direction at the two face points
  * (metric[2][3]) and jacobians jac[2].  flux has 5+ns entries. */

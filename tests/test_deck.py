@@ -343,7 +343,7 @@ def test_grid_file_errors(tmp_path):
 
 
 def test_grid_scale(oracle_lib, tmp_path):
-    """`double gridScale` multiplies every coordinate (Mesh.cpp:534-536,
+    """`double gridScale` multiplies every coordinate (Mesh.cpp:537-543,
     704-716) unless it is 1 within 1e-15."""
     case, init = deck_cases.lattice_deck(str(tmp_path), sizes=((6, 5), (5,), (5,)))
     ctl = tmp_path / "input_files" / "setup" / "0_global_control.txt"

@@ -6,16 +6,16 @@ function is pinned by closed forms, invariants and brute force chosen so
 that a dropped term, a wrong sign or index, or a transposed operand fails
 at least one check:
 
-  NASA-7 thermo (Thermo.cu:26-82)      dh/dT = cp by finite differences;
+  NASA-7 thermo (Thermo.cu:7-63)      dh/dT = cp by finite differences;
                                        constant-cp closed form; linear
                                        extension and frozen cp below t_low
-  Newton T inversion (Thermo.cu:108)   round trip T -> e(T) -> T; constant-cp
+  Newton T inversion (Thermo.cu:89-125)   round trip T -> e(T) -> T; constant-cp
                                        closed form T = (e - sum Y r a5)/(sum Y r (a0-1))
-  Sutherland (Transport.cu:170)        closed form
-  Wilke/Eucken (Transport.cu:206)      one species = Chapman-Enskog closed
+  Sutherland (Transport.cu:7-10)        closed form
+  Wilke/Eucken (Transport.cu:43-96)      one species = Chapman-Enskog closed
                                        form; N identical species = one;
                                        long-double brute force; rho*D closed form
-  limiters (Limiter.cuh:135)           closed forms, symmetry
+  limiters (Limiter.cuh:13-16)           closed forms, symmetry
   MUSCL + AUSM+ (InviscidScheme.cu)    F(U,U) = physical flux |k|; supersonic
                                        upwinding F = F(U_L); linear data
   metrics (Mesh.cpp:226-488)           Cartesian closed form; 2-D discrete GCL
@@ -24,7 +24,7 @@ at least one check:
                                        slip-wall box; mirror symmetry of the
                                        slip wall; i<->j transpose symmetry;
                                        2-block split = 1 block; RK3 order
-  residual (Driver.cu:273-331)         numpy brute force
+  residual (Driver.cu:219-277)         numpy brute force
 """
 import math
 

@@ -294,7 +294,7 @@ def test_wall_outflow_slip_ghost_identities(oracle_lib):
         jin = jw - sgn
         p_in = bv[4][kk, jin, ii]
         t_wall = bv[5][kk, jin, ii] if tw is None else np.full_like(p_in, tw)
-        # wall point (BoundCond.cu:553-575)
+        # wall point (BoundCond.cu:546-580)
         np.testing.assert_array_equal(yk[:, kk, jw, ii], yk[:, kk, jin, ii])
         np.testing.assert_allclose(bv[0][kk, jw, ii], p_in * mw(yk[:, kk, jin, ii]) / (t_wall * RU), rtol=1e-15)
         assert np.all(bv[1:4][:, kk, jw, ii] == 0) and np.all(cv[1:4][:, kk, jw, ii] == 0)
