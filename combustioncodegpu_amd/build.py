@@ -27,7 +27,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno
 # Species counts with compiled kernels (the species loops are unrolled at
 # compile time); kernels_ns.hip is built once per count and precision.
 SPECIES_COUNTS = (9, 5, 3, 2, 1, 0)
-SOURCES = ["cfx_api.cpp", "surrogate.cpp", "mesh.cpp", "deck_io.cpp", "kernels_common.hip", "mlp_kernels.hip"]
+SOURCES = ["cfx_api.cpp", "surrogate.cpp", "mesh.cpp", "deck_io.cpp", "kernels_common.hip", "mlp_kernels.hip",
+           "noalloc.hip"]
 
 
 def _units():

@@ -75,6 +75,8 @@ def load():
     lib.cfx_mlp_gradient.restype = C.c_int
     lib.cfx_mlp_predict.argtypes = [vp, fp, C.c_int, fp]
     lib.cfx_mlp_predict.restype = C.c_int
+    lib.cfx_mlp_predict_noalloc.argtypes = [vp, fp, C.c_int, fp]
+    lib.cfx_mlp_predict_noalloc.restype = C.c_int
     lib.cfx_set_transport.argtypes = [vp, C.c_int]
     lib.cfx_set_transport.restype = C.c_int
     lib.cfx_mlp_stats.argtypes = [vp, C.POINTER(C.c_longlong)]
@@ -222,6 +224,15 @@ class Solver(StepContext):
         y = np.zeros_like(x)
         f = C.POINTER(C.c_float)
         self._call("mlp_predict", x.ctypes.data_as(f), int(x.shape[0]), y.ctypes.data_as(f))
+        return y
+
+    def mlp_predict_noalloc(self, x, y=None):
+        """cfx_mlp_predict_noalloc: y (n x 16 float32, written in place) from x."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        if y is None:
+            y = np.full_like(x, np.nan)
+        f = C.POINTER(C.c_float)
+        self._call("mlp_predict_noalloc", x.ctypes.data_as(f), int(x.shape[0]), y.ctypes.data_as(f))
         return y
 
     def set_transport(self, transport):

@@ -350,6 +350,14 @@ int cfx_mlp_set_check(cfx_ctx *ctx, double check_tol, int check_every);
 int cfx_mlp_gradient(cfx_ctx *ctx, const float *x, const float *t, int n, float *grad, double *loss);
 /* Normalised outputs y[n][16] of normalised inputs x[n][16]. */
 int cfx_mlp_predict(cfx_ctx *ctx, const float *x, int n, float *y);
+/* cfx_mlp_predict without device allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (serving loops, code running
+ * while a graph is captured on the device).  Stages x and y through a fixed
+ * device array of 2 x 65536 x 16 floats (8 MiB per device, held until the
+ * process ends) instead of buffers allocated per call; n > 65536 returns
+ * CFX_E_ARG.  Results equal cfx_mlp_predict's.  Every call on one device
+ * shares that array: calls that overlap on one device race. */
+int cfx_mlp_predict_noalloc(cfx_ctx *ctx, const float *x, int n, float *y);
 int cfx_set_transport(cfx_ctx *ctx, int transport);
 /* Counters since training/loading: [0] points outside the envelope,
  * [1] points checked against Wilke, [2] checks failed. */
