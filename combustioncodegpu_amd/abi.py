@@ -72,6 +72,7 @@ class CfxParams(C.Structure):
         ("precision", C.c_int), ("transport", C.c_int), ("quirks", C.c_int),
         ("n_inflow", C.c_int), ("inflow", C.POINTER(CfxInflow)),
         ("n_wall", C.c_int), ("wall", C.POINTER(CfxWall)),
+        ("lusgs", C.c_int), ("lusgs_omega", C.c_double),
     ]
 
 
@@ -138,6 +139,7 @@ class Marshalled:
             quirks=p.get("quirks", QUIRKS_REFERENCE),
             n_inflow=len(inflows), inflow=self.inflow_arr,
             n_wall=len(walls), wall=self.wall_arr,
+            lusgs=p.get("lusgs", 0), lusgs_omega=p.get("lusgs_omega", 0.0),
         )
         sp = case.species
         self.species = CfxSpecies(n_spec=0)
@@ -187,6 +189,7 @@ def declare_api(lib, prefix: str):
         "rhs": ([vp, C.c_int], C.c_int),
         "time_step": ([vp, C.c_int, _dp], C.c_int),
         "update": ([vp, C.c_int], C.c_int),
+        "lusgs": ([vp], C.c_int),
         "apply_bc": ([vp], C.c_int),
         "exchange": ([vp], C.c_int),
         "eliminate_k_gradient": ([vp], C.c_int),

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <limits>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -64,6 +65,16 @@ void fill_thermo(SpeciesT<T> &s) {
     v[kThHtlow] = s.h_tlow[l];
     v[kThCptlow] = s.cp_tlow[l];
   }
+  T tmid_min = std::numeric_limits<T>::max(), tmid_max = std::numeric_limits<T>::lowest();
+  T tlow_max = std::numeric_limits<T>::lowest();
+  for (int l = 0; l < s.ns; ++l) {
+    tmid_min = std::min(tmid_min, s.t_mid[l]);
+    tmid_max = std::max(tmid_max, s.t_mid[l]);
+    tlow_max = std::max(tlow_max, s.t_low[l]);
+  }
+  s.hi_from = std::max(tmid_max, tlow_max);
+  s.lo_from = tlow_max;
+  s.lo_to = tmid_min;
 }
 
 int build_species(const cfx_species *sp, SpeciesDev &tables, std::string &err) {
@@ -188,9 +199,12 @@ int chunks(long long n, int per) { return (int)((n + per - 1) / per); }
 
 int build_tables(cfx_ctx *ctx) {
   const int nb = (int)ctx->hb.size();
-  std::vector<int> pad(nb), in1(nb), dtc(nb), last(nb), res(nb);
+  std::vector<int> pad(nb), in1(nb), dtc(nb), last(nb), res(nb), pln(nb);
+  ctx->n_planes = 0;
   for (int b = 0; b < nb; ++b) {
     const HostBlock &H = ctx->hb[b];
+    pln[b] = chunks((long long)H.my * H.mz, kTpb);
+    ctx->n_planes = std::max(ctx->n_planes, H.mx + H.my + H.mz - 2);
     pad[b] = chunks(H.np, kTpb);
     in1[b] = chunks(H.ni, kTpb);
     dtc[b] = chunks(H.ni, kTpb * kDtIter);
@@ -200,7 +214,7 @@ int build_tables(cfx_ctx *ctx) {
   int rc;
   if ((rc = upload_table(ctx, ctx->t_pad, pad)) || (rc = upload_table(ctx, ctx->t_int, in1)) ||
       (rc = upload_table(ctx, ctx->t_dt, dtc)) || (rc = upload_table(ctx, ctx->t_last, last)) ||
-      (rc = upload_table(ctx, ctx->t_res, res)))
+      (rc = upload_table(ctx, ctx->t_res, res)) || (rc = upload_table(ctx, ctx->t_plane, pln)))
     return rc;
   for (int d = 0; d < ctx->dim; ++d)
     for (int v = 0; v < rhs_variants(ctx->dim); ++v) {
@@ -510,6 +524,23 @@ int do_update(cfx_ctx *ctx, int stage) {
   return 0;
 }
 
+// LU-SGS solve (the implicit treatment of Driver.cu:156): coefficients, then
+// the forward and backward sweeps hyperplane by hyperplane; dq <- dQ
+int do_lusgs(cfx_ctx *ctx) {
+  const KernelSet64 &k = g_kernels64[ctx->ns];
+  CFX_TRY(per_batch(ctx, ctx->t_int, [&](const Batch &b) {
+    return ctx->timed_launch(K_UPDATE, [&] { return k.lusgs_coef(b, ctx->S, ctx->stream); });
+  }));
+  for (int pass = 0; pass < 2; ++pass)
+    for (int n = 0; n < ctx->n_planes; ++n) {
+      const int plane = pass == 0 ? n : ctx->n_planes - 1 - n;
+      CFX_TRY(per_batch(ctx, ctx->t_plane, [&](const Batch &b) {
+        return ctx->timed_launch(K_UPDATE, [&] { return k.lusgs_sweep(b, ctx->S, plane, pass == 0, ctx->stream); });
+      }));
+    }
+  return 0;
+}
+
 int do_bc(cfx_ctx *ctx) {
   if (ctx->batched) return run_levels(ctx, ctx->sched_bc, 0, ctx->sched_bc.levels());
   return run_levels(ctx, ctx->sched_seq, 0, ctx->n_bc_ops);
@@ -576,6 +607,7 @@ int do_step(cfx_ctx *ctx) {
     } else {
       if ((rc = do_dt(ctx, 0, 1))) return rc;
     }
+    if (ctx->lusgs && (rc = do_lusgs(ctx))) return rc;
     if ((rc = do_update(ctx, s))) return rc;
     if ((rc = do_ghosts(ctx, true))) return rc;
     if ((rc = do_elim(ctx))) return rc;
@@ -742,7 +774,14 @@ int cfx_create(const cfx_params *params, const cfx_species *species, const cfx_b
   S.r_air = params->r_air > 0 ? params->r_air : kRuH / kMwAir;
   S.inv_sc = 1.0 / S.sc;
   S.inv_pr = 1.0 / S.pr;
+  S.lusgs = params->lusgs ? 1 : 0;
+  S.omega = params->lusgs_omega > 0 ? params->lusgs_omega : 1.0;
   ctx->rk = S.rk;
+  ctx->lusgs = S.lusgs != 0;
+  if (ctx->lusgs && (S.rk != 1 || ctx->mixed)) {
+    ctx->err = "the LU-SGS step needs rk_stages 1 (local time step) and FP64";
+    return bail(CFX_E_UNSUPPORTED);
+  }
   for (int i = 0; i < params->n_inflow; ++i) {
     InflowDev d;
     rc = build_inflow(params->inflow[i], ctx->sp_host.d, S, d, ctx->err);
@@ -821,6 +860,7 @@ int cfx_create(const cfx_params *params, const cfx_species *species, const cfx_b
     V.qn = ctx->dalloc<double>(ctx->nv * ni, rc);
     V.dt_local = ctx->dalloc<double>(ni, rc);
     V.bv_last = ctx->dalloc<double>(4 * ni, rc);
+    if (ctx->lusgs) V.lus = ctx->dalloc<double>(4 * ni, rc);
     if (rc) return bail(ctx->fail(rc, "hipMalloc block arrays"));
     V.mixed = ctx->mixed ? 1 : 0;
     if (ctx->mixed) {
@@ -987,6 +1027,12 @@ int cfx_update(cfx_ctx *ctx, int stage) {
   if (!ctx) return CFX_E_ARG;
   if (stage < 0 || stage > 2 || (ctx->rk == 1 && stage != 0)) return ctx->fail(CFX_E_ARG, "bad stage");
   return do_update(ctx, stage);
+}
+
+int cfx_lusgs(cfx_ctx *ctx) {
+  if (!ctx) return CFX_E_ARG;
+  if (!ctx->lusgs) return ctx->fail(CFX_E_ARG, "cfx_lusgs needs cfx_params.lusgs = 1");
+  return do_lusgs(ctx);
 }
 
 int cfx_apply_bc(cfx_ctx *ctx) { return ctx ? do_bc(ctx) : CFX_E_ARG; }

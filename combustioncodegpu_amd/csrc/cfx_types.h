@@ -49,6 +49,10 @@ struct SpeciesT {
   T cond_add[kMaxSp];        // 1.25 R_u / W_l (Transport.cu:59)
   T wj4[kMaxSp][kMaxSp];     // (W_n/W_m)^(1/4)           (ChemData.cpp:257)
   T sqw8[kMaxSp][kMaxSp];    // 1/sqrt(8(1+W_m/W_n))      (ChemData.cpp:258)
+  // range bounds over the species: T >= hi_from puts every species in its
+  // high set (and above its t_low), lo_from <= T < lo_to every species in its
+  // low set above t_low (lo_from > lo_to when no such T exists)
+  T hi_from, lo_from, lo_to;
 };
 
 // The FP64 table and its FP32 rounding, side by side in one allocation.
@@ -61,6 +65,8 @@ struct SchemeDev {
   int reconstruction, limiter, viscous, quirks, rk, dim;
   double cfl, pr, sc, gamma_air, r_air;
   double inv_sc, inv_pr;
+  int lusgs;     // implicit LU-SGS step: the update adds dq (the increment)
+  double omega;  // LU-SGS over-relaxation of the inviscid spectral radius
 };
 
 // One block as seen by the kernels.
@@ -73,6 +79,7 @@ struct BlockView {
   double *cv, *bv, *yk, *props;  // props: gamma, c, mu, kappa, rhoD[ns]
   double *metric, *jac;          // metric: 9 x np (xi_x..zeta_z), jac: np
   double *dq, *qn, *dt_local, *bv_last;
+  double *lus;  // LU-SGS: D, then a_d for d = 0..2, each over the interior (null otherwise)
   // mixed precision: FP32 copies of the RHS inputs (refreshed by the
   // properties kernel over the padded range) and the FP32 dq; null in FP64 mode
   int mixed;
@@ -110,7 +117,7 @@ struct GView {
   int mx, my, mz;
   int px, py, pz;
   long long np, ni, sy, sz;
-  gptr<double> cv, bv, yk, props, metric, jac, dq, qn, dt_local, bv_last;
+  gptr<double> cv, bv, yk, props, metric, jac, dq, qn, dt_local, bv_last, lus;
   int mixed;
   gptr<float> bv32, yk32, props32, metric32, jac32, dq32;
   int fuse[6];
@@ -124,7 +131,7 @@ __device__ __forceinline__ GView gview(const BlockView &B) {
   g.cv = (gptr<double>)B.cv, g.bv = (gptr<double>)B.bv, g.yk = (gptr<double>)B.yk;
   g.props = (gptr<double>)B.props, g.metric = (gptr<double>)B.metric, g.jac = (gptr<double>)B.jac;
   g.dq = (gptr<double>)B.dq, g.qn = (gptr<double>)B.qn, g.dt_local = (gptr<double>)B.dt_local;
-  g.bv_last = (gptr<double>)B.bv_last;
+  g.bv_last = (gptr<double>)B.bv_last, g.lus = (gptr<double>)B.lus;
   g.mixed = B.mixed;
   g.bv32 = (gptr<float>)B.bv32, g.yk32 = (gptr<float>)B.yk32, g.props32 = (gptr<float>)B.props32;
   g.metric32 = (gptr<float>)B.metric32, g.jac32 = (gptr<float>)B.jac32, g.dq32 = (gptr<float>)B.dq32;

@@ -70,6 +70,9 @@ struct cfx_ctx {
   long long n_grid_total = 0;
   BlockView *views_dev = nullptr;
   Table t_pad, t_int, t_dt, t_last, t_res;     // point-kernel workgroup prefixes
+  Table t_plane;                               // LU-SGS sweeps: my * mz columns per block
+  int n_planes = 0;                            // LU-SGS hyperplanes of the largest block
+  bool lusgs = false;
   Table t_rhs[3][kNVar3];                      // pencil tiles per direction and variant
   int occ[2][3][kNVar3] = {};                  // resident workgroups per CU [fp32][dir][variant]
   int n_cu = 256;

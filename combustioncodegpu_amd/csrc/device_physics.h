@@ -9,6 +9,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "cfx_types.h"
 
 namespace cfx {
@@ -41,18 +43,36 @@ struct Tiny<float> {
   static constexpr float v = 1e-30f;
 };
 
-// Division, reciprocal and square root of the flux work: exact IEEE
-// operations in FP64 (parity with the oracle); in FP32 (mixed precision) the
-// hardware reciprocal and square root (v_rcp_f32, v_sqrt_f32, 1 ulp), whose
-// error is far below the FP32 rounding already accepted there.
-template <typename R>
-__device__ __forceinline__ R fdiv(R a, R b) { return a / b; }
-template <>
-__device__ __forceinline__ float fdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+// Division, reciprocal and square root of the flux work.  FP64: the hardware
+// reciprocal (v_rcp_f64) refined by two Newton steps, r <- r + r (1 - b r),
+// and the quotient corrected once, q <- q + r (a - b q): within an ulp of the
+// IEEE quotient for the normal, finite operands of the flux work (positive
+// denominators well inside the exponent range), without the scaling and
+// fix-up steps the IEEE sequence spends on special values.  FP32 (mixed
+// precision): the hardware reciprocal and square root (v_rcp_f32,
+// v_sqrt_f32, 1 ulp), far below the FP32 rounding already accepted there.
 template <typename R>
 __device__ __forceinline__ R frcp(R a) { return R(1) / a; }
 template <>
+__device__ __forceinline__ double frcp<double>(double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-b, r, 1.0);
+  return fma(r, e, r);
+}
+template <>
 __device__ __forceinline__ float frcp<float>(float a) { return __builtin_amdgcn_rcpf(a); }
+template <typename R>
+__device__ __forceinline__ R fdiv(R a, R b) { return a / b; }
+template <>
+__device__ __forceinline__ double fdiv<double>(double a, double b) {
+  const double r = frcp<double>(b);
+  const double q = a * r;
+  return fma(r, fma(-b, q, a), q);
+}
+template <>
+__device__ __forceinline__ float fdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 template <typename R>
 __device__ __forceinline__ R fsqrt(R a) { return sqrt(a); }
 template <>
@@ -89,11 +109,32 @@ __device__ __forceinline__ R on_all_lanes(R v) {
   return v;
 }
 
+// The NASA-7 range every lane of the wavefront is in, for every species:
+// kRangeHi (all in the high set), kRangeLo (all in the low set, above
+// t_low), or kRangeMixed.  Smooth fields put most wavefronts on one side of
+// t_mid; those evaluate one polynomial set instead of both (the value is
+// the same bit for bit: the general path selects the same expression).
+constexpr int kRangeMixed = 0, kRangeLo = 1, kRangeHi = 2;
+template <typename R>
+__device__ __forceinline__ int thermo_range(SpeciesPtrT<R> sp, R t0, R t1) {
+  const R hf = R(sp->hi_from), lf = R(sp->lo_from), lt = R(sp->lo_to);
+  if (__all(t0 >= hf && t1 >= hf)) return kRangeHi;
+  if (__all(t0 >= lf && t0 < lt && t1 >= lf && t1 < lt)) return kRangeLo;
+  return kRangeMixed;
+}
+
 // h and cp of one species from its thermo record c (c[i] = value i of the
 // record: constant memory read with scalar loads, or a copy in LDS).
-template <typename R, typename C>
+// RG: kRangeMixed (per-lane selection), or the wavefront's single range.
+template <typename R, int RG = kRangeMixed, typename C>
 __device__ __forceinline__ void thermo_h_cp(const C &c, R t, R &h, R &cp) {
   auto p5 = [&](int o) { return R(c[o]) + t * (R(c[o + 1]) + t * (R(c[o + 2]) + t * (R(c[o + 3]) + t * R(c[o + 4])))); };
+  if constexpr (RG != kRangeMixed) {
+    const int r = RG == kRangeHi ? 1 : 0;
+    cp = p5(kThCp + 5 * r);
+    h = t * p5(kThH + 6 * r) + R(c[kThH + 6 * r + 5]);
+    return;
+  }
   const R cp_lo = p5(kThCp), cp_hi = p5(kThCp + 5);
   const R h_lo = t * p5(kThH) + R(c[kThH + 5]);
   const R h_hi = t * p5(kThH + 6) + R(c[kThH + 11]);
@@ -114,10 +155,14 @@ __device__ __forceinline__ void species_h_cp(SpeciesPtrT<R> sp, int l, R t, R &h
 // h_l(T) alone (the same value as thermo_h_cp's h), for loops that need no
 // cp: the marker on the selected cp would otherwise keep both cp
 // polynomials alive.
-template <typename R>
+template <typename R, int RG = kRangeMixed>
 __device__ __forceinline__ R species_h(SpeciesPtrT<R> sp, int l, R t) {
   const auto &c = sp->th[l].v;
   auto p5 = [&](int o) { return R(c[o]) + t * (R(c[o + 1]) + t * (R(c[o + 2]) + t * (R(c[o + 3]) + t * R(c[o + 4])))); };
+  if constexpr (RG != kRangeMixed) {
+    const int r = RG == kRangeHi ? 1 : 0;
+    return t * p5(kThH + 6 * r) + R(c[kThH + 6 * r + 5]);
+  }
   const R h_lo = t * p5(kThH) + R(c[kThH + 5]);
   const R h_hi = t * p5(kThH + 6) + R(c[kThH + 11]);
   const R hp = on_all_lanes<R>(t >= R(c[kThTmid]) ? h_hi : h_lo);
@@ -206,16 +251,23 @@ __device__ __forceinline__ void ausm_from_states(SpeciesPtrT<R> sp, const TH &th
     const R t_l = fdiv<R>(pl[4], pl[0] * Ru * wl);
     const R t_r = fdiv<R>(pr[4], pr[0] * Ru * wr);
     R cpl = R(0), cpr = R(0);
+    auto species_loop = [&](auto rgc) {
+      constexpr int RG = decltype(rgc)::value;
 #pragma unroll
-    for (int l = 0; l < NS; ++l) {
-      R h, cp;
-      thermo_h_cp<R>(th.rec(l), t_l, h, cp);
-      el += h * pl[5 + l];
-      cpl += cp * pl[5 + l];
-      thermo_h_cp<R>(th.rec(l), t_r, h, cp);
-      er += h * pr[5 + l];
-      cpr += cp * pr[5 + l];
-    }
+      for (int l = 0; l < NS; ++l) {
+        R h, cp;
+        thermo_h_cp<R, RG>(th.rec(l), t_l, h, cp);
+        el += h * pl[5 + l];
+        cpl += cp * pl[5 + l];
+        thermo_h_cp<R, RG>(th.rec(l), t_r, h, cp);
+        er += h * pr[5 + l];
+        cpr += cp * pr[5 + l];
+      }
+    };
+    const int rg = thermo_range<R>(sp, t_l, t_r);
+    if (rg == kRangeHi) species_loop(std::integral_constant<int, kRangeHi>{});
+    else if (rg == kRangeLo) species_loop(std::integral_constant<int, kRangeLo>{});
+    else species_loop(std::integral_constant<int, kRangeMixed>{});
     E_l = pl[0] * el - pl[4];
     E_r = pr[0] * er - pr[4];
     gam_l = fdiv<R>(cpl, cpl - Ru * wl);

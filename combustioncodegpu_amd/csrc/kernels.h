@@ -82,6 +82,11 @@ struct KernelSet64 {
   // rho*D) from the FP64 transport; lim = t_lo, t_hi, log p_lo, log p_hi
   int (*mlp_samples)(const SpeciesDev *sp, const SchemeDev &S, long long n, unsigned long long seed,
                      const double lim[4], double alpha, double *x, double *y, hipStream_t st);
+  // LU-SGS: the diagonal and neighbour coefficients over the interior (point
+  // batch), and one hyperplane i + j + k = plane of the forward (fwd = 1) or
+  // backward sweep (plane batch: my * mz threads per block)
+  int (*lusgs_coef)(const Batch &b, const SchemeDev &S, hipStream_t st);
+  int (*lusgs_sweep)(const Batch &b, const SchemeDev &S, int plane, int fwd, hipStream_t st);
 };
 
 struct MlpDev;

@@ -304,7 +304,8 @@ __global__ void __launch_bounds__(kTpb) update_kernel(const BlockView *__restric
     for (int l = 0; l < NV; ++l) qnv[l] = __builtin_nontemporal_load(&B.qn[l * ni + q]);
   }
   const double t_old = B.bv[5 * np + p];
-  const double dtj = (rk3 ? *dt_dev : B.dt_local[q]) / B.jac[p];
+  // LU-SGS: dq already holds the increment
+  const double dtj = S.lusgs ? 1.0 : (rk3 ? *dt_dev : B.dt_local[q]) / B.jac[p];
   double c[NV];
 #pragma unroll
   for (int l = 0; l < NV; ++l) {
@@ -669,6 +670,133 @@ int launch_update(const Batch &b, const SpeciesDev *sp, const SchemeDev &S, int 
   return (int)hipGetLastError();
 }
 
+// ---------------------------------------------------------------- LU-SGS ---
+// Matrix-free LU-SGS (Yoon & Jameson, AIAA J. 26 (1988) 1025-1026), the
+// implicit treatment the reference declares without a body (LUSGS,
+// TemporalScheme.cuh:19-21).  Per interior cell:
+//   D = J/dt + sum_d (omega r_d + 2 v_d),  a_d = omega r_d / 2 + v_d,
+//   r_d = J (|V.grad xi_d| + c |grad xi_d|),  v_d = max(gamma, 4/3) mu/(rho Pr) J |grad xi_d|^2;
+//   forward  D dQ*_i = R_i + sum_d [ dG_d(i-e_d)/2 + a_d(i-e_d) dQ*_(i-e_d) ]
+//   backward dQ_i = dQ*_i + sum_d [ a_d(i+e_d) dQ_(i+e_d) - dG_d(i+e_d)/2 ] / D_i
+// dG_d(n) = G_d(Q_n + dQ_n) - G_d(Q_n): the inviscid flux of cell n through
+// S_d = J grad xi_d, the pressure change at the cell's frozen gamma; dQ = 0
+// outside the block.  dq holds R on entry and dQ on exit (in place: a cell
+// reads only final neighbour values).  The sweeps run hyperplane by
+// hyperplane, i + j + k = plane, every cell of a plane independent.
+
+// lus: D over the interior, then a_d for d = 0, 1, 2
+__global__ void __launch_bounds__(kTpb) lusgs_coef_kernel(const BlockView *__restrict__ views,
+                                                          const int *__restrict__ off, int nb, SchemeDev S) {
+  const PointWork pw = point_work(off, nb);
+  const GView B = gview(views[pw.blk]);
+  const long long np = B.np, ni = B.ni;
+  const long long q = pw.base + threadIdx.x;
+  if (q >= ni) return;
+  const long long p = interior_pidx(B, q);
+  const double J = B.jac[p], rho = B.bv[p];
+  const double u = B.bv[np + p], v = B.bv[2 * np + p], w = B.bv[3 * np + p];
+  const double gam = B.props[p], c = B.props[np + p], mu = B.props[2 * np + p];
+  const double vc = fmax(gam, 4.0 / 3.0) * mu / rho / S.pr;
+  double diag = J / B.dt_local[q];
+  for (int d = 0; d < S.dim; ++d) {
+    const double gx = B.metric[(3 * d) * np + p], gy = B.metric[(3 * d + 1) * np + p],
+                 gz = B.metric[(3 * d + 2) * np + p];
+    const double g2 = gx * gx + gy * gy + gz * gz;
+    const double r = J * (fabs(u * gx + v * gy + w * gz) + c * sqrt(g2));
+    const double vd = vc * J * g2;
+    diag += S.omega * r + 2 * vd;
+    B.lus[(1 + d) * ni + q] = 0.5 * S.omega * r + vd;
+  }
+  B.lus[q] = diag;
+}
+
+__device__ __forceinline__ void lusgs_flux_change(const GView &B, const SchemeDev &S, int d, long long p,
+                                                  const double *dQ, double *dG) {
+  constexpr int NV = 5 + NS;
+  const long long np = B.np;
+  const double J = B.jac[p];
+  const double Sx = J * B.metric[(3 * d) * np + p], Sy = J * B.metric[(3 * d + 1) * np + p],
+               Sz = J * B.metric[(3 * d + 2) * np + p];
+  const double gam = B.props[p];
+  double q0[NV], q1[NV];
+#pragma unroll
+  for (int l = 0; l < NV; ++l) {
+    q0[l] = B.cv[l * np + p];
+    q1[l] = q0[l] + dQ[l];
+  }
+  const double p0 = B.bv[4 * np + p];
+  const double ke0 = 0.5 * (q0[1] * q0[1] + q0[2] * q0[2] + q0[3] * q0[3]) / q0[0];
+  const double ke1 = 0.5 * (q1[1] * q1[1] + q1[2] * q1[2] + q1[3] * q1[3]) / q1[0];
+  const double p1 = p0 + (gam - 1) * (dQ[4] - (ke1 - ke0));
+  const double U0 = (Sx * q0[1] + Sy * q0[2] + Sz * q0[3]) / q0[0];
+  const double U1 = (Sx * q1[1] + Sy * q1[2] + Sz * q1[3]) / q1[0];
+  dG[0] = q1[0] * U1 - q0[0] * U0;
+  dG[1] = (q1[1] * U1 + Sx * p1) - (q0[1] * U0 + Sx * p0);
+  dG[2] = (q1[2] * U1 + Sy * p1) - (q0[2] * U0 + Sy * p0);
+  dG[3] = (q1[3] * U1 + Sz * p1) - (q0[3] * U0 + Sz * p0);
+  dG[4] = (q1[4] + p1) * U1 - (q0[4] + p0) * U0;
+#pragma unroll
+  for (int l = 5; l < NV; ++l) dG[l] = q1[l] * U1 - q0[l] * U0;
+}
+
+// One hyperplane of a sweep: thread t of a block is the column (j, k) =
+// (t % my, t / my), its cell on the plane i = plane - j - k (if inside).
+template <bool FWD>
+__global__ void __launch_bounds__(kTpb) lusgs_sweep_kernel(const BlockView *__restrict__ views,
+                                                           const int *__restrict__ off, int nb, SchemeDev S,
+                                                           int plane) {
+  constexpr int NV = 5 + NS;
+  const PointWork pw = point_work(off, nb);
+  const GView B = gview(views[pw.blk]);
+  const int mx = B.mx, my = B.my, mz = B.mz;
+  const long long t = pw.base + threadIdx.x;
+  if (t >= (long long)my * mz) return;
+  const int j = (int)(t % my), k = (int)(t / my);
+  const int i = plane - j - k;
+  if (i < 0 || i >= mx) return;
+  const long long ni = B.ni;
+  const long long q = ((long long)k * my + j) * mx + i;
+  const long long p = pidx(B, i, j, k);
+  const int ijk[3] = {i, j, k}, m[3] = {mx, my, mz};
+  const long long qs[3] = {1, mx, (long long)mx * my}, ps[3] = {1, B.sy, B.sz};
+  double acc[NV];
+#pragma unroll
+  for (int l = 0; l < NV; ++l) acc[l] = FWD ? B.dq[l * ni + q] : 0.0;
+  for (int d = 0; d < S.dim; ++d) {
+    if (FWD ? ijk[d] == 0 : ijk[d] == m[d] - 1) continue;
+    const long long qn = FWD ? q - qs[d] : q + qs[d];
+    const long long pn = FWD ? p - ps[d] : p + ps[d];
+    double dn[NV], dG[NV];
+#pragma unroll
+    for (int l = 0; l < NV; ++l) dn[l] = B.dq[l * ni + qn];
+    lusgs_flux_change(B, S, d, pn, dn, dG);
+    const double an = B.lus[(1 + d) * ni + qn];
+#pragma unroll
+    for (int l = 0; l < NV; ++l) acc[l] += FWD ? 0.5 * dG[l] + an * dn[l] : an * dn[l] - 0.5 * dG[l];
+  }
+  const double D = B.lus[q];
+#pragma unroll
+  for (int l = 0; l < NV; ++l) {
+    if (FWD) B.dq[l * ni + q] = acc[l] / D;
+    else B.dq[l * ni + q] += acc[l] / D;
+  }
+}
+
+int launch_lusgs_coef(const Batch &b, const SchemeDev &S, hipStream_t st) {
+  if (b.total <= 0) return 0;
+  lusgs_coef_kernel<<<b.total, kTpb, 0, st>>>(b.views, b.off, b.nb, S);
+  return (int)hipGetLastError();
+}
+
+int launch_lusgs_sweep(const Batch &b, const SchemeDev &S, int plane, int fwd, hipStream_t st) {
+  if (b.total <= 0) return 0;
+  if (fwd)
+    lusgs_sweep_kernel<true><<<b.total, kTpb, 0, st>>>(b.views, b.off, b.nb, S, plane);
+  else
+    lusgs_sweep_kernel<false><<<b.total, kTpb, 0, st>>>(b.views, b.off, b.nb, S, plane);
+  return (int)hipGetLastError();
+}
+
 int launch_cv_from_bv(const Batch &b, const SpeciesDev *sp, const SchemeDev &S, hipStream_t st) {
   if (b.total <= 0) return 0;
   cv_from_bv_kernel<<<b.total, kTpb, 0, st>>>(b.views, b.off, b.nb, sp, S);
@@ -770,6 +898,8 @@ struct Register64 {
     k.cv_from_bv = launch_cv_from_bv;
     k.update = launch_update;
     k.face_ops = launch_face_ops;
+    k.lusgs_coef = launch_lusgs_coef;
+    k.lusgs_sweep = launch_lusgs_sweep;
     register_rhs<double>(k);
     k.present = 1;
   }

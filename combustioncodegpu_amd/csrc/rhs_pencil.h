@@ -332,6 +332,7 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
       R dvel[4][3], avg[4];
       R dp[NS > 0 ? NS : 1], yb[NS > 0 ? NS : 1];
       R ucx = 0, ucy = 0, ucz = 0, e4 = 0;
+      int vrg = kRangeMixed;  // NASA-7 range of the face temperatures of the wavefront
       static_for<NVB>([&](auto bi) {
         constexpr int nb2 = decltype(bi)::value;
         constexpr int n = NI + nb2;
@@ -370,11 +371,15 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
               fbuf[2 * NT + tid] -= G2;
               fbuf[3 * NT + tid] -= G3;
               fbuf[4 * NT + tid] -= G4;
+              if constexpr (NS > 0) vrg = thermo_range<R>(sp, avg[3], avg[3]);
             }
           } else {
             constexpr int s2 = j - 4;
             yb[s2] = R(0.5) * (q[0] + q[sa]);
-            const R h = species_h<R>(sp, s2, avg[3]);
+            R h;
+            if (vrg == kRangeHi) h = species_h<R, kRangeHi>(sp, s2, avg[3]);
+            else if (vrg == kRangeLo) h = species_h<R, kRangeLo>(sp, s2, avg[3]);
+            else h = species_h<R>(sp, s2, avg[3]);
             const R proj = k1 * d[0] + k2 * d[1] + k3 * d[2];
             e4 += dif[s2] * h * proj;
             ucx += dif[s2] * d[0];

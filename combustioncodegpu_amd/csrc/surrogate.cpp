@@ -94,7 +94,7 @@ int mlp_alloc(cfx_ctx *ctx) {
   ctx->mlp_dev = ctx->dalloc<MlpDev>(1, rc);
   ctx->mlp_counters = ctx->dalloc<unsigned long long>(3, rc);
   if (rc) return ctx->fail(rc, "hipMalloc surrogate");
-  CFX_TRY((int)hipMemset(ctx->mlp_counters, 0, 3 * sizeof(unsigned long long)));
+  CFX_TRY((int)hipMemsetAsync(ctx->mlp_counters, 0, 3 * sizeof(unsigned long long), ctx->stream));
   std::memset(&ctx->mlp_host, 0, sizeof(MlpDev));
   ctx->mlp_host.nin = ctx->mlp_shape.nin;
   ctx->mlp_host.nout = ctx->mlp_shape.nout;
@@ -365,7 +365,7 @@ int cfx_mlp_train(cfx_ctx *ctx, const cfx_mlp_config *cfg_in, cfx_mlp_report *re
   }
   m.ready = 1;
   if (int r = upload_mlp(ctx)) return r;
-  CFX_TRY((int)hipMemset(ctx->mlp_counters, 0, 3 * sizeof(unsigned long long)));
+  CFX_TRY((int)hipMemsetAsync(ctx->mlp_counters, 0, 3 * sizeof(unsigned long long), ctx->stream));
   ctx->mlp_ready = true;
   ctx->drop_graph();
   if (rep) {
@@ -486,7 +486,9 @@ int cfx_mlp_predict(cfx_ctx *ctx, const float *x, int n, float *y) {
   if (rc) return ctx->fail(rc, "hipMalloc predict");
   CFX_TRY((int)hipStreamSynchronize(ctx->stream));
   CFX_TRY((int)hipMemcpy(dx, x, sizeof(float) * n * kMlpIO, hipMemcpyHostToDevice));
-  CFX_TRY((int)hipMemset(dy, 0, sizeof(float) * n * kMlpIO));
+  // on the context's stream: the stream does not synchronise with the null
+  // stream, so a plain hipMemset could land after the kernel below
+  CFX_TRY((int)hipMemsetAsync(dy, 0, sizeof(float) * n * kMlpIO, ctx->stream));
   CFX_TRY(launch_mlp_predict(dx, n, ctx->mlp_ops_inf, ctx->mlp_bias_inf, ctx->mlp_shape.nout, dy, ctx->stream));
   CFX_TRY((int)hipStreamSynchronize(ctx->stream));
   CFX_TRY((int)hipMemcpy(y, dy, sizeof(float) * n * kMlpIO, hipMemcpyDeviceToHost));

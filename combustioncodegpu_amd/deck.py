@@ -37,6 +37,15 @@ Decisions where the reference's behaviour is undefined or silent:
     columns raises;
   * label 3 faces become the library's slip wall (the reference applies
     nothing to them, BoundCond.cu:290);
+  * 6_inflow.txt / 7_wall.txt records are matched to their own labels.  The
+    reference's register_bc (BoundCond.cu:166-207 for inflows, 127-164 for
+    walls) does not refresh its input line after the first record, so with
+    two or more records it parses the first label line again, writes record
+    2 into label 1's slot and leaves the other slots uninitialised; parity
+    on multi-record decks is parity with this corrected reading;
+  * an element without a standard atomic weight (Tc, Pm, Po and heavier
+    unstable ones, and Ag / Hf, which the reference's table spells in mixed
+    case) raises; the reference prints a warning and uses -1;
   * CR-LF line ends are read as LF.
 """
 from __future__ import annotations
@@ -57,16 +66,27 @@ MW_AIR = 28.9647   # Constants.h:28
 AVOGADRO = 6.02214076e26   # 1/kmol
 ELECTRON_MASS = 9.1093837015e-31
 
-# Standard atomic weights (IUPAC 2021 abridged values) of the elements a
-# combustion mechanism names, by upper-case symbol; D and E as isotope /
-# electron entries (Element.cpp:13-145 holds the same published values).
-ATOMIC_WEIGHT = {
-    "H": 1.008, "HE": 4.002602, "LI": 6.94, "BE": 9.0121831, "B": 10.81, "C": 12.011, "N": 14.007,
-    "O": 15.999, "F": 18.998403163, "NE": 20.1797, "NA": 22.98976928, "MG": 24.305, "AL": 26.9815384,
-    "SI": 28.085, "P": 30.973761998, "S": 32.06, "CL": 35.45, "AR": 39.95, "K": 39.0983, "CA": 40.078,
-    "FE": 55.845, "KR": 83.798, "XE": 131.293,
-    "D": 2.0141017781, "E": ELECTRON_MASS * AVOGADRO,
-}
+# Standard atomic weights by upper-case symbol: the IUPAC 2017 table (the
+# values the reference's element table holds, Element.cpp:13-145, which
+# differ from the 2021 revision for Sc, Y, Zr, Ho and Tm), plus deuterium and
+# the electron.  Elements without a standard atomic weight are absent, as are
+# Ag and Hf: the reference spells those two in mixed case and compares them
+# with the upper-case names it reads, so it never finds them either.
+_IUPAC_2017 = """
+H 1.008 HE 4.002602 LI 6.94 BE 9.0121831 B 10.81 C 12.011 N 14.007 O 15.999 F 18.998403163
+NE 20.1797 NA 22.98976928 MG 24.305 AL 26.9815384 SI 28.085 P 30.973761998 S 32.06 CL 35.45
+AR 39.95 K 39.0983 CA 40.078 SC 44.955908 TI 47.867 V 50.9415 CR 51.9961 MN 54.938043 FE 55.845
+CO 58.933194 NI 58.6934 CU 63.546 ZN 65.38 GA 69.723 GE 72.63 AS 74.921595 SE 78.971 BR 79.904
+KR 83.798 RB 85.4678 SR 87.62 Y 88.90584 ZR 91.224 NB 92.90637 MO 95.95 RU 101.07 RH 102.90549
+PD 106.42 CD 112.414 IN 114.818 SN 118.71 SB 121.76 TE 127.6 I 126.90447 XE 131.293
+CS 132.90545196 BA 137.327 LA 138.90547 CE 140.116 PR 140.90766 ND 144.242 SM 150.36 EU 151.964
+GD 157.25 TB 158.925354 DY 162.5 HO 164.930328 ER 167.259 TM 168.934218 YB 173.045 LU 174.9668
+TA 180.94788 W 183.84 RE 186.207 OS 190.23 IR 192.217 PT 195.084 AU 196.96657 HG 200.592
+TL 204.38 PB 207.2 BI 208.9804 TH 232.0377 PA 231.03588 U 238.02891
+"""
+ATOMIC_WEIGHT = {k: float(v) for k, v in zip(*[iter(_IUPAC_2017.split())] * 2)}
+ATOMIC_WEIGHT["D"] = 2.0141017781
+ATOMIC_WEIGHT["E"] = ELECTRON_MASS * AVOGADRO
 
 
 class DeckError(ValueError):
@@ -384,7 +404,7 @@ def read_therm(path, species, elements):
         raise DeckError(f"no therm data for {missing}")
     for e in elements:
         if e not in ATOMIC_WEIGHT:
-            raise DeckError(f"unknown element {e}")
+            raise DeckError(f"element {e} has no standard atomic weight in the table")
     mw = np.zeros(ns)
     for i in range(ns):   # summed over the element map in its (sorted) key order
         for e in sorted(elements):

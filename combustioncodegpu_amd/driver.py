@@ -119,6 +119,9 @@ class StepContext:
     def update(self, stage=0):
         self._call("update", stage)
 
+    def lusgs(self):
+        self._call("lusgs")
+
     def apply_bc(self):
         self._call("apply_bc")
 

@@ -145,6 +145,14 @@ typedef struct {
   const cfx_inflow *inflow;
   int n_wall;
   const cfx_wall *wall;
+  /* Implicit step: lusgs = 1 replaces the explicit update by one matrix-free
+   * LU-SGS solve per step with the local time step (rk_stages must be 1).
+   * The reference declares the scheme (temporal_scheme 1, TemporalScheme.cuh:19-21,
+   * Field.cu:138-141) but gives it no body, so its tag alone keeps the explicit
+   * Euler step there and here.  lusgs_omega is the over-relaxation factor of
+   * the inviscid spectral radius (<= 0: 1.0). */
+  int lusgs;
+  double lusgs_omega;
 } cfx_params;
 
 /* Species data (Species, ChemData.h:11-46).  NASA-7 coefficient arrays are
@@ -220,8 +228,21 @@ int cfx_rhs(cfx_ctx *ctx, int blk);
 int cfx_time_step(cfx_ctx *ctx, int global, double *dt_out);
 /* update_cv_and_bv (TimeAdvanceFunc.cu:255-290) for every block.  With
  * rk_stages == 1 the local dt is used (reference); with rk_stages == 3,
- * stage = 0,1,2 applies the SSP-RK3 combination with the device dt. */
+ * stage = 0,1,2 applies the SSP-RK3 combination with the device dt; with
+ * lusgs = 1, dq holds the increment of cfx_lusgs and cv += dq. */
 int cfx_update(cfx_ctx *ctx, int stage);
+/* The implicit treatment the reference leaves empty ("implicit treatment if
+ * needed", Driver.cu:156; LUSGS, TemporalScheme.cuh:19-21): replaces dq (the
+ * RHS) of every block by the LU-SGS increment dQ of
+ *   (D + L) D^-1 (D + U) dQ = dq,  D = J/dt_local + sum_d (omega r_d + 2 v_d),
+ * L dQ = -sum_d [ (dG_d + omega r_d dQ)/2 + v_d dQ ] at the lower neighbour,
+ * U dQ =  sum_d [ (dG_d - omega r_d dQ)/2 - v_d dQ ] at the upper neighbour,
+ * with r_d = |V.S_d| + c|S_d| and v_d = max(gamma, 4/3) mu/(rho Pr) J |grad xi_d|^2
+ * at that neighbour (S_d = J grad xi_d), dG_d = G_d(Q + dQ) - G_d(Q) the
+ * change of its inviscid flux through S_d (pressure change at frozen gamma),
+ * and dQ = 0 outside the block (Yoon & Jameson, AIAA J. 26 (1988) 1025).
+ * Needs cfx_time_step's local dt.  Requires lusgs = 1 and FP64. */
+int cfx_lusgs(cfx_ctx *ctx);
 /* DBoundCond::apply_boundary_conditions (BoundCond.cu:251-313) + slip wall. */
 int cfx_apply_bc(cfx_ctx *ctx);
 /* Driver::data_communication (Driver.cu:194-217), inner faces on device. */

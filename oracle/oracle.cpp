@@ -58,8 +58,8 @@ struct Species {
 };
 
 struct Params {
-  int reconstruction = 2, limiter = 0, viscous = 2, rk = 1, quirks = CFX_QUIRKS_REFERENCE;
-  double cfl = 0.5, pr = 0.72, sc = 0.9, gamma_air = 1.4, r_air = kRu / kMwAir;
+  int reconstruction = 2, limiter = 0, viscous = 2, rk = 1, quirks = CFX_QUIRKS_REFERENCE, lusgs = 0;
+  double cfl = 0.5, pr = 0.72, sc = 0.9, gamma_air = 1.4, r_air = kRu / kMwAir, omega = 1.0;
 };
 
 inline int isgn(int a) { return (a > 0) - (a < 0); }
@@ -919,6 +919,122 @@ void local_dt(const orc_ctx &C, Block &B) {
       }
 }
 
+// Matrix-free LU-SGS (Yoon & Jameson, AIAA J. 26 (1988) 1025-1026) for the
+// implicit treatment the reference declares but leaves empty (LUSGS,
+// TemporalScheme.cuh:19-21; "implicit treatment if needed", Driver.cu:156).
+// Linearised implicit Euler with the local time step,
+//   (J/dt - dR/dQ) dQ = R,   R = dq of the RHS,
+// with the inviscid flux Jacobian through S_d = J grad xi_d split as
+// A+- = (A +- omega r_d I)/2, r_d = |V.S_d| + c|S_d|, and the viscous part
+// approximated by its spectral radius v_d = max(gamma, 4/3) mu/(rho Pr) J |grad xi_d|^2
+// (the viscous terms of the local time step, TemporalScheme.cu:30-40):
+//   D   = J/dt + sum_d (omega r_d + 2 v_d)            (at the cell)
+//   fwd D dQ*_i = R_i + sum_d [ dG_d(i-e_d)/2 + a_d(i-e_d) dQ*_(i-e_d) ]
+//   bwd dQ_i   = dQ*_i + sum_d [ a_d(i+e_d) dQ_(i+e_d) - dG_d(i+e_d)/2 ] / D_i
+// with a_d = omega r_d / 2 + v_d, dG_d(n) = G_d(Q_n + dQ_n) - G_d(Q_n) the
+// inviscid flux of cell n through S_d(n), the pressure change taken at the
+// cell's frozen gamma (exact for the calorically perfect gas), and dQ = 0
+// outside the block.  Every dQ*_i depends only on final lower-neighbour
+// values, so any order that respects the i+j+k partial order (lexicographic
+// here, hyperplanes on the GPU) gives the same result.
+void lusgs_flux_change(const orc_ctx &C, const Block &B, int d, long p, const double *dQ, double *dG) {
+  const long np = B.np;
+  const int nv = C.nv;
+  const double *M = B.metric.data();
+  const double J = B.jac[p];
+  const double Sx = J * M[(3 * d) * np + p], Sy = J * M[(3 * d + 1) * np + p], Sz = J * M[(3 * d + 2) * np + p];
+  const double gam = C.sp.ns > 0 ? B.gamma[p] : C.P.gamma_air;
+  double q0[kMaxVar], q1[kMaxVar];
+  for (int l = 0; l < nv; ++l) {
+    q0[l] = B.cv[l * np + p];
+    q1[l] = q0[l] + dQ[l];
+  }
+  const double p0 = B.bv[4 * np + p];
+  const double ke0 = 0.5 * (q0[1] * q0[1] + q0[2] * q0[2] + q0[3] * q0[3]) / q0[0];
+  const double ke1 = 0.5 * (q1[1] * q1[1] + q1[2] * q1[2] + q1[3] * q1[3]) / q1[0];
+  const double p1 = p0 + (gam - 1) * (dQ[4] - (ke1 - ke0));
+  const double U0 = (Sx * q0[1] + Sy * q0[2] + Sz * q0[3]) / q0[0];
+  const double U1 = (Sx * q1[1] + Sy * q1[2] + Sz * q1[3]) / q1[0];
+  dG[0] = q1[0] * U1 - q0[0] * U0;
+  dG[1] = (q1[1] * U1 + Sx * p1) - (q0[1] * U0 + Sx * p0);
+  dG[2] = (q1[2] * U1 + Sy * p1) - (q0[2] * U0 + Sy * p0);
+  dG[3] = (q1[3] * U1 + Sz * p1) - (q0[3] * U0 + Sz * p0);
+  dG[4] = (q1[4] + p1) * U1 - (q0[4] + p0) * U0;
+  for (int l = 5; l < nv; ++l) dG[l] = q1[l] * U1 - q0[l] * U0;
+}
+
+void lusgs(orc_ctx &C, Block &B) {
+  const long np = B.np, ni = B.ni;
+  const int nv = C.nv, dim = C.dim;
+  const double om = C.P.omega;
+  std::vector<double> Dg(ni), a(3 * ni, 0.0);
+  const double *M = B.metric.data();
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 0; k < B.mz; ++k)
+    for (int j = 0; j < B.my; ++j)
+      for (int i = 0; i < B.mx; ++i) {
+        const long p = B.id(i, j, k), q = B.iid(i, j, k);
+        const double J = B.jac[p], rho = B.bv[p];
+        const double u = B.bv[np + p], v = B.bv[2 * np + p], w = B.bv[3 * np + p];
+        const double gam = C.sp.ns > 0 ? B.gamma[p] : C.P.gamma_air;
+        const double vc = std::max(gam, 4.0 / 3.0) * B.mu[p] / rho / C.P.pr;
+        double diag = J / B.dt_local[q];
+        for (int d = 0; d < dim; ++d) {
+          const double gx = M[(3 * d) * np + p], gy = M[(3 * d + 1) * np + p], gz = M[(3 * d + 2) * np + p];
+          const double g2 = gx * gx + gy * gy + gz * gz;
+          const double r = J * (std::abs(u * gx + v * gy + w * gz) + B.c[p] * std::sqrt(g2));
+          const double vd = vc * J * g2;
+          diag += om * r + 2 * vd;
+          a[d * ni + q] = 0.5 * om * r + vd;
+        }
+        Dg[q] = diag;
+      }
+  double *dQ = B.dq.data();
+  const int m[3] = {B.mx, B.my, B.mz};
+  // forward sweep: lower neighbours final
+  for (int k = 0; k < B.mz; ++k)
+    for (int j = 0; j < B.my; ++j)
+      for (int i = 0; i < B.mx; ++i) {
+        const long q = B.iid(i, j, k);
+        double acc[kMaxVar];
+        for (int l = 0; l < nv; ++l) acc[l] = dQ[l * ni + q];
+        const int ijk[3] = {i, j, k};
+        for (int d = 0; d < dim; ++d) {
+          if (ijk[d] == 0) continue;
+          int n3[3] = {i, j, k};
+          n3[d] -= 1;
+          const long qn = B.iid(n3[0], n3[1], n3[2]), pn = B.id(n3[0], n3[1], n3[2]);
+          double dn[kMaxVar], dG[kMaxVar];
+          for (int l = 0; l < nv; ++l) dn[l] = dQ[l * ni + qn];
+          lusgs_flux_change(C, B, d, pn, dn, dG);
+          const double an = a[d * ni + qn];
+          for (int l = 0; l < nv; ++l) acc[l] += 0.5 * dG[l] + an * dn[l];
+        }
+        for (int l = 0; l < nv; ++l) dQ[l * ni + q] = acc[l] / Dg[q];
+      }
+  // backward sweep: upper neighbours final
+  for (int k = B.mz - 1; k >= 0; --k)
+    for (int j = B.my - 1; j >= 0; --j)
+      for (int i = B.mx - 1; i >= 0; --i) {
+        const long q = B.iid(i, j, k);
+        double acc[kMaxVar];
+        for (int l = 0; l < nv; ++l) acc[l] = 0.0;
+        const int ijk[3] = {i, j, k};
+        for (int d = 0; d < dim; ++d) {
+          if (ijk[d] == m[d] - 1) continue;
+          int n3[3] = {i, j, k};
+          n3[d] += 1;
+          const long qn = B.iid(n3[0], n3[1], n3[2]), pn = B.id(n3[0], n3[1], n3[2]);
+          double dn[kMaxVar], dG[kMaxVar];
+          for (int l = 0; l < nv; ++l) dn[l] = dQ[l * ni + qn];
+          lusgs_flux_change(C, B, d, pn, dn, dG);
+          const double an = a[d * ni + qn];
+          for (int l = 0; l < nv; ++l) acc[l] += an * dn[l] - 0.5 * dG[l];
+        }
+        for (int l = 0; l < nv; ++l) dQ[l * ni + q] += acc[l] / Dg[q];
+      }
+}
+
 // update_cv_and_bv (TimeAdvanceFunc.cu:255-290) with the RK stage combination
 // q = a*qn + b*(q + dt*dq/J) (a=0,b=1 for the reference's Euler step).
 void update(orc_ctx &C, Block &B, int stage) {
@@ -941,6 +1057,8 @@ void update(orc_ctx &C, Block &B, int stage) {
             for (int l = 0; l < nv; ++l) B.qn[l * ni + q] = cv[l * np + p];
           for (int l = 0; l < nv; ++l)
             cv[l * np + p] = A[stage] * B.qn[l * ni + q] + Bc[stage] * (cv[l * np + p] + B.dq[l * ni + q] * dt_div_jac);
+        } else if (C.P.lusgs) {
+          for (int l = 0; l < nv; ++l) cv[l * np + p] += B.dq[l * ni + q];
         } else {
           const double dt_div_jac = B.dt_local[q] / B.jac[p];
           for (int l = 0; l < nv; ++l) cv[l * np + p] += B.dq[l * ni + q] * dt_div_jac;
@@ -1280,6 +1398,9 @@ int orc_create(const cfx_params *params, const cfx_species *species, const cfx_b
   P.gamma_air = params->gamma_air > 0 ? params->gamma_air : 1.4;
   P.r_air = params->r_air > 0 ? params->r_air : kRu / kMwAir;
   P.quirks = params->quirks;
+  P.lusgs = params->lusgs ? 1 : 0;
+  P.omega = params->lusgs_omega > 0 ? params->lusgs_omega : 1.0;
+  if (P.lusgs && P.rk != 1) { delete C; return CFX_E_ARG; }
   C->nv = 5 + C->sp.ns;
   C->dim = blocks[0].mz == 1 ? 2 : 3;  // Mesh.cpp:579-581
   C->ngg = 2;                          // Parameter.cpp:32-36
@@ -1452,6 +1573,19 @@ int orc_update(orc_ctx *C, int stage) {
   return 0;
 }
 
+int orc_set_dq(orc_ctx *C, int blk, const double *dq) {
+  if (!C || blk < 0 || blk >= (int)C->blocks.size() || !dq) return CFX_E_ARG;
+  Block &B = C->blocks[blk];
+  std::memcpy(B.dq.data(), dq, sizeof(double) * B.dq.size());
+  return 0;
+}
+
+int orc_lusgs(orc_ctx *C) {
+  if (!C || !C->P.lusgs) return CFX_E_ARG;
+  for (auto &B : C->blocks) lusgs(*C, B);
+  return 0;
+}
+
 int orc_apply_bc(orc_ctx *C) { apply_bc(*C); return 0; }
 int orc_exchange(orc_ctx *C) { exchange(*C); return 0; }
 int orc_eliminate_k_gradient(orc_ctx *C) {
@@ -1498,6 +1632,7 @@ int orc_step(orc_ctx *C, int n_steps) {
       } else {
         orc_time_step(C, 0, nullptr);
       }
+      if (C->P.lusgs) orc_lusgs(C);
       orc_update(C, st);
       apply_bc(*C);
       exchange(*C);

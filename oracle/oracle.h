@@ -34,6 +34,9 @@ int orc_store_last_step(orc_ctx *ctx);
 int orc_rhs(orc_ctx *ctx, int blk);
 int orc_time_step(orc_ctx *ctx, int global, double *dt_out);
 int orc_update(orc_ctx *ctx, int stage);
+int orc_lusgs(orc_ctx *ctx);
+/* Test helper: overwrite dq of block blk (n_var x interior). */
+int orc_set_dq(orc_ctx *ctx, int blk, const double *dq);
 int orc_apply_bc(orc_ctx *ctx);
 int orc_exchange(orc_ctx *ctx);
 int orc_eliminate_k_gradient(orc_ctx *ctx);

@@ -34,6 +34,8 @@ def lib():
         abi.declare_api(_lib, "orc_")
         _lib.orc_set_threads.argtypes = [C.c_int]
         _lib.orc_set_threads.restype = C.c_int
+        _lib.orc_set_dq.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
+        _lib.orc_set_dq.restype = C.c_int
         _lib.orc_newton_failures.argtypes = [C.c_void_p]
         _lib.orc_newton_failures.restype = C.c_longlong
         dp = C.POINTER(C.c_double)
@@ -66,6 +68,12 @@ class Oracle(StepContext):
 
     def newton_failures(self):
         return self.lib.orc_newton_failures(self.h)
+
+    def set_dq(self, blk, dq):
+        a = np.ascontiguousarray(dq, dtype=np.float64)
+        rc = self.lib.orc_set_dq(self.h, blk, abi.dptr(a))
+        if rc:
+            raise RuntimeError(f"orc_set_dq: {rc}")
 
     def face_flux(self, blk, d, i, j, k, viscous):
         """Face flux between interior point (i,j,k) and (i,j,k)+e_d."""

@@ -169,6 +169,31 @@ def test_mechanism_therm_tran(tmp_path):
         deck.read_tran(str(tmp_path / "tran.dat"), species + ["OH"])
 
 
+def test_heavier_elements_and_missing_weights(tmp_path):
+    """Molecular weights of species with heavier elements (Br, Zn) from the
+    standard atomic weights: HBr = 1.008 + 79.904, ZnO = 65.38 + 15.999;
+    elements without a standard weight (Tc) and the two the reference never
+    matches (Ag, Hf) raise."""
+    def chem(elems, sp_lines):
+        mech = tmp_path / "m.inp"
+        mech.write_text("Elements\n--------\n%d elements\n--------\n%s\nSpecies\n--------\n%d species\n"
+                        "--------\n%s\nReactions\n" % (len(elems), " ".join(elems), len(sp_lines),
+                                                         " ".join(n for n, _ in sp_lines)))
+        therm = ["THERMO ALL", "   200.000  1000.000  6000.000"]
+        for name, comp in sp_lines:
+            therm += [_therm_head(name, comp, "300.000", "5000.000", "1000.000")] + N2_ROWS
+        (tmp_path / "t.dat").write_text("\n".join(therm + ["END"]) + "\n")
+        e, s_ = deck.read_mechanism(str(mech))
+        return deck.read_therm(str(tmp_path / "t.dat"), s_, e)
+
+    th = chem(["H", "BR", "ZN", "O"], [("HBR", "H   1BR  1"), ("ZNO", "ZN  1O   1")])
+    assert th["mw"][0] == 1.008 + 79.904
+    assert th["mw"][1] == 15.999 + 65.38   # sorted element order: O before ZN
+    for el in ("TC", "AG", "HF"):
+        with pytest.raises(deck.DeckError, match="no standard atomic weight"):
+            chem(["H", el], [("X", "H   1%-3s1" % el)])
+
+
 # ------------------------------------------------------------ round trips ---
 def _same_blocks(a, b):
     assert len(a.blocks) == len(b.blocks)

@@ -31,6 +31,35 @@ def test_deck_case_parity(tmp_path, which, steps):
     assert rc == 0 and nfail == 0
 
 
+def test_deck_without_rk_key_runs_the_reference_time_scheme(tmp_path):
+    """A deck without the library-only rk_stages key runs the reference's
+    explicit Euler with local time steps on the GPU: the library matches the
+    oracle at the per-step bar after 3 steps, and its local time step
+    (CFX_DTLOCAL, filled by the step) equals the oracle's."""
+    import numpy as np
+    from combustioncodegpu_amd import abi
+    from combustioncodegpu_amd.lib import Solver
+    from oracle.pyoracle import Oracle
+    from tests.helpers import assert_close
+    deck_cases.walls_deck(str(tmp_path), box=False)
+    ctl = tmp_path / "input_files" / "setup" / "0_global_control.txt"
+    ctl.write_text("\n".join(l for l in ctl.read_text().splitlines() if "rk_stages" not in l) + "\n")
+    case = deck.load_case(str(tmp_path))
+    assert case.params["rk_stages"] == 1
+    print("[deck] walls case without rk_stages: explicit Euler, local dt, 3 steps", flush=True)
+    o, g = Oracle(case), Solver(case)
+    for c in (o, g):
+        c.upload_initial()
+        c.init_conservative()
+        c.step(3)
+    compare_state(case, o, g, 3e-12, "3 Euler steps")
+    gd, od = g.download(0, abi.DTLOCAL), o.download(0, abi.DTLOCAL)
+    assert np.isfinite(gd).all() and (gd > 0).all()
+    assert_close(gd, od, 1e-12, "dt_local")
+    rc, nfail = g.check()
+    assert rc == 0 and nfail == 0
+
+
 def test_context_coordinates_match_host_ghost_grid():
     """CFX_COORD of a context (the ghost grid its metrics were computed from)
     equals the host-only cfx_ghost_grid, block by block."""

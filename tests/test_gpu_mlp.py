@@ -90,37 +90,76 @@ def test_training_reaches_accuracy():
 
 
 def test_surrogate_step_gain_and_error():
+    """What the surrogate does to the mixed-precision step, against the same
+    step with the Wilke routine: the error against the FP64 oracle after 10
+    and 100 SSP-RK3 steps (bar 1e-4 per field), and the time of the
+    properties pass.  On MI355X the surrogate's properties pass is slower
+    than FP32 Wilke (FP32 Wilke with hardware rcp/sqrt/exp2/log2 already runs
+    at the HBM floor of a separate properties pass, and the network adds 192
+    SiLU evaluations per point); the test pins that documented loss on a
+    block large enough that launch overhead does not hide it:
+    1.05 <= t_props(surrogate) / t_props(Wilke) <= 2.0."""
     from oracle.pyoracle import Oracle
     ref = cases.c1(24, 20, 16)
     o = Oracle(ref)
     o.upload_initial()
     o.init_conservative()
-    o.step(10)
+    marks = (10, 100)
+    ref_states = {}
+    done = 0
+    for m in marks:
+        o.step(m - done)
+        done = m
+        ref_states[m] = (o.download(0, abi.BV), o.download(0, abi.YK))
     out = {}
+    params = None
     for name in ("wilke", "mlp"):
         g = solver(small_mixed_case())
         if name == "mlp":
             g.mlp_train()
+            params = g.mlp_get_params()
             g.set_transport(abi.TRANSPORT_MLP)
         g.upload_initial()
         g.init_conservative()
-        g.step(10)
-        bg, bo = g.download(0, abi.BV), o.download(0, abi.BV)
-        yg, yo = g.download(0, abi.YK), o.download(0, abi.YK)
-        err = {"rho": float(np.abs(bg[0] - bo[0]).max() / np.abs(bo[0]).max()),
-               "T": float(np.abs(bg[5] - bo[5]).max() / np.abs(bo[5]).max()),
-               "Y": float(np.abs(yg - yo).max())}
-        vel = np.sqrt((bo[1:4] ** 2).sum(axis=0)).max()
-        err["velocity"] = float(np.sqrt(((bg[1:4] - bo[1:4]) ** 2).sum(axis=0)).max() / vel)
-        g.set_profiling(True)
-        g.time_steps(3)
-        kt = g.kernel_times()
-        out[name] = {"err": err, "props_ms": kt["props"][0] / 3, "stats": g.mlp_stats() if name == "mlp" else None}
+        errs, done = {}, 0
+        for m in marks:
+            g.step(m - done)
+            done = m
+            bo, yo = ref_states[m]
+            bg, yg = g.download(0, abi.BV), g.download(0, abi.YK)
+            vel = np.sqrt((bo[1:4] ** 2).sum(axis=0)).max()
+            errs[m] = {"rho": float(np.abs(bg[0] - bo[0]).max() / np.abs(bo[0]).max()),
+                       "T": float(np.abs(bg[5] - bo[5]).max() / np.abs(bo[5]).max()),
+                       "Y": float(np.abs(yg - yo).max()),
+                       "velocity": float(np.sqrt(((bg[1:4] - bo[1:4]) ** 2).sum(axis=0)).max() / vel)}
+        out[name] = {"err": errs, "stats": g.mlp_stats() if name == "mlp" else None}
         rc, nfail = g.check()
         assert rc == 0 and nfail == 0
         g.close()
-    print(f"[mlp] ablation: {out}", flush=True)
-    for k, v in out["mlp"]["err"].items():
-        assert v <= 1e-4, (k, v)
+    # properties-pass time on a 128x64x64 block, trained network loaded
+    t = {}
+    for name in ("wilke", "mlp"):
+        c = cases.c1(128, 64, 64)
+        c.params["precision"] = abi.MIXED
+        if name == "mlp":
+            c.params["transport"] = abi.TRANSPORT_MLP
+        g = solver(c)
+        if name == "mlp":
+            g.mlp_set_params(*params)
+            g.set_transport(abi.TRANSPORT_MLP)
+        g.upload_initial()
+        g.init_conservative()
+        g.step(2)
+        g.set_profiling(True)
+        g.time_steps(5)
+        kt = g.kernel_times()
+        t[name] = kt["props"][0] / 5
+        g.close()
+    ratio = t["mlp"] / t["wilke"]
+    print(f"[mlp] ablation: {out}; props ms per step on 128x64x64: {t}, surrogate / Wilke {ratio:.3f}", flush=True)
+    for m in marks:
+        for k, v in out["mlp"]["err"][m].items():
+            assert v <= 1e-4, (m, k, v)
     st = out["mlp"]["stats"]
     assert st["checked"] > 0 and st["check_failed"] <= st["checked"] // 100 + 1
+    assert 1.05 <= ratio <= 2.0, ratio

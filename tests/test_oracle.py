@@ -17,7 +17,8 @@ at least one check:
                                        long-double brute force; rho*D closed form
   limiters (Limiter.cuh:13-16)           closed forms, symmetry
   MUSCL + AUSM+ (InviscidScheme.cu)    F(U,U) = physical flux |k|; supersonic
-                                       upwinding F = F(U_L); linear data
+                                       upwinding F = F(U_L); linear data;
+                                       first order takes the cell values
   metrics (Mesh.cpp:226-488)           Cartesian closed form; 2-D discrete GCL
   full step                            freestream preservation on a curved
                                        2-D grid; mass conservation in a closed
@@ -242,6 +243,26 @@ def test_ausm_supersonic_upwind(orc):
     jac = np.array([1.0, 1.0])
     f = orc.inviscid_face(None, {"reconstruction": 1, "limiter": 0, "r_air": 287.0}, pv, metric, jac)
     np.testing.assert_allclose(f, physical_flux(ql, [1.0, 0.0, 0.0], None), rtol=1e-14)
+
+
+def test_first_order_takes_the_cell_values(orc):
+    """First-order reconstruction (Reconstruction.cu:12-20): the face states
+    are the two adjacent cell values whatever the outer stencil points hold,
+    so the flux equals the MUSCL flux of data that is flat on each side."""
+    ql = np.array([1.0, 300.0, 20.0, -5.0, 1.0e5])
+    qr = np.array([0.9, 250.0, -10.0, 4.0, 0.9e5])
+    far_l = np.array([1.3, 100.0, 0.0, 0.0, 1.4e5])
+    far_r = np.array([0.5, 600.0, 30.0, 9.0, 0.5e5])
+    metric = np.array([[0.8, 0.3, 0.1], [0.7, 0.2, 0.2]])
+    jac = np.array([1.1, 0.9])
+    f1 = orc.inviscid_face(None, {"reconstruction": 1, "limiter": 1, "r_air": 287.0},
+                           np.array([far_l, ql, qr, far_r]), metric, jac)
+    f2 = orc.inviscid_face(None, {"reconstruction": 2, "limiter": 1, "r_air": 287.0},
+                           np.array([ql, ql, qr, qr]), metric, jac)
+    assert np.array_equal(f1, f2)
+    f3 = orc.inviscid_face(None, {"reconstruction": 2, "limiter": 1, "r_air": 287.0},
+                           np.array([far_l, ql, qr, far_r]), metric, jac)
+    assert not np.allclose(f1, f3)
 
 
 def test_muscl_reproduces_linear_data(orc):
