@@ -652,26 +652,88 @@ extern "C" {
 namespace cfx {
 namespace {
 
+// The (ngg+1)-layer coordinate arrays of every block built on the device
+// (Mesh::init_ghost_grid, Mesh.cpp:718-922): interior vertices uploaded and
+// scattered, mirror extrapolation per block, the inter-block copies in the
+// reference's block and face order, then edges and corners per block; the
+// arrays stay resident (CFX_COORD, the metric fallback).  A block whose inner
+// face names the block itself would read points the same copy writes, so
+// such meshes keep the host path (ctx->coords empty, hb[].xyz filled).
+int device_ghost_grid(cfx_ctx *ctx, const cfx_block *blocks, int n_block) {
+  const int nb = n_block;
+  ctx->coords.assign(nb, nullptr);
+  int rc = 0;
+  long long max_ni = 0;
+  for (int b = 0; b < nb; ++b) {
+    const HostBlock &H = ctx->hb[b];
+    ctx->coords[b] = ctx->dalloc<double>(3 * (size_t)H.gx * H.gy * H.gz, rc);
+    max_ni = std::max(max_ni, H.ni);
+  }
+  if (rc) return ctx->fail(rc, "hipMalloc coordinates");
+  double *raw = nullptr;
+  rc = (int)hipMalloc(&raw, sizeof(double) * 3 * (size_t)max_ni);
+  for (int b = 0; b < nb && !rc; ++b) {
+    const HostBlock &H = ctx->hb[b];
+    const size_t ni = (size_t)H.ni;
+    const double *src[3] = {blocks[b].x, blocks[b].y, blocks[b].z};
+    for (int c = 0; c < 3 && !rc; ++c) rc = (int)hipMemcpy(raw + c * ni, src[c], sizeof(double) * ni, hipMemcpyHostToDevice);
+    if (!rc) rc = launch_coord_scatter(raw, H.mx, H.my, H.mz, ctx->coords[b], nullptr);
+    if (!rc) rc = launch_ghost_extrapolate(ctx->coords[b], H.mx, H.my, H.mz, ctx->dim, nullptr);
+    if (!rc) rc = (int)hipDeviceSynchronize();  // raw is reused by the next block
+  }
+  (void)hipFree(raw);
+  if (rc) return ctx->fail(rc, "device ghost grid");
+  for (const GhostCopy &g : inner_ghost_copies(ctx->hb, blocks, ctx->dim)) {
+    const HostBlock &D = ctx->hb[g.blk], &S = ctx->hb[g.src];
+    if ((rc = launch_ghost_inner(g, ctx->coords[g.blk], D.mx, D.my, D.mz, ctx->coords[g.src], S.mx, S.my, S.mz,
+                                 nullptr)))
+      return ctx->fail(rc, "device inner ghost grid");
+  }
+  for (int b = 0; b < nb; ++b) {
+    const HostBlock &H = ctx->hb[b];
+    if ((rc = launch_ghost_corners(ctx->coords[b], H.mx, H.my, H.mz, nullptr))) return ctx->fail(rc, "device corners");
+  }
+  rc = (int)hipDeviceSynchronize();
+  return rc ? ctx->fail(rc, "device ghost grid") : 0;
+}
+
+// The host copy of block b's coordinates (downloaded from the device arrays
+// when the ghost grid was built there).
+int host_coords(cfx_ctx *ctx, int b) {
+  HostBlock &H = ctx->hb[b];
+  if (!H.xyz[0].empty() || ctx->coords.empty()) return 0;
+  const size_t ng = (size_t)H.gx * H.gy * H.gz;
+  for (int c = 0; c < 3; ++c) {
+    H.xyz[c].resize(ng);
+    if (int rc = (int)hipMemcpy(H.xyz[c].data(), ctx->coords[b] + c * ng, sizeof(double) * ng, hipMemcpyDeviceToHost))
+      return ctx->fail(rc, "coordinate download");
+  }
+  return 0;
+}
+
 // Metrics of one block computed on the device from its ghost-grid
 // coordinates; a block with a non-positive Jacobian anywhere is recomputed on
 // the host with the reference's fallback rules and uploaded.
-int device_metrics(cfx_ctx *ctx, HostBlock &H, const BlockView &V) {
-  const size_t ng = H.xyz[0].size(), np = (size_t)H.np;
-  double *xyz = nullptr;
+int device_metrics(cfx_ctx *ctx, int b, const BlockView &V) {
+  HostBlock &H = ctx->hb[b];
+  const size_t ng = (size_t)H.gx * H.gy * H.gz, np = (size_t)H.np;
+  const bool resident = !ctx->coords.empty();
+  double *xyz = resident ? ctx->coords[b] : nullptr;
   int *bad = nullptr;
-  int rc = (int)hipMalloc(&xyz, sizeof(double) * 3 * ng);
+  int rc = resident ? 0 : (int)hipMalloc(&xyz, sizeof(double) * 3 * ng);
   if (!rc) rc = (int)hipMalloc(&bad, sizeof(int));
-  for (int c = 0; c < 3 && !rc; ++c)
+  for (int c = 0; c < 3 && !rc && !resident; ++c)
     rc = (int)hipMemcpy(xyz + c * ng, H.xyz[c].data(), sizeof(double) * ng, hipMemcpyHostToDevice);
   if (!rc) rc = (int)hipMemset(bad, 0, sizeof(int));
   if (!rc) rc = launch_metrics(xyz, H.mx, H.my, H.mz, V.metric, V.jac, V.mixed ? V.metric32 : nullptr,
                                V.mixed ? V.jac32 : nullptr, bad, nullptr);
   int n_bad = 0;
   if (!rc) rc = (int)hipMemcpy(&n_bad, bad, sizeof(int), hipMemcpyDeviceToHost);
-  (void)hipFree(xyz);
+  if (!resident) (void)hipFree(xyz);
   (void)hipFree(bad);
   if (rc) return ctx->fail(rc, "device metrics");
   if (n_bad == 0) return 0;
+  if ((rc = host_coords(ctx, b))) return rc;
   if ((rc = compute_metrics(H, ctx->err))) return rc;
   rc = (int)hipMemcpy(V.metric, H.metric.data(), sizeof(double) * 9 * np, hipMemcpyHostToDevice);
   if (!rc) rc = (int)hipMemcpy(V.jac, H.jac.data(), sizeof(double) * np, hipMemcpyHostToDevice);
@@ -684,6 +746,8 @@ int device_metrics(cfx_ctx *ctx, HostBlock &H, const BlockView &V) {
   }
   std::vector<double>().swap(H.metric);
   std::vector<double>().swap(H.jac);
+  if (resident)
+    for (int c = 0; c < 3; ++c) std::vector<double>().swap(H.xyz[c]);
   return rc ? ctx->fail(rc, "metric upload") : 0;
 }
 
@@ -758,7 +822,10 @@ int cfx_create(const cfx_params *params, const cfx_species *species, const cfx_b
     ctx->err = "no FP32 kernels built for " + std::to_string(ctx->ns) + " species";
     return bail(CFX_E_UNSUPPORTED);
   }
-  rc = build_mesh(blocks, n_block, ctx->hb, ctx->dim, ctx->err, /*metrics=*/false);
+  bool self_face = false;  // an inner face that names its own block keeps the host ghost grid
+  for (int b = 0; b < n_block; ++b)
+    for (int f = 0; f < blocks[b].n_inner; ++f) self_face |= blocks[b].inner[f].target_block == b + 1;
+  rc = build_mesh(blocks, n_block, ctx->hb, ctx->dim, ctx->err, /*metrics=*/false, /*ghosts=*/self_face);
   if (rc) return bail(rc);
   SchemeDev &S = ctx->S;
   S.reconstruction = params->reconstruction;
@@ -874,12 +941,14 @@ int cfx_create(const cfx_params *params, const cfx_species *species, const cfx_b
       (void)hipMemset(V.props32, 0, sizeof(float) * (4 + ctx->ns) * np);
       (void)hipMemset(V.dq32, 0, sizeof(float) * ctx->nv * ni);
     }
-    if ((rc = device_metrics(ctx, ctx->hb[b], V))) return bail(rc);
     (void)hipMemset(V.props, 0, sizeof(double) * (4 + ctx->ns) * np);
     (void)hipMemset(V.dq, 0, sizeof(double) * ctx->nv * ni);
     ctx->views.push_back(V);
     ctx->n_grid_total += H.ni;
   }
+  if (!self_face && (rc = device_ghost_grid(ctx, blocks, n_block))) return bail(rc);
+  for (int b = 0; b < n_block; ++b)
+    if ((rc = device_metrics(ctx, b, ctx->views[b]))) return bail(rc);
   ctx->views_dev = ctx->dalloc<BlockView>(ctx->views.size(), rc);
   if (rc) return bail(ctx->fail(rc, "hipMalloc block table"));
   (void)hipMemcpy(ctx->views_dev, ctx->views.data(), sizeof(BlockView) * ctx->views.size(), hipMemcpyHostToDevice);
@@ -986,6 +1055,7 @@ int cfx_download(cfx_ctx *ctx, int blk, int what, double *dst) {
       CFX_TRY((int)hipMemcpy(dst + 9 * np, V.jac, sizeof(double) * np, hipMemcpyDeviceToHost));
       return 0;
     case CFX_COORD: {
+      CFX_TRY(host_coords(ctx, blk));
       const HostBlock &B = ctx->hb[blk];
       for (int c = 0; c < 3; ++c)
         for (int k = -kNgg; k < B.mz + kNgg; ++k)

@@ -69,6 +69,7 @@ struct cfx_ctx {
   int n_partial = 0;
   long long n_grid_total = 0;
   BlockView *views_dev = nullptr;
+  std::vector<double *> coords;  // per block: the (ngg+1)-layer coordinates built on the device
   Table t_pad, t_int, t_dt, t_last, t_res;     // point-kernel workgroup prefixes
   Table t_plane;                               // LU-SGS sweeps: my * mz columns per block
   int n_planes = 0;                            // LU-SGS hyperplanes of the largest block

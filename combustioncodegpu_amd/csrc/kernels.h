@@ -113,6 +113,16 @@ int launch_store_last(const Batch &b, hipStream_t st);
 // metric32 is non-null; *bad counts points with a non-positive Jacobian
 int launch_metrics(const double *xyz, int mx, int my, int mz, double *metric, double *jac, float *metric32,
                    float *jac32, int *bad, hipStream_t st);
+// The ghost grid of Mesh::init_ghost_grid on the device, over the (ngg+1)-
+// layer coordinate array of a block (3 planes of (mx+2ngg+2)(my+2ngg+2)(mz+2ngg+2)):
+// raw (3 x mx*my*mz, i fastest) scattered into it; the mirror extrapolation
+// (and the 2-D z layers); one inter-block copy; the edges and corners.
+struct GhostCopy;
+int launch_coord_scatter(const double *raw, int mx, int my, int mz, double *xyz, hipStream_t st);
+int launch_ghost_extrapolate(double *xyz, int mx, int my, int mz, int dim, hipStream_t st);
+int launch_ghost_inner(const GhostCopy &g, double *dst, int dmx, int dmy, int dmz, const double *src, int smx,
+                       int smy, int smz, hipStream_t st);
+int launch_ghost_corners(double *xyz, int mx, int my, int mz, hipStream_t st);
 // partial sums of workgroup w go to partial[4 * (w)] with w counted from 0 of
 // the full prefix, so a per-block launch and a batched one fill the same slots
 int launch_residual_partial(const Batch &b, double *partial, hipStream_t st);

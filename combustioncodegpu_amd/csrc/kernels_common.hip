@@ -251,6 +251,203 @@ __global__ void __launch_bounds__(kTpb) metric_kernel(const double *__restrict__
   }
 }
 
+// ------------------------------------------------------------ ghost grid ---
+// The (ngg+1)-layer coordinate arrays of Mesh::init_ghost_grid
+// (Mesh.cpp:718-922) built on the device, each step in the host restatement's
+// arithmetic and, where one value feeds another, in its order: a thread owns
+// a whole line or copy point, so the result is bit-identical to the host path.
+// Layout: 3 planes (x, y, z) of gx * gy * gz, gx = mx + 2(ngg+1), i fastest.
+__device__ __forceinline__ long long gidx(int i, int j, int k, int gx, int gy) {
+  constexpr int G = kNgg + 1;
+  return ((long long)(k + G) * gy + (j + G)) * gx + (i + G);
+}
+
+// interior vertices (3 x mx*my*mz, i fastest) into the padded array
+__global__ void __launch_bounds__(kTpb) coord_scatter_kernel(const double *__restrict__ raw, int mx, int my, int mz,
+                                                              double *__restrict__ xyz) {
+  constexpr int G = kNgg + 1;
+  const long long ni = (long long)mx * my * mz;
+  const long long s = (long long)blockIdx.x * kTpb + threadIdx.x;
+  if (s >= ni) return;
+  const int gx = mx + 2 * G, gy = my + 2 * G;
+  const long long ng = (long long)gx * gy * (mz + 2 * G);
+  const int i = (int)(s % mx);
+  const long long r = s / mx;
+  const int j = (int)(r % my), k = (int)(r / my);
+  const long long g = gidx(i, j, k, gx, gy);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) xyz[c * ng + g] = raw[c * ni + s];
+}
+
+// Mirror extrapolation along each direction a over the interior of the other
+// two (Mesh.cpp:720-768): one thread per line, layers g = 1..ngg+1 in order.
+__global__ void __launch_bounds__(kTpb) ghost_extrapolate_kernel(double *__restrict__ xyz, int mx, int my, int mz,
+                                                                 int dim) {
+  constexpr int G = kNgg + 1;
+  const int m[3] = {mx, my, mz};
+  const int gx = mx + 2 * G, gy = my + 2 * G;
+  const long long ng = (long long)gx * gy * (mz + 2 * G);
+  long long l = (long long)blockIdx.x * kTpb + threadIdx.x;
+  int a = 0;
+  for (; a < dim; ++a) {
+    const long long nl = (long long)m[(a + 1) % 3] * m[(a + 2) % 3];
+    if (l < nl) break;
+    l -= nl;
+  }
+  if (a >= dim) return;
+  const int b = (a + 1) % 3, c = (a + 2) % 3;
+  const int u = (int)(l / m[c]), v = (int)(l % m[c]);
+  int p[3];
+  p[b] = u;
+  p[c] = v;
+  auto at = [&](int cc, int s) -> double & {
+    p[a] = s;
+    return xyz[cc * ng + gidx(p[0], p[1], p[2], gx, gy)];
+  };
+  for (int g = 1; g <= G; ++g)
+    for (int cc = 0; cc < 3; ++cc) {
+      at(cc, -g) = 2 * at(cc, 0) - at(cc, g);
+      at(cc, m[a] - 1 + g) = 2 * at(cc, m[a] - 1) - at(cc, m[a] - 1 - g);
+    }
+}
+
+// 2-D: unit spacing in z (Mesh.cpp:769-785), one thread per (i, j)
+__global__ void __launch_bounds__(kTpb) ghost_z2d_kernel(double *__restrict__ xyz, int mx, int my) {
+  constexpr int G = kNgg + 1;
+  const int gx = mx + 2 * G, gy = my + 2 * G, mz = 1;
+  const long long ng = (long long)gx * gy * (mz + 2 * G);
+  const long long s = (long long)blockIdx.x * kTpb + threadIdx.x;
+  if (s >= (long long)mx * my) return;
+  const int i = (int)(s % mx), j = (int)(s / mx);
+  auto X = [&](int cc, int k) -> double & { return xyz[cc * ng + gidx(i, j, k, gx, gy)]; };
+  for (int g = 1; g <= G; ++g) {
+    X(0, -g) = X(0, 0);
+    X(1, -g) = X(1, 0);
+    X(2, -g) = X(2, 0) - g;
+    X(0, mz - 1 + g) = X(0, mz - 1);
+    X(1, mz - 1 + g) = X(1, mz - 1);
+    X(2, mz - 1 + g) = X(2, mz - 1) + g;
+  }
+}
+
+// One inter-block copy (Mesh.cpp:874-922): dst point lo + step * d from src
+// point t0 + tstep * d[src_tar]; one thread per point.
+struct GhostCopyArgs {
+  int lo[3], n[3], step[3], t0[3], tstep[3], src_tar[3];
+  int dgx, dgy, sgx, sgy;
+  long long dng, sng;
+};
+__global__ void __launch_bounds__(kTpb) ghost_inner_kernel(double *__restrict__ dst, const double *__restrict__ src,
+                                                           GhostCopyArgs A) {
+  const long long t = (long long)blockIdx.x * kTpb + threadIdx.x;
+  if (t >= (long long)A.n[0] * A.n[1] * A.n[2]) return;
+  const int c = (int)(t % A.n[2]);
+  const long long r = t / A.n[2];
+  const int b = (int)(r % A.n[1]), a = (int)(r / A.n[1]);
+  const int di[3] = {A.step[0] * a, A.step[1] * b, A.step[2] * c};
+  const int ts[3] = {A.t0[0] + A.tstep[0] * di[A.src_tar[0]], A.t0[1] + A.tstep[1] * di[A.src_tar[1]],
+                     A.t0[2] + A.tstep[2] * di[A.src_tar[2]]};
+  const long long gd = gidx(A.lo[0] + di[0], A.lo[1] + di[1], A.lo[2] + di[2], A.dgx, A.dgy);
+  const long long gs = gidx(ts[0], ts[1], ts[2], A.sgx, A.sgy);
+#pragma unroll
+  for (int cc = 0; cc < 3; ++cc) dst[cc * A.dng + gd] = src[cc * A.sng + gs];
+}
+
+// Edges and corners by the parallelogram rule (Mesh.cpp:793-867), three
+// phases; every point a phase reads was written by an earlier step.
+template <int PHASE>
+__global__ void __launch_bounds__(kTpb) ghost_corner_kernel(double *__restrict__ xyz, int mx, int my, int mz) {
+  constexpr int G = kNgg + 1;
+  const int gx = mx + 2 * G, gy = my + 2 * G;
+  const long long ng = (long long)gx * gy * (mz + 2 * G);
+  const long long t = (long long)blockIdx.x * kTpb + threadIdx.x;
+  // PHASE 0: (s, t, k); 1: (s, j, t); 2: (i, s, t)
+  const long long n0 = PHASE == 2 ? (long long)(mx + kNgg + G + 1) : G;
+  const long long n1 = PHASE == 1 ? my : G;
+  const long long n2 = PHASE == 0 ? mz : G;
+  if (t >= n0 * n1 * n2) return;
+  const int x2 = (int)(t % n2);
+  const long long r = t / n2;
+  const int x1 = (int)(r % n1), x0 = (int)(r / n1);
+  for (int cc = 0; cc < 3; ++cc) {
+    auto X = [&](int i, int j, int k) -> double & { return xyz[cc * ng + gidx(i, j, k, gx, gy)]; };
+    if constexpr (PHASE == 0) {
+      const int s = x0 + 1, tt = x1 + 1, k = x2;
+      const int i1 = -s, j1 = -tt, i2 = mx - 1 + s, j2 = my - 1 + tt;
+      X(i1, j1, k) = X(i1, 0, k) + X(0, j1, k) - X(0, 0, k);
+      X(i2, j2, k) = X(i2, my - 1, k) + X(mx - 1, j2, k) - X(mx - 1, my - 1, k);
+      X(i1, j2, k) = X(i1, my - 1, k) + X(0, j2, k) - X(0, my - 1, k);
+      X(i2, j1, k) = X(i2, 0, k) + X(mx - 1, j1, k) - X(mx - 1, 0, k);
+    } else if constexpr (PHASE == 1) {
+      const int s = x0 + 1, j = x1, tt = x2 + 1;
+      const int i1 = -s, k1 = -tt, i2 = mx - 1 + s, k2 = mz - 1 + tt;
+      X(i1, j, k1) = X(i1, j, 0) + X(0, j, k1) - X(0, j, 0);
+      X(i2, j, k2) = X(i2, j, mz - 1) + X(mx - 1, j, k2) - X(mx - 1, j, mz - 1);
+      X(i1, j, k2) = X(i1, j, mz - 1) + X(0, j, k2) - X(0, j, mz - 1);
+      X(i2, j, k1) = X(i2, j, 0) + X(mx - 1, j, k1) - X(mx - 1, j, 0);
+    } else {
+      const int i = x0 - G, s = x1 + 1, tt = x2 + 1;
+      const int j1 = -s, k1 = -tt, j2 = my - 1 + s, k2 = mz - 1 + tt;
+      X(i, j1, k1) = X(i, j1, 0) + X(i, 0, k1) - X(i, 0, 0);
+      X(i, j2, k2) = X(i, j2, mz - 1) + X(i, my - 1, k2) - X(i, my - 1, mz - 1);
+      X(i, j1, k2) = X(i, j1, mz - 1) + X(i, 0, k2) - X(i, 0, mz - 1);
+      X(i, j2, k1) = X(i, j2, 0) + X(i, my - 1, k1) - X(i, my - 1, 0);
+    }
+  }
+}
+
+namespace {
+unsigned grid_of(long long n) { return (unsigned)((n + kTpb - 1) / kTpb); }
+}  // namespace
+
+int launch_coord_scatter(const double *raw, int mx, int my, int mz, double *xyz, hipStream_t st) {
+  const long long ni = (long long)mx * my * mz;
+  if (ni <= 0) return 0;
+  coord_scatter_kernel<<<grid_of(ni), kTpb, 0, st>>>(raw, mx, my, mz, xyz);
+  return (int)hipGetLastError();
+}
+
+int launch_ghost_extrapolate(double *xyz, int mx, int my, int mz, int dim, hipStream_t st) {
+  const long long nl = (long long)my * mz + (long long)mz * mx + (dim == 3 ? (long long)mx * my : 0);
+  ghost_extrapolate_kernel<<<grid_of(nl), kTpb, 0, st>>>(xyz, mx, my, mz, dim);
+  if (hipError_t e = hipGetLastError()) return (int)e;
+  if (dim == 2) {
+    ghost_z2d_kernel<<<grid_of((long long)mx * my), kTpb, 0, st>>>(xyz, mx, my);
+    return (int)hipGetLastError();
+  }
+  return 0;
+}
+
+int launch_ghost_inner(const GhostCopy &g, double *dst, int dmx, int dmy, int dmz, const double *src, int smx,
+                       int smy, int smz, hipStream_t st) {
+  constexpr int G = kNgg + 1;
+  GhostCopyArgs A{};
+  for (int q = 0; q < 3; ++q) {
+    A.lo[q] = g.lo[q];
+    A.n[q] = g.n[q];
+    A.step[q] = g.step[q];
+    A.t0[q] = g.t0[q];
+    A.tstep[q] = g.tstep[q];
+    A.src_tar[q] = g.src_tar[q];
+  }
+  A.dgx = dmx + 2 * G, A.dgy = dmy + 2 * G, A.dng = (long long)A.dgx * A.dgy * (dmz + 2 * G);
+  A.sgx = smx + 2 * G, A.sgy = smy + 2 * G, A.sng = (long long)A.sgx * A.sgy * (smz + 2 * G);
+  const long long n = (long long)g.n[0] * g.n[1] * g.n[2];
+  if (n <= 0) return 0;
+  ghost_inner_kernel<<<grid_of(n), kTpb, 0, st>>>(dst, src, A);
+  return (int)hipGetLastError();
+}
+
+int launch_ghost_corners(double *xyz, int mx, int my, int mz, hipStream_t st) {
+  constexpr int G = kNgg + 1;
+  ghost_corner_kernel<0><<<grid_of((long long)G * G * mz), kTpb, 0, st>>>(xyz, mx, my, mz);
+  if (hipError_t e = hipGetLastError()) return (int)e;
+  ghost_corner_kernel<1><<<grid_of((long long)G * my * G), kTpb, 0, st>>>(xyz, mx, my, mz);
+  if (hipError_t e = hipGetLastError()) return (int)e;
+  ghost_corner_kernel<2><<<grid_of((long long)(mx + kNgg + G + 1) * G * G), kTpb, 0, st>>>(xyz, mx, my, mz);
+  return (int)hipGetLastError();
+}
+
 int launch_metrics(const double *xyz, int mx, int my, int mz, double *metric, double *jac, float *metric32,
                    float *jac32, int *bad, hipStream_t st) {
   const long long np = (long long)(mx + 2 * kNgg) * (my + 2 * kNgg) * (mz + 2 * kNgg);

@@ -252,34 +252,55 @@ void corner_ghost_grid(HostBlock &B) {
 // Inter-block ghost grid (Mesh.cpp:874-922): the ngg+1 layers beyond the
 // shared face come from the neighbour's points next to its face.
 void inner_ghost_grid(std::vector<HostBlock> &blocks, const cfx_block *desc, int dim) {
+  for (const GhostCopy &g : inner_ghost_copies(blocks, desc, dim)) {
+    HostBlock &B = blocks[g.blk];
+    const HostBlock &N = blocks[g.src];
+    for (int a = 0; a < g.n[0]; ++a)
+      for (int b = 0; b < g.n[1]; ++b)
+        for (int c = 0; c < g.n[2]; ++c) {
+          const int di[3] = {g.step[0] * a, g.step[1] * b, g.step[2] * c};
+          const int t[3] = {g.t0[0] + g.tstep[0] * di[g.src_tar[0]], g.t0[1] + g.tstep[1] * di[g.src_tar[1]],
+                            g.t0[2] + g.tstep[2] * di[g.src_tar[2]]};
+          for (int cc = 0; cc < 3; ++cc)
+            B.coord(cc, g.lo[0] + di[0], g.lo[1] + di[1], g.lo[2] + di[2]) = N.xyz[cc][N.gidx(t[0], t[1], t[2])];
+        }
+  }
+}
+
+}  // namespace
+
+std::vector<GhostCopy> inner_ghost_copies(const std::vector<HostBlock> &blocks, const cfx_block *desc, int dim) {
+  std::vector<GhostCopy> out;
   for (size_t b = 0; b < blocks.size(); ++b) {
-    HostBlock &B = blocks[b];
+    const HostBlock &B = blocks[b];
     for (size_t f = 0; f < B.inner.size(); ++f) {
       const InnerDev &fc = B.inner[f];
       const InnerGeom tg = inner_target_geom(desc[b].inner[f], dim);
-      HostBlock &N = blocks[fc.target_block];
-      int lo[3], hi[3], t0[3];
+      GhostCopy g{};
+      g.blk = (int)b;
+      g.src = fc.target_block;
+      int hi[3];
       for (int q = 0; q < 3; ++q) {
-        lo[q] = fc.rs[q];
+        g.lo[q] = fc.rs[q];
         hi[q] = fc.re[q];
-        t0[q] = fc.ts[q];
+        g.t0[q] = fc.ts[q];
+        g.step[q] = fc.loop_dir[q];
+        g.tstep[q] = fc.target_loop_dir[q];
+        g.src_tar[q] = fc.src_tar[q];
       }
       const int shift = fc.direction < 0 ? -1 : 1;
-      lo[fc.face] += shift;
+      g.lo[fc.face] += shift;
       hi[fc.face] += shift;
-      t0[tg.tface] += tg.tdirection > 0 ? -1 : 1;
-      for (int i = lo[0]; fc.loop_dir[0] * (i - hi[0]) != 1; i += fc.loop_dir[0])
-        for (int j = lo[1]; fc.loop_dir[1] * (j - hi[1]) != 1; j += fc.loop_dir[1])
-          for (int k = lo[2]; fc.loop_dir[2] * (k - hi[2]) != 1; k += fc.loop_dir[2]) {
-            const int di[3] = {i - lo[0], j - lo[1], k - lo[2]};
-            const int t[3] = {t0[0] + fc.target_loop_dir[0] * di[fc.src_tar[0]],
-                              t0[1] + fc.target_loop_dir[1] * di[fc.src_tar[1]],
-                              t0[2] + fc.target_loop_dir[2] * di[fc.src_tar[2]]};
-            for (int cc = 0; cc < 3; ++cc) B.coord(cc, i, j, k) = N.coord(cc, t[0], t[1], t[2]);
-          }
+      g.t0[tg.tface] += tg.tdirection > 0 ? -1 : 1;
+      // the loop runs lo, lo + step, ... until it has passed hi
+      for (int q = 0; q < 3; ++q) g.n[q] = g.step[q] * (hi[q] - g.lo[q]) + 1;
+      out.push_back(g);
     }
   }
+  return out;
 }
+
+namespace {
 
 bool metric_at(HostBlock &B, int i, int j, int k, double &J, double m[9]) {
   const auto X = [&](int cc, int di, int dj, int dk) { return B.coord(cc, i + di, j + dj, k + dk); };
@@ -331,7 +352,7 @@ int compute_metrics(HostBlock &B, std::string &err) {
 }
 
 int build_mesh(const cfx_block *blocks, int n_block, std::vector<HostBlock> &out, int &dim, std::string &err,
-               bool metrics) {
+               bool metrics, bool ghosts) {
   if (!blocks || n_block <= 0) {
     err = "no blocks";
     return CFX_E_ARG;
@@ -358,15 +379,17 @@ int build_mesh(const cfx_block *blocks, int n_block, std::vector<HostBlock> &out
     B.gy = d.my + 2 * (kNgg + 1);
     B.gz = d.mz + 2 * (kNgg + 1);
     const long long ng = (long long)B.gx * B.gy * B.gz;
-    for (int c = 0; c < 3; ++c) B.xyz[c].assign(ng, 0.0);
-    for (int k = 0; k < d.mz; ++k)
-      for (int j = 0; j < d.my; ++j)
-        for (int i = 0; i < d.mx; ++i) {
-          const long long s = ((long long)k * d.my + j) * d.mx + i;
-          B.coord(0, i, j, k) = d.x[s];
-          B.coord(1, i, j, k) = d.y[s];
-          B.coord(2, i, j, k) = d.z[s];
-        }
+    if (ghosts) {
+      for (int c = 0; c < 3; ++c) B.xyz[c].assign(ng, 0.0);
+      for (int k = 0; k < d.mz; ++k)
+        for (int j = 0; j < d.my; ++j)
+          for (int i = 0; i < d.mx; ++i) {
+            const long long s = ((long long)k * d.my + j) * d.mx + i;
+            B.coord(0, i, j, k) = d.x[s];
+            B.coord(1, i, j, k) = d.y[s];
+            B.coord(2, i, j, k) = d.z[s];
+          }
+    }
     for (int f = 0; f < d.n_bc; ++f) {
       const cfx_bc &bc = d.bc[f];
       if (bc.type != CFX_BC_WALL && bc.type != CFX_BC_SYMMETRY && bc.type != CFX_BC_INFLOW &&
@@ -386,6 +409,7 @@ int build_mesh(const cfx_block *blocks, int n_block, std::vector<HostBlock> &out
       B.inner.push_back(register_inner_face(d.inner[f], dim));
     }
   }
+  if (!ghosts) return 0;
   for (auto &B : out) extrapolate_ghost_grid(B, dim);
   inner_ghost_grid(out, blocks, dim);
   for (auto &B : out) corner_ghost_grid(B);

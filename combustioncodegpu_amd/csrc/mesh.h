@@ -66,12 +66,24 @@ __host__ __device__ inline bool metric_point(const C &X, double &J, double m[9])
   return true;
 }
 
-// Builds every block from the ABI descriptors: registers faces, extrapolates
-// the ghost grid (including inter-block ghost grid and corners) and, if
-// metrics is set, computes the metrics on the host.  Returns 0 or a CFX_E_*
-// code with a message in err.
+// One inter-block ghost-grid copy (Mesh.cpp:874-922): block blk's points
+// lo + step * d (d in [0, n)) take the coordinates of block src's points
+// t0 + tstep * d[src_tar], in the reference's block and face order.
+struct GhostCopy {
+  int blk, src;
+  int lo[3], n[3], step[3], t0[3], tstep[3], src_tar[3];
+};
+
+// Builds every block from the ABI descriptors: registers faces and, if
+// ghosts is set, fills the (ngg+1)-layer coordinate arrays: interior
+// vertices, the extrapolated ghost grid, the inter-block ghost grid and the
+// corners; if metrics is set too, computes the metrics on the host.  With
+// ghosts off the coordinate arrays stay empty (the device builds them).
+// Returns 0 or a CFX_E_* code with a message in err.
 int build_mesh(const cfx_block *blocks, int n_block, std::vector<HostBlock> &out, int &dim, std::string &err,
-               bool metrics = true);
+               bool metrics = true, bool ghosts = true);
+// The inter-block ghost-grid copies of registered blocks, in order.
+std::vector<GhostCopy> inner_ghost_copies(const std::vector<HostBlock> &blocks, const cfx_block *desc, int dim);
 // Host metrics of one block, with the reference's fallback for ghost points
 // whose Jacobian is not positive (copy from the neighbour towards the block).
 int compute_metrics(HostBlock &B, std::string &err);

@@ -60,17 +60,24 @@ def test_deck_without_rk_key_runs_the_reference_time_scheme(tmp_path):
     assert rc == 0 and nfail == 0
 
 
-def test_context_coordinates_match_host_ghost_grid():
-    """CFX_COORD of a context (the ghost grid its metrics were computed from)
-    equals the host-only cfx_ghost_grid, block by block."""
+@pytest.mark.parametrize("which", ["lattice", "lattice_ragged", "curved", "air_2d", "c3_reduced"])
+def test_context_coordinates_match_host_ghost_grid(which):
+    """CFX_COORD of a context (the ghost grid the device builds and the
+    metrics are computed from) equals the host-only cfx_ghost_grid, block by
+    block, bit for bit: lattices (inter-block copies with their corners), a
+    curved block, the 2-D z layers and the 64-block lattice."""
     import numpy as np
     from combustioncodegpu_amd import abi, cases, lib
-    case = cases.lattice([[7, 6], [6, 5], [5, 6]])
-    print("[deck] coordinates of a 2x2x2 lattice", flush=True)
+    case = {"lattice": lambda: cases.lattice([[7, 6], [6, 5], [5, 6]]),
+            "lattice_ragged": lambda: cases.lattice([[9, 3, 6], [5, 7], [4]]),
+            "curved": lambda: cases.c1(14, 10, 8, perturb=0.3),
+            "air_2d": lambda: cases.c0(mx=20, my=12),
+            "c3_reduced": lambda: cases.c3(lo=16, hi=24)}[which]()
+    print(f"[deck] coordinates of {which}: {len(case.blocks)} blocks", flush=True)
     host = lib.ghost_grid(case)
     g = lib.Solver(case)
     for b in range(len(case.blocks)):
-        assert np.array_equal(g.download(b, abi.COORD), host[b])
+        assert np.array_equal(g.download(b, abi.COORD), host[b]), b
     g.close()
 
 
