@@ -55,21 +55,74 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8}))
 
 
-def cpu_baseline(case, threads):
+def cpu_baseline(case, threads, steps=3):
+    """The oracle on the host cores: the median of `steps` SSP-RK3 steps of
+    the same block (about 3 s each on 16 threads)."""
+    import statistics
     from oracle import pyoracle
     pyoracle.build()
     used = pyoracle.set_threads(threads)
     o = pyoracle.Oracle(case)
     o.upload_initial()
     o.init_conservative()
-    t0 = time.perf_counter()
-    o.step(1)
-    dt = time.perf_counter() - t0
+    times = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        o.step(1)
+        times.append(time.perf_counter() - t0)
+    dt = statistics.median(times)
     cs = case.n_cells * 3
     o.close()
     return {"value": cs / dt / 1e6, "unit": "Mcell-stage updates/s", "cores": used, "kind": "port",
-            "sample": f"1 SSP-RK3 step (3 stages) of the same 256x128x128 9-species block, "
-                      f"{cs} cell-stages in {dt:.2f} s, OpenMP on {used} host threads"}
+            "sample": f"median of {steps} SSP-RK3 steps (3 stages each) of the same 256x128x128 9-species block, "
+                      f"{cs} cell-stages in {dt:.2f} s per step, OpenMP on {used} host threads (the host share "
+                      f"of one GPU)"}
+
+
+def measure_traffic(precision="fp64", timeout_s=240):
+    """HBM bytes of one RHS evaluation (three direction launches, default
+    tiles) measured in this run: two rocprofv3 counter passes over a child
+    `bench.py --rhs-only 2` (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch),
+    each scaled by the factor the same counter reads for a plain device copy
+    of known size (profiles/rhs_c1_<precision>_traffic.json: a 2 GiB float64
+    copy, measured once with `bench.py --copy-calibration`).  Returns bytes per
+    stage, or None when the profiler is not available."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3") or ("/opt/rocm/bin/rocprofv3" if os.path.exists("/opt/rocm/bin/rocprofv3")
+                                         else None)
+    tpath = os.path.join(ROOT, "profiles", f"rhs_c1_{precision}_traffic.json")
+    if prof is None or not os.path.exists(tpath):
+        return None
+    with open(tpath) as f:
+        cal = json.load(f)["calibration"]
+    scale = {"FETCH_SIZE": cal["read_bytes_per_FETCH_SIZE_KiB"], "WRITE_SIZE": cal["write_bytes_per_WRITE_SIZE_KiB"]}
+    total = 0.0
+    env = dict(os.environ, TMPDIR="/tmp")
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        out = tempfile.mkdtemp(prefix="cfx_traffic_", dir="/tmp")
+        cmd = ["timeout", "-s", "KILL", str(timeout_s), prof, "--kernel-trace", "--pmc", counter, "-d", out, "-o", "p",
+               "--output-format", "csv", "--", sys.executable, os.path.join(ROOT, "bench.py"), "--rhs-only", "2",
+               "--precision", precision]
+        r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True)
+        files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
+        if r.returncode != 0 and not files:
+            shutil.rmtree(out, ignore_errors=True)
+            return None
+        kib = 0.0
+        for fn in files:
+            with open(fn) as f:
+                for row in csv.DictReader(f):
+                    if "rhs_pencil_kernel" in row.get("Kernel_Name", "") and row.get("Counter_Name") == counter:
+                        kib += float(row["Counter_Value"])
+        shutil.rmtree(out, ignore_errors=True)
+        if kib <= 0:
+            return None
+        total += kib / 2 * scale[counter]  # two RHS evaluations
+    return total
 
 
 def prepare(case, args, name, precision="fp64", batched=True, graph=False, autotune=False, mlp=False, **_):
@@ -111,7 +164,7 @@ def prepare(case, args, name, precision="fp64", batched=True, graph=False, autot
     return g, info
 
 
-def measure_group(case, args, specs, rounds=5):
+def measure_group(case, args, specs, rounds=5, traffic=None):
     """Throughput of several execution forms of one case.  Every form is set
     up first; the timed region then interleaves them: in each of `rounds`
     rounds every form runs its share of args.steps steps between
@@ -171,14 +224,13 @@ def measure_group(case, args, specs, rounds=5):
         if kw.get("roofline", True):
             bpc = BYTES_PER_CELL_STAGE_FP64 if precision == "fp64" else BYTES_PER_CELL_STAGE_MIXED
             achieved = bpc * case.n_cells / rhs_stage_s / 1e9
-            traffic = None
-            tpath = os.path.join(ROOT, "profiles", f"rhs_c1_{precision}_traffic.json")
-            if os.path.exists(tpath):
-                with open(tpath) as f:
-                    traffic = json.load(f).get("hbm_bytes_per_stage")
+            # HBM bytes per RHS evaluation measured in this run with the
+            # default tiles (None for autotuned tiles, whose traffic differs)
+            tr = (traffic or {}).get(precision) if "tiles" not in info else None
             out["roofline"] = {
                 "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": tr,
+                "traffic_bytes_per_cell_stage": round(tr / case.n_cells, 1) if tr else None,
                 "kernel": "rhs_pencil_kernel (LDS-staged RHS, one launch per face direction)",
                 "kernel_avg_ms_per_stage": round(rhs_stage_s * 1e3, 4),
                 "launches_per_stage": round(rhs_n / stages, 2), "bytes_per_cell_stage": round(bpc, 2)}
@@ -199,6 +251,37 @@ def measure_group(case, args, specs, rounds=5):
         g.close()
         outs[name] = out
     return outs
+
+
+def lusgs_form(case, args, cfl=5.0, steps=10):
+    """The implicit LU-SGS step (local time step, CFL 5) on the same block,
+    FP64, batched and graph-captured: time per step (one stage: RHS, local
+    dt, coefficient kernel, forward and backward hyperplane sweeps, update,
+    ghost fill, properties) and per-kernel times; Mcell-updates/s counts
+    interior cells x steps."""
+    import copy
+    from combustioncodegpu_amd.lib import Solver
+    c = copy.copy(case)
+    c.params = dict(case.params, rk_stages=1, lusgs=1, lusgs_omega=1.0, cfl=cfl)
+    g = Solver(c)
+    g.set_execution(batched=True, graph=True)
+    g.upload_initial()
+    g.init_conservative()
+    g.step(2)
+    g.sync()
+    res0 = g.residual()
+    ms = g.time_steps(steps)
+    res1 = g.residual()
+    g.set_profiling(True)
+    g.time_steps(2)
+    kt = g.kernel_times()
+    rc, nfail = g.check()
+    g.close()
+    return {"ms_per_step": round(ms / steps, 4), "value": round(case.n_cells * steps / ms / 1e3, 3),
+            "unit": "Mcell updates/s (one implicit stage per step)", "cfl": cfl, "health_rc": rc,
+            "kernel_ms_per_step": {k: round(v[0] / 2, 4) for k, v in kt.items()},
+            "launches_per_step": {k: round(v[1] / 2, 1) for k, v in kt.items()},
+            "residual_before_after": [[float(x) for x in res0], [float(x) for x in res1]]}
 
 
 def mixed_errors(case, checkpoints=(1, 10, 100), mlp=True):
@@ -255,6 +338,8 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH/WRITE passes of the RHS")
+    ap.add_argument("--no-lusgs", action="store_true", help="skip the implicit LU-SGS form")
     ap.add_argument("--no-c3", action="store_true", help="skip the 64-block batched workload")
     ap.add_argument("--no-c4", action="store_true", help="skip the 384x256x256 surrogate workload")
     ap.add_argument("--no-mlp", action="store_true", help="skip every surrogate-transport form")
@@ -306,6 +391,11 @@ def main():
         g.close()
         return 0
 
+    traffic = {}
+    if not args.no_traffic and (mx, my, mz) == (256, 128, 128):
+        for prec in ("fp64", "mixed"):
+            log(f"[bench] HBM traffic of the {prec} RHS (two rocprofv3 counter passes)")
+            traffic[prec] = measure_traffic(prec)
     log(f"[bench] generating workload {mx}x{my}x{mz}, 9 species")
     t0 = time.perf_counter()
     case = cases.c1(mx, my, mz)
@@ -318,9 +408,10 @@ def main():
     if not args.no_mlp:
         specs.append(("mixed_graph_tuned_mlp", dict(precision="mixed", batched=True, graph=True, autotune=True,
                                                      mlp=True)))
-    forms = measure_group(case, args, specs)
+    forms = measure_group(case, args, specs, traffic=traffic)
     if forms is None:
         return 3
+    lusgs_out = None if args.no_lusgs else lusgs_form(case, args)
     log("[bench] mixed-precision error against the FP64 path")
     errors = mixed_errors(case, mlp=not args.no_mlp)
     c4_out = None
@@ -408,6 +499,7 @@ def main():
             "mixed_error_vs_fp64": errors,
             "c3_batched": c3_out,
             "c4_surrogate": c4_out,
+            "fp64_lusgs": lusgs_out,
         },
         "dt": head["dt"],
         "residual": head["residual"],

@@ -203,7 +203,7 @@ int build_tables(cfx_ctx *ctx) {
   ctx->n_planes = 0;
   for (int b = 0; b < nb; ++b) {
     const HostBlock &H = ctx->hb[b];
-    pln[b] = chunks((long long)H.my * H.mz, kTpb);
+    pln[b] = chunks((long long)H.my * H.mz, kSweepTpb);
     ctx->n_planes = std::max(ctx->n_planes, H.mx + H.my + H.mz - 2);
     pad[b] = chunks(H.np, kTpb);
     in1[b] = chunks(H.ni, kTpb);
@@ -844,6 +844,14 @@ int cfx_create(const cfx_params *params, const cfx_species *species, const cfx_b
   S.lusgs = params->lusgs ? 1 : 0;
   S.omega = params->lusgs_omega > 0 ? params->lusgs_omega : 1.0;
   ctx->rk = S.rk;
+  // default RHS tiles of a 3-D context: shapes among the fastest the
+  // autotuner measures on MI355X for the 256x128x128 9-species block (FP64:
+  // x 32x8x1, y 16x16x1, z 8x32x1 with c fastest; FP32: x 32x4x2 at 4 waves,
+  // y 8x32x1, z 8x32x1 with c fastest); cfx_autotune re-picks per context
+  if (ctx->dim == 3) {
+    const int d64[3] = {4, 4, 1}, d32[3] = {5, 2, 1};
+    for (int d = 0; d < 3; ++d) ctx->rhs_var[d] = ctx->mixed ? d32[d] : d64[d];
+  }
   ctx->lusgs = S.lusgs != 0;
   if (ctx->lusgs && (S.rk != 1 || ctx->mixed)) {
     ctx->err = "the LU-SGS step needs rk_stages 1 (local time step) and FP64";

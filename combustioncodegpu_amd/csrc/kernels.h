@@ -19,6 +19,7 @@ constexpr int kTpb = 256;        // threads per workgroup of the point kernels
 constexpr int kDtIter = 8;       // points per thread of the time-step reduction
 constexpr int kLastIter = 4;     // points per thread of the last-step snapshot
 constexpr int kResIter = 8;      // points per thread of the residual partials
+constexpr int kSweepTpb = 64;    // threads per workgroup of the LU-SGS hyperplane sweeps
 
 // A batch of blocks: views[0..nb) with workgroup prefix off[0..nb] (device
 // pointers, off[0] need not be 0) and total = off[nb] - off[0] workgroups.

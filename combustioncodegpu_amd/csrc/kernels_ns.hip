@@ -741,15 +741,19 @@ __device__ __forceinline__ void lusgs_flux_change(const GView &B, const SchemeDe
 
 // One hyperplane of a sweep: thread t of a block is the column (j, k) =
 // (t % my, t / my), its cell on the plane i = plane - j - k (if inside).
+// A plane holds at most my * mz cells, too few to fill the device with
+// kTpb-thread workgroups, so the sweeps use one wavefront per workgroup
+// (kSweepTpb), which spreads a plane over four times as many CUs.
 template <bool FWD>
-__global__ void __launch_bounds__(kTpb) lusgs_sweep_kernel(const BlockView *__restrict__ views,
-                                                           const int *__restrict__ off, int nb, SchemeDev S,
-                                                           int plane) {
+__global__ void __launch_bounds__(kSweepTpb) lusgs_sweep_kernel(const BlockView *__restrict__ views,
+                                                                const int *__restrict__ off, int nb, SchemeDev S,
+                                                                int plane) {
   constexpr int NV = 5 + NS;
-  const PointWork pw = point_work(off, nb);
-  const GView B = gview(views[pw.blk]);
+  const int w = blockIdx.x + off[0];
+  const int blk = find_seg(off, nb, w);
+  const GView B = gview(views[blk]);
   const int mx = B.mx, my = B.my, mz = B.mz;
-  const long long t = pw.base + threadIdx.x;
+  const long long t = (long long)(w - off[blk]) * kSweepTpb + threadIdx.x;
   if (t >= (long long)my * mz) return;
   const int j = (int)(t % my), k = (int)(t / my);
   const int i = plane - j - k;
@@ -791,9 +795,9 @@ int launch_lusgs_coef(const Batch &b, const SchemeDev &S, hipStream_t st) {
 int launch_lusgs_sweep(const Batch &b, const SchemeDev &S, int plane, int fwd, hipStream_t st) {
   if (b.total <= 0) return 0;
   if (fwd)
-    lusgs_sweep_kernel<true><<<b.total, kTpb, 0, st>>>(b.views, b.off, b.nb, S, plane);
+    lusgs_sweep_kernel<true><<<b.total, kSweepTpb, 0, st>>>(b.views, b.off, b.nb, S, plane);
   else
-    lusgs_sweep_kernel<false><<<b.total, kTpb, 0, st>>>(b.views, b.off, b.nb, S, plane);
+    lusgs_sweep_kernel<false><<<b.total, kSweepTpb, 0, st>>>(b.views, b.off, b.nb, S, plane);
   return (int)hipGetLastError();
 }
 
@@ -940,7 +944,10 @@ __device__ __forceinline__ bool mlp_phase1(const GView &B, SpeciesPtrT<float> sp
   const long long np = B.np;
   float cp[NSA], winv, gc[2], x[kMlpIO];
   mixture_gamma_c<float>(sp, b32[5], Y, cp, winv, gc);
-  const bool outside = mlp_inputs<NS>(mlp, b32[5], flog<float>(b32[4]), Y, x);
+  float ymax = 0.0f;  // largest Y_l / W_l: the largest mole fraction times winv
+#pragma unroll
+  for (int l = 0; l < NS; ++l) ymax = fmaxf(ymax, Y[l] * sp->inv_mw[l]);
+  const bool outside = mlp_inputs<NS>(mlp, b32[5], flog<float>(b32[4]), Y, x, ymax * frcp<float>(winv));
 #pragma unroll
   for (int f = 0; f < kMlpIO; ++f) xl[lane][f] = x[f];
   if (store) {

@@ -35,6 +35,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kMlpH = 64;     // hidden width
 constexpr int kMlpIO = 16;    // padded input/output width (ns + 2 <= 16)
 constexpr int kMlpMaxSp = kMlpIO - 2;
+constexpr float kMlpPureX = 0.98f;  // largest mole fraction the trained surrogate answers
 
 // Packed A operands, in blocks of 64 lanes x 8 bf16 (one MFMA's A operand):
 //   forward  W1: 4 (M)            W2, W3: 4 x 2 (M x K)     W4: 2 (K)
@@ -75,6 +76,11 @@ struct MlpDev {
   // (quad = P^T diag(inv_var) P with the off-diagonal terms doubled)
   float quad[kMlpIO][kMlpIO];
   float d2_max;
+  // near-pure mixtures: the transport routine switches rho*D_l to mu/Sc at
+  // |X_l - 1| < 1e-3 (Transport.cu:88-91), a jump by the ratio of the other
+  // species' mean molecular weight to W_l that a smooth network cannot
+  // follow; points with a mole fraction >= x_pure take the Wilke routine
+  float x_pure;
   float check_tol;            // relative error bound of the sampled Wilke check
   int check_every;            // one wavefront in check_every is checked
   int ready;
@@ -86,7 +92,7 @@ struct MlpDev {
 // training inputs beyond the largest one seen in training.
 template <int NSP>
 __device__ __forceinline__ bool mlp_inputs(const MlpDev *__restrict__ m, float t, float lp, const float *Y,
-                                           float x[kMlpIO]) {
+                                           float x[kMlpIO], float x_max = 0.0f) {
   x[0] = (t - m->x_mean[0]) * m->x_inv_std[0];
   x[1] = (lp - m->x_mean[1]) * m->x_inv_std[1];
 #pragma unroll
@@ -100,7 +106,7 @@ __device__ __forceinline__ bool mlp_inputs(const MlpDev *__restrict__ m, float t
     d2 += x[i] * s;
   }
   const bool box = t >= m->t_lo && t <= m->t_hi && lp >= m->lp_lo && lp <= m->lp_hi;
-  return !box || !(d2 <= m->d2_max);
+  return !box || !(d2 <= m->d2_max) || !(x_max < m->x_pure);
 }
 
 // ----------------------------------------------------------- activations ---

@@ -112,6 +112,7 @@ void open_envelope(MlpDev &m) {
     for (int i = 0; i < kMlpIO; ++i) m.pca[k][i] = k == i ? 1.0f : 0.0f;
   }
   m.d2_max = 1e30f;
+  m.x_pure = 2.0f;
 }
 
 // the envelope distance in the principal axes as one quadratic form
@@ -267,6 +268,7 @@ int cfx_mlp_train(cfx_ctx *ctx, const cfx_mlp_config *cfg_in, cfx_mlp_report *re
     }
     m.d2_max = (float)(d2max * 1.0001);
   }
+  m.x_pure = kMlpPureX;
   m.check_tol = (float)cfg.check_tol;
   m.check_every = cfg.check_every;
   // ---- initial weights: uniform with variance 1/fan_in, zero biases
@@ -340,12 +342,25 @@ int cfx_mlp_train(cfx_ctx *ctx, const cfx_mlp_config *cfg_in, cfx_mlp_report *re
     CFX_TRY(launch_mlp_predict(xs + n_train * kMlpIO, n_val, ctx->mlp_ops_inf, ctx->mlp_bias_inf, nout, ys,
                                ctx->stream));
     std::vector<float> hy((size_t)n_val * kMlpIO), ht2((size_t)n_val * kMlpIO);
+    std::vector<double> hxr((size_t)n_val * nin);  // raw inputs: T, log p, Y_l
+    CFX_TRY((int)hipMemcpyAsync(hxr.data(), xr + n_train * nin, sizeof(double) * hxr.size(), hipMemcpyDeviceToHost,
+                                ctx->stream));
     CFX_TRY((int)hipMemcpyAsync(hy.data(), ys, sizeof(float) * hy.size(), hipMemcpyDeviceToHost, ctx->stream));
     CFX_TRY((int)hipMemcpyAsync(ht2.data(), ts + n_train * kMlpIO, sizeof(float) * ht2.size(),
                                 hipMemcpyDeviceToHost, ctx->stream));
     CFX_TRY((int)hipStreamSynchronize(ctx->stream));
     std::vector<double> rel[3];
+    long long answered = 0;
     for (long long s = 0; s < n_val; ++s) {
+      // statistics over the samples the surrogate answers (below x_pure)
+      double winv = 0.0, ymax = 0.0;
+      for (int l = 0; l < nin - 2; ++l) {
+        const double yl = hxr[s * nin + 2 + l] * ctx->sp_host.d.inv_mw[l];
+        winv += yl;
+        ymax = std::max(ymax, yl);
+      }
+      if (!(ymax < (double)m.x_pure * winv)) continue;
+      ++answered;
       double worst_d = 0.0;
       for (int o = 0; o < nout; ++o) {
         const double d = (double)hy[s * kMlpIO + o] - (double)ht2[s * kMlpIO + o];
@@ -356,8 +371,8 @@ int cfx_mlp_train(cfx_ctx *ctx, const cfx_mlp_config *cfg_in, cfx_mlp_report *re
       }
       rel[2].push_back(worst_d);
     }
-    rmse = std::sqrt(rmse / ((double)n_val * nout));
-    for (int q = 0; q < 3; ++q) {
+    rmse = std::sqrt(rmse / ((double)std::max(answered, 1LL) * nout));
+    for (int q = 0; q < 3 && answered > 0; ++q) {
       std::sort(rel[q].begin(), rel[q].end());
       vmax[q] = rel[q].back();
       vp99[q] = rel[q][(size_t)(0.99 * (double)(rel[q].size() - 1))];
