@@ -89,7 +89,7 @@ class CfxMlpConfig(C.Structure):
         ("n_samples", C.c_longlong), ("holdout", C.c_double), ("epochs", C.c_int), ("batch", C.c_int),
         ("lr", C.c_double), ("lr_final", C.c_double), ("t_min", C.c_double), ("t_max", C.c_double),
         ("p_min", C.c_double), ("p_max", C.c_double), ("dirichlet_alpha", C.c_double), ("check_tol", C.c_double),
-        ("check_every", C.c_int), ("seed", C.c_ulonglong),
+        ("check_every", C.c_int), ("seed", C.c_ulonglong), ("envelope_quantile", C.c_double),
     ]
 
 
@@ -98,7 +98,7 @@ class CfxMlpReport(C.Structure):
         ("samples_ms", C.c_double), ("train_ms", C.c_double), ("epochs", C.c_int), ("steps", C.c_int),
         ("loss_first", C.c_double), ("loss_last", C.c_double), ("val_rmse", C.c_double),
         ("val_max_rel", C.c_double * 3), ("val_p99_rel", C.c_double * 3), ("n_train", C.c_longlong),
-        ("n_val", C.c_longlong), ("n_params", C.c_int),
+        ("n_val", C.c_longlong), ("n_params", C.c_int), ("val_answered", C.c_double),
     ]
 
 

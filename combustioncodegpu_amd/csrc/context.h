@@ -71,7 +71,8 @@ struct cfx_ctx {
   BlockView *views_dev = nullptr;
   std::vector<double *> coords;  // per block: the (ngg+1)-layer coordinates built on the device
   Table t_pad, t_int, t_dt, t_last, t_res;     // point-kernel workgroup prefixes
-  Table t_plane;                               // LU-SGS sweeps: my * mz columns per block
+  Table t_plane;                               // LU-SGS sweeps: one hyperplane's slots per block
+  Table t_lsk;                                 // LU-SGS pack: every hyperplane slot per block
   int n_planes = 0;                            // LU-SGS hyperplanes of the largest block
   bool lusgs = false;
   Table t_rhs[3][kNVar3];                      // pencil tiles per direction and variant

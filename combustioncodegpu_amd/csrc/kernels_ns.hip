@@ -680,51 +680,90 @@ int launch_update(const Batch &b, const SpeciesDev *sp, const SchemeDev &S, int 
 //   backward dQ_i = dQ*_i + sum_d [ a_d(i+e_d) dQ_(i+e_d) - dG_d(i+e_d)/2 ] / D_i
 // dG_d(n) = G_d(Q_n + dQ_n) - G_d(Q_n): the inviscid flux of cell n through
 // S_d = J grad xi_d, the pressure change at the cell's frozen gamma; dQ = 0
-// outside the block.  dq holds R on entry and dQ on exit (in place: a cell
-// reads only final neighbour values).  The sweeps run hyperplane by
-// hyperplane, i + j + k = plane, every cell of a plane independent.
+// outside the block.  The sweeps run hyperplane by hyperplane, i + j + k =
+// plane, every cell of a plane independent, in place on dQ (a cell reads
+// only final neighbour values).
+//
+// No direction of a hyperplane is contiguous in the block's own layout, so
+// the sweeps work on a copy of their operands in hyperplane order
+// (BlockView::lsk): a plane's slots are contiguous, and a cell's neighbour
+// along the plane-index axis sits one plane back in the same slot, along the
+// fastest in-plane axis one plane and one slot back, along the other one
+// plane and one row back.  Every sweep access then coalesces; the pack
+// kernel (which also forms the coefficients) and the unpack kernel pay the
+// transposition once per step, reading the block layout at strided points.
+constexpr int kNv = 5 + NS;
+constexpr int kLskCv = kNv, kLskP = 2 * kNv, kLskGam = 2 * kNv + 1, kLskS = 2 * kNv + 2, kLskA = 2 * kNv + 11,
+              kLskD = 2 * kNv + 14;
 
-// lus: D over the interior, then a_d for d = 0, 1, 2
-__global__ void __launch_bounds__(kTpb) lusgs_coef_kernel(const BlockView *__restrict__ views,
+// Slot z of a block's hyperplane layout -> plane s and cell coordinates c;
+// false for the slots of a plane that lie outside the block.
+__device__ __forceinline__ bool lusgs_slot_cell(const GView &B, unsigned z, unsigned &s, int c[3]) {
+  const unsigned plane = (unsigned)B.sk_plane, n1 = (unsigned)B.sk_n1;
+  s = z / plane;
+  const unsigned t = z - s * plane;
+  const int u = (int)(t % n1), v = (int)(t / n1);
+  const int m[3] = {B.mx, B.my, B.mz};
+  c[B.sk_b] = u;
+  c[B.sk_c] = v;
+  c[B.sk_a] = (int)s - u - v;
+  return c[B.sk_a] >= 0 && c[B.sk_a] < m[B.sk_a];
+}
+
+// One thread per slot, in slot order (the strided side is the read): R, the
+// state and geometry the flux change reads, and the coefficients D and a_d.
+__global__ void __launch_bounds__(kTpb) lusgs_pack_kernel(const BlockView *__restrict__ views,
                                                           const int *__restrict__ off, int nb, SchemeDev S) {
   const PointWork pw = point_work(off, nb);
   const GView B = gview(views[pw.blk]);
-  const long long np = B.np, ni = B.ni;
-  const long long q = pw.base + threadIdx.x;
-  if (q >= ni) return;
-  const long long p = interior_pidx(B, q);
+  const long long z = pw.base + threadIdx.x;
+  if (z >= B.sk_slots) return;
+  unsigned s;
+  int c[3];
+  if (!lusgs_slot_cell(B, (unsigned)z, s, c)) return;
+  const long long np = B.np, ni = B.ni, ns = B.sk_slots;
+  const long long q = ((long long)c[2] * B.my + c[1]) * B.mx + c[0];
+  const long long p = pidx(B, c[0], c[1], c[2]);
   const double J = B.jac[p], rho = B.bv[p];
   const double u = B.bv[np + p], v = B.bv[2 * np + p], w = B.bv[3 * np + p];
-  const double gam = B.props[p], c = B.props[np + p], mu = B.props[2 * np + p];
+  const double gam = B.props[p], cs = B.props[np + p], mu = B.props[2 * np + p];
   const double vc = fmax(gam, 4.0 / 3.0) * mu / rho / S.pr;
   double diag = J / B.dt_local[q];
   for (int d = 0; d < S.dim; ++d) {
     const double gx = B.metric[(3 * d) * np + p], gy = B.metric[(3 * d + 1) * np + p],
                  gz = B.metric[(3 * d + 2) * np + p];
     const double g2 = gx * gx + gy * gy + gz * gz;
-    const double r = J * (fabs(u * gx + v * gy + w * gz) + c * sqrt(g2));
+    const double r = J * (fabs(u * gx + v * gy + w * gz) + cs * sqrt(g2));
     const double vd = vc * J * g2;
     diag += S.omega * r + 2 * vd;
-    B.lus[(1 + d) * ni + q] = 0.5 * S.omega * r + vd;
+    B.lsk[(kLskA + d) * ns + z] = 0.5 * S.omega * r + vd;
+    B.lsk[(kLskS + 3 * d) * ns + z] = J * gx;
+    B.lsk[(kLskS + 3 * d + 1) * ns + z] = J * gy;
+    B.lsk[(kLskS + 3 * d + 2) * ns + z] = J * gz;
   }
-  B.lus[q] = diag;
+  B.lsk[kLskD * ns + z] = diag;
+  B.lsk[kLskP * ns + z] = B.bv[4 * np + p];
+  B.lsk[kLskGam * ns + z] = gam;
+#pragma unroll
+  for (int l = 0; l < kNv; ++l) {
+    B.lsk[l * ns + z] = B.dq[l * ni + q];
+    B.lsk[(kLskCv + l) * ns + z] = B.cv[l * np + p];
+  }
 }
 
-__device__ __forceinline__ void lusgs_flux_change(const GView &B, const SchemeDev &S, int d, long long p,
-                                                  const double *dQ, double *dG) {
-  constexpr int NV = 5 + NS;
-  const long long np = B.np;
-  const double J = B.jac[p];
-  const double Sx = J * B.metric[(3 * d) * np + p], Sy = J * B.metric[(3 * d + 1) * np + p],
-               Sz = J * B.metric[(3 * d + 2) * np + p];
-  const double gam = B.props[p];
-  double q0[NV], q1[NV];
+// dG_d of the cell in slot z for the increment dQ
+__device__ __forceinline__ void lusgs_flux_change(const GView &B, int d, long long z, const double *dQ, double *dG) {
+  const long long ns = B.sk_slots;
+  const double Sx = B.lsk[(kLskS + 3 * d) * ns + z], Sy = B.lsk[(kLskS + 3 * d + 1) * ns + z],
+               Sz = B.lsk[(kLskS + 3 * d + 2) * ns + z];
+  const double gam = B.lsk[kLskGam * ns + z];
+  double q0[kNv], q1[kNv];
 #pragma unroll
-  for (int l = 0; l < NV; ++l) {
-    q0[l] = B.cv[l * np + p];
+  for (int l = 0; l < kNv; ++l) {
+    q0[l] = B.lsk[(kLskCv + l) * ns + z];
     q1[l] = q0[l] + dQ[l];
   }
-  const double p0 = B.bv[4 * np + p];
+  const double p0 = B.lsk[kLskP * ns + z];
   const double ke0 = 0.5 * (q0[1] * q0[1] + q0[2] * q0[2] + q0[3] * q0[3]) / q0[0];
   const double ke1 = 0.5 * (q1[1] * q1[1] + q1[2] * q1[2] + q1[3] * q1[3]) / q1[0];
   const double p1 = p0 + (gam - 1) * (dQ[4] - (ke1 - ke0));
@@ -736,59 +775,70 @@ __device__ __forceinline__ void lusgs_flux_change(const GView &B, const SchemeDe
   dG[3] = (q1[3] * U1 + Sz * p1) - (q0[3] * U0 + Sz * p0);
   dG[4] = (q1[4] + p1) * U1 - (q0[4] + p0) * U0;
 #pragma unroll
-  for (int l = 5; l < NV; ++l) dG[l] = q1[l] * U1 - q0[l] * U0;
+  for (int l = 5; l < kNv; ++l) dG[l] = q1[l] * U1 - q0[l] * U0;
 }
 
-// One hyperplane of a sweep: thread t of a block is the column (j, k) =
-// (t % my, t / my), its cell on the plane i = plane - j - k (if inside).
-// A plane holds at most my * mz cells, too few to fill the device with
+// One hyperplane of a sweep: thread t of a block is slot t of the plane.
+// A plane holds at most sk_plane cells, too few to fill the device with
 // kTpb-thread workgroups, so the sweeps use one wavefront per workgroup
 // (kSweepTpb), which spreads a plane over four times as many CUs.
 template <bool FWD>
 __global__ void __launch_bounds__(kSweepTpb) lusgs_sweep_kernel(const BlockView *__restrict__ views,
                                                                 const int *__restrict__ off, int nb, SchemeDev S,
                                                                 int plane) {
-  constexpr int NV = 5 + NS;
   const int w = blockIdx.x + off[0];
   const int blk = find_seg(off, nb, w);
   const GView B = gview(views[blk]);
-  const int mx = B.mx, my = B.my, mz = B.mz;
   const long long t = (long long)(w - off[blk]) * kSweepTpb + threadIdx.x;
-  if (t >= (long long)my * mz) return;
-  const int j = (int)(t % my), k = (int)(t / my);
-  const int i = plane - j - k;
-  if (i < 0 || i >= mx) return;
-  const long long ni = B.ni;
-  const long long q = ((long long)k * my + j) * mx + i;
-  const long long p = pidx(B, i, j, k);
-  const int ijk[3] = {i, j, k}, m[3] = {mx, my, mz};
-  const long long qs[3] = {1, mx, (long long)mx * my}, ps[3] = {1, B.sy, B.sz};
-  double acc[NV];
+  if (t >= B.sk_plane) return;
+  const long long z = (long long)plane * B.sk_plane + t;
+  if (z >= B.sk_slots) return;
+  unsigned s;
+  int c[3];
+  if (!lusgs_slot_cell(B, (unsigned)z, s, c)) return;
+  const int m[3] = {B.mx, B.my, B.mz};
+  const long long ns = B.sk_slots;
+  double acc[kNv];
 #pragma unroll
-  for (int l = 0; l < NV; ++l) acc[l] = FWD ? B.dq[l * ni + q] : 0.0;
+  for (int l = 0; l < kNv; ++l) acc[l] = FWD ? B.lsk[l * ns + z] : 0.0;
   for (int d = 0; d < S.dim; ++d) {
-    if (FWD ? ijk[d] == 0 : ijk[d] == m[d] - 1) continue;
-    const long long qn = FWD ? q - qs[d] : q + qs[d];
-    const long long pn = FWD ? p - ps[d] : p + ps[d];
-    double dn[NV], dG[NV];
+    if (FWD ? c[d] == 0 : c[d] == m[d] - 1) continue;
+    const long long dz = B.sk_plane + (d == B.sk_b ? 1 : (d == B.sk_c ? B.sk_n1 : 0));
+    const long long zn = FWD ? z - dz : z + dz;
+    double dn[kNv], dG[kNv];
 #pragma unroll
-    for (int l = 0; l < NV; ++l) dn[l] = B.dq[l * ni + qn];
-    lusgs_flux_change(B, S, d, pn, dn, dG);
-    const double an = B.lus[(1 + d) * ni + qn];
+    for (int l = 0; l < kNv; ++l) dn[l] = B.lsk[l * ns + zn];
+    lusgs_flux_change(B, d, zn, dn, dG);
+    const double an = B.lsk[(kLskA + d) * ns + zn];
 #pragma unroll
-    for (int l = 0; l < NV; ++l) acc[l] += FWD ? 0.5 * dG[l] + an * dn[l] : an * dn[l] - 0.5 * dG[l];
+    for (int l = 0; l < kNv; ++l) acc[l] += FWD ? 0.5 * dG[l] + an * dn[l] : an * dn[l] - 0.5 * dG[l];
   }
-  const double D = B.lus[q];
+  const double D = B.lsk[kLskD * ns + z];
 #pragma unroll
-  for (int l = 0; l < NV; ++l) {
-    if (FWD) B.dq[l * ni + q] = acc[l] / D;
-    else B.dq[l * ni + q] += acc[l] / D;
+  for (int l = 0; l < kNv; ++l) {
+    if (FWD) B.lsk[l * ns + z] = acc[l] / D;
+    else B.lsk[l * ns + z] += acc[l] / D;
   }
 }
 
-int launch_lusgs_coef(const Batch &b, const SchemeDev &S, hipStream_t st) {
+// dq <- dQ, one thread per interior cell in the block's order
+__global__ void __launch_bounds__(kTpb) lusgs_unpack_kernel(const BlockView *__restrict__ views,
+                                                            const int *__restrict__ off, int nb) {
+  const PointWork pw = point_work(off, nb);
+  const GView B = gview(views[pw.blk]);
+  const long long q = pw.base + threadIdx.x;
+  if (q >= B.ni) return;
+  const unsigned uq = (unsigned)q, mx = (unsigned)B.mx, my = (unsigned)B.my;
+  const int c[3] = {(int)(uq % mx), (int)((uq / mx) % my), (int)(uq / mx / my)};
+  const long long z = (long long)(c[0] + c[1] + c[2]) * B.sk_plane + (long long)c[B.sk_c] * B.sk_n1 + c[B.sk_b];
+  const long long ns = B.sk_slots, ni = B.ni;
+#pragma unroll
+  for (int l = 0; l < kNv; ++l) B.dq[l * ni + q] = B.lsk[l * ns + z];
+}
+
+int launch_lusgs_pack(const Batch &b, const SchemeDev &S, hipStream_t st) {
   if (b.total <= 0) return 0;
-  lusgs_coef_kernel<<<b.total, kTpb, 0, st>>>(b.views, b.off, b.nb, S);
+  lusgs_pack_kernel<<<b.total, kTpb, 0, st>>>(b.views, b.off, b.nb, S);
   return (int)hipGetLastError();
 }
 
@@ -798,6 +848,12 @@ int launch_lusgs_sweep(const Batch &b, const SchemeDev &S, int plane, int fwd, h
     lusgs_sweep_kernel<true><<<b.total, kSweepTpb, 0, st>>>(b.views, b.off, b.nb, S, plane);
   else
     lusgs_sweep_kernel<false><<<b.total, kSweepTpb, 0, st>>>(b.views, b.off, b.nb, S, plane);
+  return (int)hipGetLastError();
+}
+
+int launch_lusgs_unpack(const Batch &b, hipStream_t st) {
+  if (b.total <= 0) return 0;
+  lusgs_unpack_kernel<<<b.total, kTpb, 0, st>>>(b.views, b.off, b.nb);
   return (int)hipGetLastError();
 }
 
@@ -902,7 +958,8 @@ struct Register64 {
     k.cv_from_bv = launch_cv_from_bv;
     k.update = launch_update;
     k.face_ops = launch_face_ops;
-    k.lusgs_coef = launch_lusgs_coef;
+    k.lusgs_pack = launch_lusgs_pack;
+    k.lusgs_unpack = launch_lusgs_unpack;
     k.lusgs_sweep = launch_lusgs_sweep;
     register_rhs<double>(k);
     k.present = 1;

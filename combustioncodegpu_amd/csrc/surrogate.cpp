@@ -157,6 +157,7 @@ void cfx_mlp_default_config(cfx_mlp_config *c) {
   c->check_tol = 0.05;
   c->check_every = 64;
   c->seed = 20240531ull;
+  c->envelope_quantile = 0.99;
 }
 
 int cfx_mlp_train(cfx_ctx *ctx, const cfx_mlp_config *cfg_in, cfx_mlp_report *rep) {
@@ -166,7 +167,8 @@ int cfx_mlp_train(cfx_ctx *ctx, const cfx_mlp_config *cfg_in, cfx_mlp_report *re
   else cfx_mlp_default_config(&cfg);
   if (cfg.n_samples < 1024 || cfg.batch < 32 || cfg.batch % 32 || cfg.epochs < 0 || !(cfg.holdout >= 0) ||
       cfg.holdout >= 0.5 || !(cfg.t_min > 0) || !(cfg.t_max > cfg.t_min) || !(cfg.p_min > 0) ||
-      !(cfg.p_max >= cfg.p_min) || !(cfg.dirichlet_alpha > 0) || cfg.check_every < 1)
+      !(cfg.p_max >= cfg.p_min) || !(cfg.dirichlet_alpha > 0) || cfg.check_every < 1 ||
+      !(cfg.envelope_quantile > 0) || cfg.envelope_quantile > 1)
     return ctx->fail(CFX_E_ARG, "surrogate training configuration out of range");
   if (int rc = mlp_alloc(ctx)) return rc;
   const KernelSet64 &k64 = g_kernels64[ctx->ns];
@@ -251,22 +253,32 @@ int cfx_mlp_train(cfx_ctx *ctx, const cfx_mlp_config *cfg_in, cfx_mlp_report *re
     m.pca_inv_var[k] = lam[k] > 1e-9 * lmax ? (float)(1.0 / lam[k]) : 0.0f;
     for (int i = 0; i < nin; ++i) m.pca[k][i] = (float)vec[k][i];
   }
+  // squared Mahalanobis distance of a normalised input in the principal axes
+  auto dist2 = [&](const float *x) {
+    double d2 = 0.0;
+    for (int k = 0; k < nin; ++k) {
+      double a = 0.0;
+      for (int i = 0; i < nin; ++i) a += (double)m.pca[k][i] * x[i];
+      d2 += (double)m.pca_inv_var[k] * a * a;
+    }
+    return d2;
+  };
   {
+    // the envelope's radius: the envelope_quantile of the training inputs'
+    // distances (the network's largest held-out errors sit in the sparse
+    // tail of the input distribution, at the dominant species' rho*D)
     std::vector<float> hxs((size_t)n_train * kMlpIO);
     CFX_TRY((int)hipMemcpyAsync(hxs.data(), xs, sizeof(float) * hxs.size(), hipMemcpyDeviceToHost, ctx->stream));
     CFX_TRY((int)hipStreamSynchronize(ctx->stream));
-    double d2max = 0.0;
-    for (long long s = 0; s < n_train; ++s) {
-      const float *x = hxs.data() + s * kMlpIO;
-      double d2 = 0.0;
-      for (int k = 0; k < nin; ++k) {
-        double a = 0.0;
-        for (int i = 0; i < nin; ++i) a += (double)m.pca[k][i] * x[i];
-        d2 += (double)m.pca_inv_var[k] * a * a;
-      }
-      d2max = std::max(d2max, d2);
+    std::vector<double> d2s((size_t)n_train);
+    for (long long s = 0; s < n_train; ++s) d2s[(size_t)s] = dist2(hxs.data() + s * kMlpIO);
+    if (cfg.envelope_quantile >= 1.0) {
+      m.d2_max = (float)(*std::max_element(d2s.begin(), d2s.end()) * 1.0001);
+    } else {
+      const size_t r = (size_t)(cfg.envelope_quantile * (double)(n_train - 1));
+      std::nth_element(d2s.begin(), d2s.begin() + (long long)r, d2s.end());
+      m.d2_max = (float)d2s[r];
     }
-    m.d2_max = (float)(d2max * 1.0001);
   }
   m.x_pure = kMlpPureX;
   m.check_tol = (float)cfg.check_tol;
@@ -334,7 +346,7 @@ int cfx_mlp_train(cfx_ctx *ctx, const cfx_mlp_config *cfg_in, cfx_mlp_report *re
   CFX_TRY(launch_mlp_pack(ctx->mlp_params, shp, ctx->mlp_ops_inf, ctx->mlp_bias_inf, 1, ctx->stream));
   (void)hipEventRecord(e2, ctx->stream);
   // ---- validation on the held-out samples
-  double rmse = 0.0, vmax[3] = {0, 0, 0}, vp99[3] = {0, 0, 0};
+  double rmse = 0.0, vmax[3] = {0, 0, 0}, vp99[3] = {0, 0, 0}, val_answered = 0.0;
   if (n_val > 0) {
     float *ys = sc.get<float>((size_t)n_val * kMlpIO, rc);
     if (rc) return ctx->fail(rc, "hipMalloc validation");
@@ -343,8 +355,11 @@ int cfx_mlp_train(cfx_ctx *ctx, const cfx_mlp_config *cfg_in, cfx_mlp_report *re
                                ctx->stream));
     std::vector<float> hy((size_t)n_val * kMlpIO), ht2((size_t)n_val * kMlpIO);
     std::vector<double> hxr((size_t)n_val * nin);  // raw inputs: T, log p, Y_l
+    std::vector<float> hxv((size_t)n_val * kMlpIO);  // normalised inputs
     CFX_TRY((int)hipMemcpyAsync(hxr.data(), xr + n_train * nin, sizeof(double) * hxr.size(), hipMemcpyDeviceToHost,
                                 ctx->stream));
+    CFX_TRY((int)hipMemcpyAsync(hxv.data(), xs + n_train * kMlpIO, sizeof(float) * hxv.size(),
+                                hipMemcpyDeviceToHost, ctx->stream));
     CFX_TRY((int)hipMemcpyAsync(hy.data(), ys, sizeof(float) * hy.size(), hipMemcpyDeviceToHost, ctx->stream));
     CFX_TRY((int)hipMemcpyAsync(ht2.data(), ts + n_train * kMlpIO, sizeof(float) * ht2.size(),
                                 hipMemcpyDeviceToHost, ctx->stream));
@@ -352,7 +367,9 @@ int cfx_mlp_train(cfx_ctx *ctx, const cfx_mlp_config *cfg_in, cfx_mlp_report *re
     std::vector<double> rel[3];
     long long answered = 0;
     for (long long s = 0; s < n_val; ++s) {
-      // statistics over the samples the surrogate answers (below x_pure)
+      // statistics over the samples the surrogate answers (inside the
+      // envelope, largest mole fraction below x_pure)
+      if (!(dist2(hxv.data() + s * kMlpIO) <= (double)m.d2_max)) continue;
       double winv = 0.0, ymax = 0.0;
       for (int l = 0; l < nin - 2; ++l) {
         const double yl = hxr[s * nin + 2 + l] * ctx->sp_host.d.inv_mw[l];
@@ -371,6 +388,7 @@ int cfx_mlp_train(cfx_ctx *ctx, const cfx_mlp_config *cfg_in, cfx_mlp_report *re
       }
       rel[2].push_back(worst_d);
     }
+    val_answered = (double)answered / (double)n_val;
     rmse = std::sqrt(rmse / ((double)std::max(answered, 1LL) * nout));
     for (int q = 0; q < 3 && answered > 0; ++q) {
       std::sort(rel[q].begin(), rel[q].end());
@@ -399,6 +417,7 @@ int cfx_mlp_train(cfx_ctx *ctx, const cfx_mlp_config *cfg_in, cfx_mlp_report *re
     rep->n_train = n_train;
     rep->n_val = n_val;
     rep->n_params = P;
+    rep->val_answered = val_answered;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

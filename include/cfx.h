@@ -339,6 +339,8 @@ typedef struct {
   double check_tol;       /* relative bound of the sampled Wilke check (0.05) */
   int check_every;        /* one wavefront in check_every is checked (64) */
   unsigned long long seed;
+  double envelope_quantile; /* the envelope admits inputs up to this quantile of the
+                               training inputs' Mahalanobis distance (0.99; 1: all) */
 } cfx_mlp_config;
 
 typedef struct {
@@ -350,6 +352,8 @@ typedef struct {
   double val_p99_rel[3];        /* 99th percentile of the same */
   long long n_train, n_val;
   int n_params;
+  double val_answered; /* fraction of the held-out samples the surrogate answers (inside the
+                          envelope, below the near-pure limit); the val_ statistics are over these */
 } cfx_mlp_report;
 
 void cfx_mlp_default_config(cfx_mlp_config *cfg);

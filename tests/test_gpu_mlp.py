@@ -77,14 +77,21 @@ def test_gradient_matches_oracle():
 
 
 def test_training_reaches_accuracy():
-    """The default training (2M samples of the FP64 Wilke routine) on held-out
-    samples: 99% of mu, kappa and rho*D within 3 %."""
+    """The default training (2M samples of the FP64 Wilke routine) on the
+    held-out samples the surrogate answers (inside the envelope, whose radius
+    is the 0.99 quantile of the training inputs' Mahalanobis distance): at
+    least 98% of them answered, 99% of mu, kappa and rho*D within 3 %, every
+    one within 10 % (measured 1.8 %, 2.4 %, 5.3 %; with the envelope at the
+    largest training distance the dominant species' rho*D reached 60 % in
+    the sparse tail)."""
     g = solver(small_mixed_case())
     rep = g.mlp_train()
     print(f"[mlp] training: {rep}", flush=True)
     assert rep["n_train"] + rep["n_val"] == 2000000
     assert rep["loss_last"] < rep["loss_first"] / 1000
+    assert rep["val_answered"] >= 0.98
     assert max(rep["val_p99_rel"]) <= 0.03
+    assert max(rep["val_max_rel"]) <= 0.10
     rep_params, norm = g.mlp_get_params()
     assert np.all(np.isfinite(rep_params))
 
