@@ -55,9 +55,9 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8}))
 
 
-def cpu_baseline(case, threads, steps=3):
+def cpu_baseline(case, threads, steps=5):
     """The oracle on the host cores: the median of `steps` SSP-RK3 steps of
-    the same block (about 3 s each on 16 threads)."""
+    the same block (about 3 s each on 16 threads, 15 s in all)."""
     import statistics
     from oracle import pyoracle
     pyoracle.build()
