@@ -155,16 +155,26 @@ __global__ void __launch_bounds__(64) mlp_train_kernel(const float *__restrict__
   if (lane == 0) loss_out[w] = lsum * scale;
 }
 
-// Sum of the per-wavefront gradients (fixed order) and an Adam step on the
-// FP32 master parameters.
+// Sum of the per-wavefront gradients and an Adam step on the FP32 master
+// parameters.  A workgroup takes 64 parameters; its four wavefronts sum
+// quarters of the partials (each in increasing order, loads coalesced over
+// the parameters) and the quarter sums are added in order, so the result is
+// deterministic.
 __global__ void __launch_bounds__(256) mlp_adam_kernel(float *__restrict__ params, float *__restrict__ m,
                                                        float *__restrict__ v, const float *__restrict__ partial,
                                                        int n_partial, int P, float lr, float b1, float b2,
                                                        float eps, float c1, float c2, float *__restrict__ grad_out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P) return;
-  float gsum = 0.0f;
-  for (int w = 0; w < n_partial; ++w) gsum += partial[(long long)w * P + i];
+  __shared__ float quarter[4][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + lane;
+  const int w0 = (int)((long long)n_partial * wv / 4), w1 = (int)((long long)n_partial * (wv + 1) / 4);
+  float part = 0.0f;
+  if (i < P)
+    for (int w = w0; w < w1; ++w) part += partial[(long long)w * P + i];
+  quarter[wv][lane] = part;
+  __syncthreads();
+  if (wv != 0 || i >= P) return;
+  const float gsum = ((quarter[0][lane] + quarter[1][lane]) + quarter[2][lane]) + quarter[3][lane];
   if (grad_out) grad_out[i] = gsum;
   if (lr <= 0.0f) return;
   const float mi = b1 * m[i] + (1.0f - b1) * gsum;
@@ -289,8 +299,8 @@ int launch_mlp_train(const float *xs, const float *ts, long long n_train, long l
 
 int launch_mlp_adam(float *params, float *m, float *v, const float *partial, int n_partial, int P, float lr,
                     float b1, float b2, float eps, float c1, float c2, float *grad_out, hipStream_t st) {
-  mlp_adam_kernel<<<(P + 255) / 256, 256, 0, st>>>(params, m, v, partial, n_partial, P, lr, b1, b2, eps, c1, c2,
-                                                   grad_out);
+  mlp_adam_kernel<<<(P + 63) / 64, 256, 0, st>>>(params, m, v, partial, n_partial, P, lr, b1, b2, eps, c1, c2,
+                                                 grad_out);
   return (int)hipGetLastError();
 }
 
