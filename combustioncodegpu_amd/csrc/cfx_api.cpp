@@ -907,14 +907,15 @@ int cfx_create(const cfx_params *params, const cfx_species *species, const cfx_b
   ctx->d_newton = ctx->dalloc<unsigned long long>(1, rc);
   ctx->d_minscratch = ctx->dalloc<unsigned long long>(1, rc);
   if (rc) return bail(ctx->fail(rc, "hipMalloc"));
-  (void)hipMemcpy(ctx->sp_dev, &ctx->sp_host, sizeof(SpeciesDev), hipMemcpyHostToDevice);
-  if (!ctx->inflow_host.empty())
-    (void)hipMemcpy(ctx->inflow_dev, ctx->inflow_host.data(), sizeof(InflowDev) * ctx->inflow_host.size(),
-              hipMemcpyHostToDevice);
-  (void)hipMemset(ctx->d_flags, 0, sizeof(int));
-  (void)hipMemset(ctx->d_newton, 0, sizeof(unsigned long long));
+  rc = (int)hipMemcpy(ctx->sp_dev, &ctx->sp_host, sizeof(SpeciesDev), hipMemcpyHostToDevice);
+  if (!rc && !ctx->inflow_host.empty())
+    rc = (int)hipMemcpy(ctx->inflow_dev, ctx->inflow_host.data(), sizeof(InflowDev) * ctx->inflow_host.size(),
+                        hipMemcpyHostToDevice);
+  if (!rc) rc = (int)hipMemset(ctx->d_flags, 0, sizeof(int));
+  if (!rc) rc = (int)hipMemset(ctx->d_newton, 0, sizeof(unsigned long long));
   const double dt0 = 0.0;
-  (void)hipMemcpy(ctx->d_dt, &dt0, sizeof(double), hipMemcpyHostToDevice);
+  if (!rc) rc = (int)hipMemcpy(ctx->d_dt, &dt0, sizeof(double), hipMemcpyHostToDevice);
+  if (rc) return bail(ctx->fail(rc, "context upload"));
   for (int b = 0; b < n_block; ++b)
     if (ctx->hb[b].np >= (1LL << 29)) {  // the RHS kernels address a block with 32-bit byte offsets
       ctx->err = "block " + std::to_string(b) + " exceeds 2^29 padded points";
@@ -962,11 +963,12 @@ int cfx_create(const cfx_params *params, const cfx_species *species, const cfx_b
       V.jac32 = ctx->dalloc<float>(np, rc);
       V.dq32 = ctx->dalloc<float>(ctx->nv * ni, rc);
       if (rc) return bail(ctx->fail(rc, "hipMalloc FP32 arrays"));
-      (void)hipMemset(V.props32, 0, sizeof(float) * (4 + ctx->ns) * np);
-      (void)hipMemset(V.dq32, 0, sizeof(float) * ctx->nv * ni);
+      rc = (int)hipMemset(V.props32, 0, sizeof(float) * (4 + ctx->ns) * np);
+      if (!rc) rc = (int)hipMemset(V.dq32, 0, sizeof(float) * ctx->nv * ni);
     }
-    (void)hipMemset(V.props, 0, sizeof(double) * (4 + ctx->ns) * np);
-    (void)hipMemset(V.dq, 0, sizeof(double) * ctx->nv * ni);
+    if (!rc) rc = (int)hipMemset(V.props, 0, sizeof(double) * (4 + ctx->ns) * np);
+    if (!rc) rc = (int)hipMemset(V.dq, 0, sizeof(double) * ctx->nv * ni);
+    if (rc) return bail(ctx->fail(rc, "block array clear"));
     ctx->views.push_back(V);
     ctx->n_grid_total += H.ni;
   }
@@ -975,7 +977,9 @@ int cfx_create(const cfx_params *params, const cfx_species *species, const cfx_b
     if ((rc = device_metrics(ctx, b, ctx->views[b]))) return bail(rc);
   ctx->views_dev = ctx->dalloc<BlockView>(ctx->views.size(), rc);
   if (rc) return bail(ctx->fail(rc, "hipMalloc block table"));
-  (void)hipMemcpy(ctx->views_dev, ctx->views.data(), sizeof(BlockView) * ctx->views.size(), hipMemcpyHostToDevice);
+  if ((rc = (int)hipMemcpy(ctx->views_dev, ctx->views.data(), sizeof(BlockView) * ctx->views.size(),
+                           hipMemcpyHostToDevice)))
+    return bail(ctx->fail(rc, "block table upload"));
   if ((rc = build_tables(ctx))) return bail(ctx->fail(rc, "work tables"));
   if ((rc = build_ops(ctx))) return bail(ctx->fail(rc, "ghost-fill schedule"));
   ctx->n_partial = ctx->t_res.host.back();
