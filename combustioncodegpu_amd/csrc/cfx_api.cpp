@@ -199,17 +199,13 @@ int chunks(long long n, int per) { return (int)((n + per - 1) / per); }
 
 int build_tables(cfx_ctx *ctx) {
   const int nb = (int)ctx->hb.size();
-  std::vector<int> pad(nb), in1(nb), dtc(nb), last(nb), res(nb), pln(nb), lsk(nb);
-  ctx->n_planes = 0;
+  std::vector<int> pad(nb), in1(nb), dtc(nb), last(nb), res(nb), til(nb);
+  ctx->n_tplanes = 0;
   for (int b = 0; b < nb; ++b) {
     const HostBlock &H = ctx->hb[b];
-    const int m[3] = {H.mx, H.my, H.mz};
-    int a, pb, pc;
-    lusgs_axes(m, a, pb, pc);
-    const long long plane = (long long)m[pb] * m[pc];
-    pln[b] = chunks(plane, kSweepTpb);
-    lsk[b] = chunks(plane * (H.mx + H.my + H.mz - 2), kTpb);
-    ctx->n_planes = std::max(ctx->n_planes, H.mx + H.my + H.mz - 2);
+    const int nt[3] = {chunks(H.mx, kLuTi), chunks(H.my, kLuTj), chunks(H.mz, kLuTk)};
+    til[b] = nt[1] * nt[2];
+    ctx->n_tplanes = std::max(ctx->n_tplanes, nt[0] + nt[1] + nt[2] - 2);
     pad[b] = chunks(H.np, kTpb);
     in1[b] = chunks(H.ni, kTpb);
     dtc[b] = chunks(H.ni, kTpb * kDtIter);
@@ -219,8 +215,7 @@ int build_tables(cfx_ctx *ctx) {
   int rc;
   if ((rc = upload_table(ctx, ctx->t_pad, pad)) || (rc = upload_table(ctx, ctx->t_int, in1)) ||
       (rc = upload_table(ctx, ctx->t_dt, dtc)) || (rc = upload_table(ctx, ctx->t_last, last)) ||
-      (rc = upload_table(ctx, ctx->t_res, res)) || (rc = upload_table(ctx, ctx->t_plane, pln)) ||
-      (rc = upload_table(ctx, ctx->t_lsk, lsk)))
+      (rc = upload_table(ctx, ctx->t_res, res)) || (rc = upload_table(ctx, ctx->t_tile, til)))
     return rc;
   for (int d = 0; d < ctx->dim; ++d)
     for (int v = 0; v < rhs_variants(ctx->dim); ++v) {
@@ -530,24 +525,21 @@ int do_update(cfx_ctx *ctx, int stage) {
   return 0;
 }
 
-// LU-SGS solve (the implicit treatment of Driver.cu:156): operands and
-// coefficients packed in hyperplane order, the forward and backward sweeps
-// hyperplane by hyperplane, then dq <- dQ
+// LU-SGS solve (the implicit treatment of Driver.cu:156): coefficients, then
+// the forward and backward sweeps tile hyperplane by tile hyperplane; dq <- dQ
 int do_lusgs(cfx_ctx *ctx) {
   const KernelSet64 &k = g_kernels64[ctx->ns];
-  CFX_TRY(per_batch(ctx, ctx->t_lsk, [&](const Batch &b) {
-    return ctx->timed_launch(K_UPDATE, [&] { return k.lusgs_pack(b, ctx->S, ctx->stream); });
+  CFX_TRY(per_batch(ctx, ctx->t_int, [&](const Batch &b) {
+    return ctx->timed_launch(K_UPDATE, [&] { return k.lusgs_coef(b, ctx->S, ctx->stream); });
   }));
   for (int pass = 0; pass < 2; ++pass)
-    for (int n = 0; n < ctx->n_planes; ++n) {
-      const int plane = pass == 0 ? n : ctx->n_planes - 1 - n;
-      CFX_TRY(per_batch(ctx, ctx->t_plane, [&](const Batch &b) {
-        return ctx->timed_launch(K_UPDATE, [&] { return k.lusgs_sweep(b, ctx->S, plane, pass == 0, ctx->stream); });
+    for (int n = 0; n < ctx->n_tplanes; ++n) {
+      const int tplane = pass == 0 ? n : ctx->n_tplanes - 1 - n;
+      CFX_TRY(per_batch(ctx, ctx->t_tile, [&](const Batch &b) {
+        return ctx->timed_launch(K_UPDATE, [&] { return k.lusgs_sweep(b, ctx->S, tplane, pass == 0, ctx->stream); });
       }));
     }
-  return per_batch(ctx, ctx->t_int, [&](const Batch &b) {
-    return ctx->timed_launch(K_UPDATE, [&] { return k.lusgs_unpack(b, ctx->stream); });
-  });
+  return 0;
 }
 
 int do_bc(cfx_ctx *ctx) {
@@ -945,14 +937,7 @@ int cfx_create(const cfx_params *params, const cfx_species *species, const cfx_b
     V.qn = ctx->dalloc<double>(ctx->nv * ni, rc);
     V.dt_local = ctx->dalloc<double>(ni, rc);
     V.bv_last = ctx->dalloc<double>(4 * ni, rc);
-    if (ctx->lusgs) {
-      const int m[3] = {H.mx, H.my, H.mz};
-      lusgs_axes(m, V.sk_a, V.sk_b, V.sk_c);
-      V.sk_n1 = m[V.sk_b];
-      V.sk_plane = (long long)m[V.sk_b] * m[V.sk_c];
-      V.sk_slots = (long long)(H.mx + H.my + H.mz - 2) * V.sk_plane;
-      V.lsk = ctx->dalloc<double>((size_t)lusgs_vars(ctx->nv) * (size_t)V.sk_slots, rc);
-    }
+    if (ctx->lusgs) V.lus = ctx->dalloc<double>(4 * ni, rc);
     if (rc) return bail(ctx->fail(rc, "hipMalloc block arrays"));
     V.mixed = ctx->mixed ? 1 : 0;
     if (ctx->mixed) {

@@ -79,13 +79,7 @@ struct BlockView {
   double *cv, *bv, *yk, *props;  // props: gamma, c, mu, kappa, rhoD[ns]
   double *metric, *jac;          // metric: 9 x np (xi_x..zeta_z), jac: np
   double *dq, *qn, *dt_local, *bv_last;
-  // LU-SGS sweep operands in hyperplane order (null unless the context uses
-  // LU-SGS): slot(s, u, v) = s * sk_plane + v * sk_n1 + u holds the interior
-  // cell with coordinate u on axis sk_b, v on axis sk_c and s - u - v on
-  // axis sk_a (the longest), sk_slots slots per variable
-  double *lsk;
-  int sk_a, sk_b, sk_c, sk_n1;
-  long long sk_plane, sk_slots;
+  double *lus;  // LU-SGS: D, then a_d for d = 0..2, each over the interior (null otherwise)
   // mixed precision: FP32 copies of the RHS inputs (refreshed by the
   // properties kernel over the padded range) and the FP32 dq; null in FP64 mode
   int mixed;
@@ -123,9 +117,7 @@ struct GView {
   int mx, my, mz;
   int px, py, pz;
   long long np, ni, sy, sz;
-  gptr<double> cv, bv, yk, props, metric, jac, dq, qn, dt_local, bv_last, lsk;
-  int sk_a, sk_b, sk_c, sk_n1;
-  long long sk_plane, sk_slots;
+  gptr<double> cv, bv, yk, props, metric, jac, dq, qn, dt_local, bv_last, lus;
   int mixed;
   gptr<float> bv32, yk32, props32, metric32, jac32, dq32;
   int fuse[6];
@@ -139,9 +131,7 @@ __device__ __forceinline__ GView gview(const BlockView &B) {
   g.cv = (gptr<double>)B.cv, g.bv = (gptr<double>)B.bv, g.yk = (gptr<double>)B.yk;
   g.props = (gptr<double>)B.props, g.metric = (gptr<double>)B.metric, g.jac = (gptr<double>)B.jac;
   g.dq = (gptr<double>)B.dq, g.qn = (gptr<double>)B.qn, g.dt_local = (gptr<double>)B.dt_local;
-  g.bv_last = (gptr<double>)B.bv_last, g.lsk = (gptr<double>)B.lsk;
-  g.sk_a = B.sk_a, g.sk_b = B.sk_b, g.sk_c = B.sk_c, g.sk_n1 = B.sk_n1;
-  g.sk_plane = B.sk_plane, g.sk_slots = B.sk_slots;
+  g.bv_last = (gptr<double>)B.bv_last, g.lus = (gptr<double>)B.lus;
   g.mixed = B.mixed;
   g.bv32 = (gptr<float>)B.bv32, g.yk32 = (gptr<float>)B.yk32, g.props32 = (gptr<float>)B.props32;
   g.metric32 = (gptr<float>)B.metric32, g.jac32 = (gptr<float>)B.jac32, g.dq32 = (gptr<float>)B.dq32;

@@ -71,9 +71,8 @@ struct cfx_ctx {
   BlockView *views_dev = nullptr;
   std::vector<double *> coords;  // per block: the (ngg+1)-layer coordinates built on the device
   Table t_pad, t_int, t_dt, t_last, t_res;     // point-kernel workgroup prefixes
-  Table t_plane;                               // LU-SGS sweeps: one hyperplane's slots per block
-  Table t_lsk;                                 // LU-SGS pack: every hyperplane slot per block
-  int n_planes = 0;                            // LU-SGS hyperplanes of the largest block
+  Table t_tile;                                // LU-SGS sweeps: one workgroup per (j, k) tile column
+  int n_tplanes = 0;                           // LU-SGS tile hyperplanes of the largest block
   bool lusgs = false;
   Table t_rhs[3][kNVar3];                      // pencil tiles per direction and variant
   int occ[2][3][kNVar3] = {};                  // resident workgroups per CU [fp32][dir][variant]

@@ -19,20 +19,9 @@ constexpr int kTpb = 256;        // threads per workgroup of the point kernels
 constexpr int kDtIter = 8;       // points per thread of the time-step reduction
 constexpr int kLastIter = 4;     // points per thread of the last-step snapshot
 constexpr int kResIter = 8;      // points per thread of the residual partials
-constexpr int kSweepTpb = 64;    // threads per workgroup of the LU-SGS hyperplane sweeps
-
-// LU-SGS hyperplane layout of a block with extents m: the plane index s fixes
-// the coordinate on the longest axis a (s - u - v); b < c are the in-plane
-// axes, b the fastest.  Variables per slot for nv conserved variables: dQ
-// (nv), the conserved state (nv), p, gamma, J grad xi_d (3 x 3), a_d (3), D.
-inline void lusgs_axes(const int m[3], int &a, int &b, int &c) {
-  a = 0;
-  for (int d = 1; d < 3; ++d)
-    if (m[d] > m[a]) a = d;
-  b = a == 0 ? 1 : 0;
-  c = a == 2 ? 1 : 2;
-}
-inline int lusgs_vars(int nv) { return 2 * nv + 15; }
+// LU-SGS sweep tiles: kLuTi x kLuTj x kLuTk cells (i, j, k), one workgroup
+// each, whose first wavefront's lanes own the (j, k) columns
+constexpr int kLuTi = 4, kLuTj = 8, kLuTk = 8;
 
 // A batch of blocks: views[0..nb) with workgroup prefix off[0..nb] (device
 // pointers, off[0] need not be 0) and total = off[nb] - off[0] workgroups.
@@ -96,13 +85,12 @@ struct KernelSet64 {
   // rho*D) from the FP64 transport; lim = t_lo, t_hi, log p_lo, log p_hi
   int (*mlp_samples)(const SpeciesDev *sp, const SchemeDev &S, long long n, unsigned long long seed,
                      const double lim[4], double alpha, double *x, double *y, hipStream_t st);
-  // LU-SGS: pack the sweep operands and coefficients in hyperplane order
-  // (batch of every slot of a block), one hyperplane i + j + k = plane of the
-  // forward (fwd = 1) or backward sweep (batch of one plane's slots per
-  // block), and unpack dQ into dq (point batch over the interior)
-  int (*lusgs_pack)(const Batch &b, const SchemeDev &S, hipStream_t st);
-  int (*lusgs_unpack)(const Batch &b, hipStream_t st);
-  int (*lusgs_sweep)(const Batch &b, const SchemeDev &S, int plane, int fwd, hipStream_t st);
+  // LU-SGS: the diagonal and neighbour coefficients over the interior (point
+  // batch), and the tiles of one tile hyperplane I + J + K = tplane of the
+  // forward (fwd = 1) or backward sweep (batch of one workgroup per (J, K)
+  // tile column of each block)
+  int (*lusgs_coef)(const Batch &b, const SchemeDev &S, hipStream_t st);
+  int (*lusgs_sweep)(const Batch &b, const SchemeDev &S, int tplane, int fwd, hipStream_t st);
 };
 
 struct MlpDev;

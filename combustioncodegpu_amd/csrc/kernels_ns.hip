@@ -680,180 +680,193 @@ int launch_update(const Batch &b, const SpeciesDev *sp, const SchemeDev &S, int 
 //   backward dQ_i = dQ*_i + sum_d [ a_d(i+e_d) dQ_(i+e_d) - dG_d(i+e_d)/2 ] / D_i
 // dG_d(n) = G_d(Q_n + dQ_n) - G_d(Q_n): the inviscid flux of cell n through
 // S_d = J grad xi_d, the pressure change at the cell's frozen gamma; dQ = 0
-// outside the block.  The sweeps run hyperplane by hyperplane, i + j + k =
-// plane, every cell of a plane independent, in place on dQ (a cell reads
-// only final neighbour values).
-//
-// No direction of a hyperplane is contiguous in the block's own layout, so
-// the sweeps work on a copy of their operands in hyperplane order
-// (BlockView::lsk): a plane's slots are contiguous, and a cell's neighbour
-// along the plane-index axis sits one plane back in the same slot, along the
-// fastest in-plane axis one plane and one slot back, along the other one
-// plane and one row back.  Every sweep access then coalesces; the pack
-// kernel (which also forms the coefficients) and the unpack kernel pay the
-// transposition once per step, reading the block layout at strided points.
-constexpr int kNv = 5 + NS;
-constexpr int kLskCv = kNv, kLskP = 2 * kNv, kLskGam = 2 * kNv + 1, kLskS = 2 * kNv + 2, kLskA = 2 * kNv + 11,
-              kLskD = 2 * kNv + 14;
+// outside the block.  dq holds R on entry and dQ on exit (in place: a cell
+// reads only final neighbour values).
 
-// Slot z of a block's hyperplane layout -> plane s and cell coordinates c;
-// false for the slots of a plane that lie outside the block.
-__device__ __forceinline__ bool lusgs_slot_cell(const GView &B, unsigned z, unsigned &s, int c[3]) {
-  const unsigned plane = (unsigned)B.sk_plane, n1 = (unsigned)B.sk_n1;
-  s = z / plane;
-  const unsigned t = z - s * plane;
-  const int u = (int)(t % n1), v = (int)(t / n1);
-  const int m[3] = {B.mx, B.my, B.mz};
-  c[B.sk_b] = u;
-  c[B.sk_c] = v;
-  c[B.sk_a] = (int)s - u - v;
-  return c[B.sk_a] >= 0 && c[B.sk_a] < m[B.sk_a];
-}
-
-// One thread per slot, in slot order (the strided side is the read): R, the
-// state and geometry the flux change reads, and the coefficients D and a_d.
-__global__ void __launch_bounds__(kTpb) lusgs_pack_kernel(const BlockView *__restrict__ views,
+// lus: D over the interior, then a_d for d = 0, 1, 2
+__global__ void __launch_bounds__(kTpb) lusgs_coef_kernel(const BlockView *__restrict__ views,
                                                           const int *__restrict__ off, int nb, SchemeDev S) {
   const PointWork pw = point_work(off, nb);
   const GView B = gview(views[pw.blk]);
-  const long long z = pw.base + threadIdx.x;
-  if (z >= B.sk_slots) return;
-  unsigned s;
-  int c[3];
-  if (!lusgs_slot_cell(B, (unsigned)z, s, c)) return;
-  const long long np = B.np, ni = B.ni, ns = B.sk_slots;
-  const long long q = ((long long)c[2] * B.my + c[1]) * B.mx + c[0];
-  const long long p = pidx(B, c[0], c[1], c[2]);
+  const long long np = B.np, ni = B.ni;
+  const long long q = pw.base + threadIdx.x;
+  if (q >= ni) return;
+  const long long p = interior_pidx(B, q);
   const double J = B.jac[p], rho = B.bv[p];
   const double u = B.bv[np + p], v = B.bv[2 * np + p], w = B.bv[3 * np + p];
-  const double gam = B.props[p], cs = B.props[np + p], mu = B.props[2 * np + p];
+  const double gam = B.props[p], c = B.props[np + p], mu = B.props[2 * np + p];
   const double vc = fmax(gam, 4.0 / 3.0) * mu / rho / S.pr;
   double diag = J / B.dt_local[q];
   for (int d = 0; d < S.dim; ++d) {
     const double gx = B.metric[(3 * d) * np + p], gy = B.metric[(3 * d + 1) * np + p],
                  gz = B.metric[(3 * d + 2) * np + p];
     const double g2 = gx * gx + gy * gy + gz * gz;
-    const double r = J * (fabs(u * gx + v * gy + w * gz) + cs * sqrt(g2));
+    const double r = J * (fabs(u * gx + v * gy + w * gz) + c * sqrt(g2));
     const double vd = vc * J * g2;
     diag += S.omega * r + 2 * vd;
-    B.lsk[(kLskA + d) * ns + z] = 0.5 * S.omega * r + vd;
-    B.lsk[(kLskS + 3 * d) * ns + z] = J * gx;
-    B.lsk[(kLskS + 3 * d + 1) * ns + z] = J * gy;
-    B.lsk[(kLskS + 3 * d + 2) * ns + z] = J * gz;
+    B.lus[(1 + d) * ni + q] = 0.5 * S.omega * r + vd;
   }
-  B.lsk[kLskD * ns + z] = diag;
-  B.lsk[kLskP * ns + z] = B.bv[4 * np + p];
-  B.lsk[kLskGam * ns + z] = gam;
-#pragma unroll
-  for (int l = 0; l < kNv; ++l) {
-    B.lsk[l * ns + z] = B.dq[l * ni + q];
-    B.lsk[(kLskCv + l) * ns + z] = B.cv[l * np + p];
-  }
+  B.lus[q] = diag;
 }
 
-// dG_d of the cell in slot z for the increment dQ
-__device__ __forceinline__ void lusgs_flux_change(const GView &B, int d, long long z, const double *dQ, double *dG) {
-  const long long ns = B.sk_slots;
-  const double Sx = B.lsk[(kLskS + 3 * d) * ns + z], Sy = B.lsk[(kLskS + 3 * d + 1) * ns + z],
-               Sz = B.lsk[(kLskS + 3 * d + 2) * ns + z];
-  const double gam = B.lsk[kLskGam * ns + z];
-  double q0[kNv], q1[kNv];
-#pragma unroll
-  for (int l = 0; l < kNv; ++l) {
-    q0[l] = B.lsk[(kLskCv + l) * ns + z];
-    q1[l] = q0[l] + dQ[l];
-  }
-  const double p0 = B.lsk[kLskP * ns + z];
-  const double ke0 = 0.5 * (q0[1] * q0[1] + q0[2] * q0[2] + q0[3] * q0[3]) / q0[0];
-  const double ke1 = 0.5 * (q1[1] * q1[1] + q1[2] * q1[2] + q1[3] * q1[3]) / q1[0];
-  const double p1 = p0 + (gam - 1) * (dQ[4] - (ke1 - ke0));
-  const double U0 = (Sx * q0[1] + Sy * q0[2] + Sz * q0[3]) / q0[0];
-  const double U1 = (Sx * q1[1] + Sy * q1[2] + Sz * q1[3]) / q1[0];
-  dG[0] = q1[0] * U1 - q0[0] * U0;
-  dG[1] = (q1[1] * U1 + Sx * p1) - (q0[1] * U0 + Sx * p0);
-  dG[2] = (q1[2] * U1 + Sy * p1) - (q0[2] * U0 + Sy * p0);
-  dG[3] = (q1[3] * U1 + Sz * p1) - (q0[3] * U0 + Sz * p0);
-  dG[4] = (q1[4] + p1) * U1 - (q0[4] + p0) * U0;
-#pragma unroll
-  for (int l = 5; l < kNv; ++l) dG[l] = q1[l] * U1 - q0[l] * U0;
+// The sweeps in tiles of kLuTi x kLuTj x kLuTk cells.  Tiles (I, J, K) on
+// one tile hyperplane I + J + K = tplane depend only on tiles of the
+// previous one, so one launch runs a tile hyperplane, one workgroup per
+// tile.  The workgroup first stages in LDS, for its tile and the upstream
+// face layers around it (the cells at i0-1, j0-1, k0-1 for the forward
+// sweep, past the far faces for the backward one, final since an earlier
+// launch), each cell's dQ slot (R or dQ* on entry), conserved state,
+// pressure, gamma, J grad xi_d, a_d and D.  Lane (jl, kl) of the first
+// wavefront then owns the cell column (j, k) and marches along i: at local
+// step s it updates the cell il = s - jl - kl, whose upstream neighbours
+// were updated at step s-1 or
+// are staged; every read of the dependent steps is an LDS read, and a
+// cell's slot is written once, after every read of its input.  The tile's
+// increments go back to dq at the end.  The divisions share a
+// Newton-refined reciprocal per denominator.
+constexpr int kLuEi = kLuTi + 1, kLuEj = kLuTj + 1, kLuEk = kLuTk + 1, kLuExt = kLuEi * kLuEj * kLuEk;
+// a / b with r = frcp(b) shared by the quotients over one b: one residual
+// correction of a r (within an ulp of the IEEE quotient)
+__device__ __forceinline__ double lu_div(double a, double b, double r) {
+  const double q = a * r;
+  return fma(r, fma(-b, q, a), q);
 }
+// staged values per cell: dQ (NV), state (NV), p, gamma, S_d (3 x 3), a_d (3), D
+constexpr int kLuVars = 2 * (5 + NS) + 15;
+// four wavefronts stage the tile and write it back; the first one runs the
+// dependent steps (a tile's steps are one wavefront's work)
+constexpr int kLuThreads = 256;
 
-// One hyperplane of a sweep: thread t of a block is slot t of the plane.
-// A plane holds at most sk_plane cells, too few to fill the device with
-// kTpb-thread workgroups, so the sweeps use one wavefront per workgroup
-// (kSweepTpb), which spreads a plane over four times as many CUs.
 template <bool FWD>
-__global__ void __launch_bounds__(kSweepTpb) lusgs_sweep_kernel(const BlockView *__restrict__ views,
-                                                                const int *__restrict__ off, int nb, SchemeDev S,
-                                                                int plane) {
+__global__ void __launch_bounds__(kLuThreads) lusgs_tile_kernel(const BlockView *__restrict__ views,
+                                                        const int *__restrict__ off, int nb, SchemeDev S,
+                                                        int tplane) {
+  constexpr int NV = 5 + NS, STEPS = kLuTi + kLuTj + kLuTk - 2;
+  constexpr int XQ = 0, XC = NV, XP = 2 * NV, XG = 2 * NV + 1, XS = 2 * NV + 2, XA = 2 * NV + 11, XD = 2 * NV + 14;
+  static_assert(kLuTj * kLuTk == 64, "one lane per (j, k) column of a tile");
+  static_assert(kLuVars * kLuExt * 8 <= 160 * 1024, "the staged tile fits in LDS");
+  __shared__ double ext[kLuVars][kLuExt];
   const int w = blockIdx.x + off[0];
   const int blk = find_seg(off, nb, w);
   const GView B = gview(views[blk]);
-  const long long t = (long long)(w - off[blk]) * kSweepTpb + threadIdx.x;
-  if (t >= B.sk_plane) return;
-  const long long z = (long long)plane * B.sk_plane + t;
-  if (z >= B.sk_slots) return;
-  unsigned s;
-  int c[3];
-  if (!lusgs_slot_cell(B, (unsigned)z, s, c)) return;
   const int m[3] = {B.mx, B.my, B.mz};
-  const long long ns = B.sk_slots;
-  double acc[kNv];
+  const int nt0 = (m[0] + kLuTi - 1) / kLuTi, nt1 = (m[1] + kLuTj - 1) / kLuTj, nt2 = (m[2] + kLuTk - 1) / kLuTk;
+  const int t = w - off[blk];
+  const int T1 = t % nt1, T2 = t / nt1, T0 = tplane - T1 - T2;
+  if (T2 >= nt2 || T0 < 0 || T0 >= nt0) return;  // uniform over the workgroup
+  const int lane = threadIdx.x;
+  const int org[3] = {T0 * kLuTi - (FWD ? 1 : 0), T1 * kLuTj - (FWD ? 1 : 0), T2 * kLuTk - (FWD ? 1 : 0)};
+  const long long np = B.np, ni = B.ni;
+  // ---- stage the tile and its upstream faces
+  for (int e = lane; e < kLuExt; e += kLuThreads) {
+    const int c[3] = {org[0] + e % kLuEi, org[1] + (e / kLuEi) % kLuEj, org[2] + e / (kLuEi * kLuEj)};
+    if (c[0] < 0 || c[0] >= m[0] || c[1] < 0 || c[1] >= m[1] || c[2] < 0 || c[2] >= m[2]) continue;
+    const long long q = ((long long)c[2] * m[1] + c[1]) * m[0] + c[0];
+    const long long p = pidx(B, c[0], c[1], c[2]);
 #pragma unroll
-  for (int l = 0; l < kNv; ++l) acc[l] = FWD ? B.lsk[l * ns + z] : 0.0;
-  for (int d = 0; d < S.dim; ++d) {
-    if (FWD ? c[d] == 0 : c[d] == m[d] - 1) continue;
-    const long long dz = B.sk_plane + (d == B.sk_b ? 1 : (d == B.sk_c ? B.sk_n1 : 0));
-    const long long zn = FWD ? z - dz : z + dz;
-    double dn[kNv], dG[kNv];
+    for (int l = 0; l < NV; ++l) {
+      ext[XQ + l][e] = B.dq[l * ni + q];
+      ext[XC + l][e] = B.cv[l * np + p];
+    }
+    ext[XP][e] = B.bv[4 * np + p];
+    ext[XG][e] = B.props[p];
+    const double J = B.jac[p];
 #pragma unroll
-    for (int l = 0; l < kNv; ++l) dn[l] = B.lsk[l * ns + zn];
-    lusgs_flux_change(B, d, zn, dn, dG);
-    const double an = B.lsk[(kLskA + d) * ns + zn];
+    for (int v = 0; v < 9; ++v) ext[XS + v][e] = J * B.metric[v * np + p];
 #pragma unroll
-    for (int l = 0; l < kNv; ++l) acc[l] += FWD ? 0.5 * dG[l] + an * dn[l] : an * dn[l] - 0.5 * dG[l];
+    for (int d = 0; d < 3; ++d) ext[XA + d][e] = B.lus[(1 + d) * ni + q];
+    ext[XD][e] = B.lus[q];
   }
-  const double D = B.lsk[kLskD * ns + z];
+  __syncthreads();
+  // ---- the dependent steps, in LDS
+  const int jl = lane % kLuTj, kl = lane / kLuTj;
+  const int j = T1 * kLuTj + jl, k = T2 * kLuTk + kl;
+  const bool col = lane < 64 && j < m[1] && k < m[2];
+  const int o = FWD ? 1 : 0;  // position of the tile's first cell in the staged box
+  const int es[3] = {1, kLuEi, kLuEi * kLuEj};
+  for (int step = 0; step < STEPS; ++step) {
+    const int il = (FWD ? step : STEPS - 1 - step) - jl - kl;
+    const int i = T0 * kLuTi + il;
+    if (col && il >= 0 && il < kLuTi && i < m[0]) {
+      const int ijk[3] = {i, j, k};
+      const int e = ((kl + o) * kLuEj + (jl + o)) * kLuEi + (il + o);
+      // the three neighbours' flux changes are independent chains (long
+      // FP64 divisions): computed side by side, then accumulated in the
+      // order d = 0, 1, 2
+      double dn[3][NV], dG[3][NV], an[3];
+      bool has[3];
 #pragma unroll
-  for (int l = 0; l < kNv; ++l) {
-    if (FWD) B.lsk[l * ns + z] = acc[l] / D;
-    else B.lsk[l * ns + z] += acc[l] / D;
+      for (int d = 0; d < 3; ++d) {
+        has[d] = d < S.dim && (FWD ? ijk[d] > 0 : ijk[d] < m[d] - 1);
+        if (!has[d]) continue;
+        const int en = FWD ? e - es[d] : e + es[d];
+        // dG_d = G_d(Q + dQ) - G_d(Q) of the neighbour, pressure change at
+        // its frozen gamma
+        double q0[NV], q1[NV];
+#pragma unroll
+        for (int l = 0; l < NV; ++l) {
+          dn[d][l] = ext[XQ + l][en];
+          q0[l] = ext[XC + l][en];
+          q1[l] = q0[l] + dn[d][l];
+        }
+        const double Sx = ext[XS + 3 * d][en], Sy = ext[XS + 3 * d + 1][en], Sz = ext[XS + 3 * d + 2][en];
+        const double gam = ext[XG][en], p0 = ext[XP][en];
+        const double r0 = frcp<double>(q0[0]), r1 = frcp<double>(q1[0]);
+        const double ke0 = lu_div(0.5 * (q0[1] * q0[1] + q0[2] * q0[2] + q0[3] * q0[3]), q0[0], r0);
+        const double ke1 = lu_div(0.5 * (q1[1] * q1[1] + q1[2] * q1[2] + q1[3] * q1[3]), q1[0], r1);
+        const double p1 = p0 + (gam - 1) * (dn[d][4] - (ke1 - ke0));
+        const double U0 = lu_div(Sx * q0[1] + Sy * q0[2] + Sz * q0[3], q0[0], r0);
+        const double U1 = lu_div(Sx * q1[1] + Sy * q1[2] + Sz * q1[3], q1[0], r1);
+        dG[d][0] = q1[0] * U1 - q0[0] * U0;
+        dG[d][1] = (q1[1] * U1 + Sx * p1) - (q0[1] * U0 + Sx * p0);
+        dG[d][2] = (q1[2] * U1 + Sy * p1) - (q0[2] * U0 + Sy * p0);
+        dG[d][3] = (q1[3] * U1 + Sz * p1) - (q0[3] * U0 + Sz * p0);
+        dG[d][4] = (q1[4] + p1) * U1 - (q0[4] + p0) * U0;
+#pragma unroll
+        for (int l = 5; l < NV; ++l) dG[d][l] = q1[l] * U1 - q0[l] * U0;
+        an[d] = ext[XA + d][en];
+      }
+      double acc[NV];
+#pragma unroll
+      for (int l = 0; l < NV; ++l) acc[l] = FWD ? ext[XQ + l][e] : 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (!has[d]) continue;
+#pragma unroll
+        for (int l = 0; l < NV; ++l)
+          acc[l] += FWD ? 0.5 * dG[d][l] + an[d] * dn[d][l] : an[d] * dn[d][l] - 0.5 * dG[d][l];
+      }
+      const double D = ext[XD][e], rD = frcp<double>(D);
+#pragma unroll
+      for (int l = 0; l < NV; ++l)
+        ext[XQ + l][e] = FWD ? lu_div(acc[l], D, rD) : ext[XQ + l][e] + lu_div(acc[l], D, rD);
+    }
+    __syncthreads();
+  }
+  // ---- the tile's increments back to dq
+  for (int c = lane; c < kLuTi * kLuTj * kLuTk; c += kLuThreads) {
+    const int il = c % kLuTi, jc = (c / kLuTi) % kLuTj, kc = c / (kLuTi * kLuTj);
+    const int i = T0 * kLuTi + il, jg = T1 * kLuTj + jc, kg = T2 * kLuTk + kc;
+    if (i >= m[0] || jg >= m[1] || kg >= m[2]) continue;
+    const int e = ((kc + o) * kLuEj + (jc + o)) * kLuEi + (il + o);
+    const long long q = ((long long)kg * m[1] + jg) * m[0] + i;
+#pragma unroll
+    for (int l = 0; l < NV; ++l) B.dq[l * ni + q] = ext[XQ + l][e];
   }
 }
 
-// dq <- dQ, one thread per interior cell in the block's order
-__global__ void __launch_bounds__(kTpb) lusgs_unpack_kernel(const BlockView *__restrict__ views,
-                                                            const int *__restrict__ off, int nb) {
-  const PointWork pw = point_work(off, nb);
-  const GView B = gview(views[pw.blk]);
-  const long long q = pw.base + threadIdx.x;
-  if (q >= B.ni) return;
-  const unsigned uq = (unsigned)q, mx = (unsigned)B.mx, my = (unsigned)B.my;
-  const int c[3] = {(int)(uq % mx), (int)((uq / mx) % my), (int)(uq / mx / my)};
-  const long long z = (long long)(c[0] + c[1] + c[2]) * B.sk_plane + (long long)c[B.sk_c] * B.sk_n1 + c[B.sk_b];
-  const long long ns = B.sk_slots, ni = B.ni;
-#pragma unroll
-  for (int l = 0; l < kNv; ++l) B.dq[l * ni + q] = B.lsk[l * ns + z];
-}
-
-int launch_lusgs_pack(const Batch &b, const SchemeDev &S, hipStream_t st) {
+int launch_lusgs_coef(const Batch &b, const SchemeDev &S, hipStream_t st) {
   if (b.total <= 0) return 0;
-  lusgs_pack_kernel<<<b.total, kTpb, 0, st>>>(b.views, b.off, b.nb, S);
+  lusgs_coef_kernel<<<b.total, kTpb, 0, st>>>(b.views, b.off, b.nb, S);
   return (int)hipGetLastError();
 }
 
-int launch_lusgs_sweep(const Batch &b, const SchemeDev &S, int plane, int fwd, hipStream_t st) {
+int launch_lusgs_sweep(const Batch &b, const SchemeDev &S, int tplane, int fwd, hipStream_t st) {
   if (b.total <= 0) return 0;
   if (fwd)
-    lusgs_sweep_kernel<true><<<b.total, kSweepTpb, 0, st>>>(b.views, b.off, b.nb, S, plane);
+    lusgs_tile_kernel<true><<<b.total, kLuThreads, 0, st>>>(b.views, b.off, b.nb, S, tplane);
   else
-    lusgs_sweep_kernel<false><<<b.total, kSweepTpb, 0, st>>>(b.views, b.off, b.nb, S, plane);
-  return (int)hipGetLastError();
-}
-
-int launch_lusgs_unpack(const Batch &b, hipStream_t st) {
-  if (b.total <= 0) return 0;
-  lusgs_unpack_kernel<<<b.total, kTpb, 0, st>>>(b.views, b.off, b.nb);
+    lusgs_tile_kernel<false><<<b.total, kLuThreads, 0, st>>>(b.views, b.off, b.nb, S, tplane);
   return (int)hipGetLastError();
 }
 
@@ -958,8 +971,7 @@ struct Register64 {
     k.cv_from_bv = launch_cv_from_bv;
     k.update = launch_update;
     k.face_ops = launch_face_ops;
-    k.lusgs_pack = launch_lusgs_pack;
-    k.lusgs_unpack = launch_lusgs_unpack;
+    k.lusgs_coef = launch_lusgs_coef;
     k.lusgs_sweep = launch_lusgs_sweep;
     register_rhs<double>(k);
     k.present = 1;
