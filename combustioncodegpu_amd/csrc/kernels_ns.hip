@@ -716,14 +716,15 @@ __global__ void __launch_bounds__(kTpb) lusgs_coef_kernel(const BlockView *__res
 // face layers around it (the cells at i0-1, j0-1, k0-1 for the forward
 // sweep, past the far faces for the backward one, final since an earlier
 // launch), each cell's dQ slot (R or dQ* on entry), conserved state,
-// pressure, gamma, J grad xi_d, a_d and D.  Lane (jl, kl) of the first
-// wavefront then owns the cell column (j, k) and marches along i: at local
-// step s it updates the cell il = s - jl - kl, whose upstream neighbours
-// were updated at step s-1 or
+// pressure, gamma, J grad xi_d, a_d and D.  Lane (jl, kl) of a wavefront
+// then stands for the cell column (j, k), marching along i: at local step s
+// the cell il = s - jl - kl is updated, whose upstream neighbours were
+// updated at step s-1 or
 // are staged; every read of the dependent steps is an LDS read, and a
 // cell's slot is written once, after every read of its input.  The tile's
 // increments go back to dq at the end.  The divisions share a
-// Newton-refined reciprocal per denominator.
+// Newton-refined reciprocal per denominator; the three neighbour terms of a
+// cell are formed by three wavefronts at once and added in a fixed order.
 constexpr int kLuEi = kLuTi + 1, kLuEj = kLuTj + 1, kLuEk = kLuTk + 1, kLuExt = kLuEi * kLuEj * kLuEk;
 // a / b with r = frcp(b) shared by the quotients over one b: one residual
 // correction of a r (within an ulp of the IEEE quotient)
@@ -733,8 +734,8 @@ __device__ __forceinline__ double lu_div(double a, double b, double r) {
 }
 // staged values per cell: dQ (NV), state (NV), p, gamma, S_d (3 x 3), a_d (3), D
 constexpr int kLuVars = 2 * (5 + NS) + 15;
-// four wavefronts stage the tile and write it back; the first one runs the
-// dependent steps (a tile's steps are one wavefront's work)
+// four wavefronts stage the tile and write it back; in the dependent steps
+// wavefronts 0-2 form one upstream direction's terms each
 constexpr int kLuThreads = 256;
 
 template <bool FWD>
@@ -744,8 +745,9 @@ __global__ void __launch_bounds__(kLuThreads) lusgs_tile_kernel(const BlockView 
   constexpr int NV = 5 + NS, STEPS = kLuTi + kLuTj + kLuTk - 2;
   constexpr int XQ = 0, XC = NV, XP = 2 * NV, XG = 2 * NV + 1, XS = 2 * NV + 2, XA = 2 * NV + 11, XD = 2 * NV + 14;
   static_assert(kLuTj * kLuTk == 64, "one lane per (j, k) column of a tile");
-  static_assert(kLuVars * kLuExt * 8 <= 160 * 1024, "the staged tile fits in LDS");
+  static_assert((kLuVars * kLuExt + 3 * NV * 64) * 8 <= 160 * 1024, "the staged tile fits in LDS");
   __shared__ double ext[kLuVars][kLuExt];
+  __shared__ double term[3][NV][64];  // the step's neighbour terms, by direction
   const int w = blockIdx.x + off[0];
   const int blk = find_seg(off, nb, w);
   const GView B = gview(views[blk]);
@@ -778,63 +780,62 @@ __global__ void __launch_bounds__(kLuThreads) lusgs_tile_kernel(const BlockView 
     ext[XD][e] = B.lus[q];
   }
   __syncthreads();
-  // ---- the dependent steps, in LDS
-  const int jl = lane % kLuTj, kl = lane / kLuTj;
+  // ---- the dependent steps, in LDS: wavefront d (0..2) forms the term of
+  // the upstream neighbour along d for every column of the step, wavefront 0
+  // then adds them in the order d = 0, 1, 2 and updates the cells
+  const int wv = lane >> 6, cl = lane & 63;
+  const int jl = cl % kLuTj, kl = cl / kLuTj;
   const int j = T1 * kLuTj + jl, k = T2 * kLuTk + kl;
-  const bool col = lane < 64 && j < m[1] && k < m[2];
+  const bool col = j < m[1] && k < m[2];
   const int o = FWD ? 1 : 0;  // position of the tile's first cell in the staged box
   const int es[3] = {1, kLuEi, kLuEi * kLuEj};
   for (int step = 0; step < STEPS; ++step) {
     const int il = (FWD ? step : STEPS - 1 - step) - jl - kl;
     const int i = T0 * kLuTi + il;
-    if (col && il >= 0 && il < kLuTi && i < m[0]) {
-      const int ijk[3] = {i, j, k};
-      const int e = ((kl + o) * kLuEj + (jl + o)) * kLuEi + (il + o);
-      // the three neighbours' flux changes are independent chains (long
-      // FP64 divisions): computed side by side, then accumulated in the
-      // order d = 0, 1, 2
-      double dn[3][NV], dG[3][NV], an[3];
-      bool has[3];
+    const bool act = col && il >= 0 && il < kLuTi && i < m[0];
+    const int ijk[3] = {i, j, k};
+    const int e = ((kl + o) * kLuEj + (jl + o)) * kLuEi + (il + o);
+    if (act && wv < S.dim && (FWD ? ijk[wv] > 0 : ijk[wv] < m[wv] - 1)) {
+      const int d = wv;
+      const int en = FWD ? e - es[d] : e + es[d];
+      // dG_d = G_d(Q + dQ) - G_d(Q) of the neighbour, pressure change at its
+      // frozen gamma
+      double dn[NV], q0[NV], q1[NV], dG[NV];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        has[d] = d < S.dim && (FWD ? ijk[d] > 0 : ijk[d] < m[d] - 1);
-        if (!has[d]) continue;
-        const int en = FWD ? e - es[d] : e + es[d];
-        // dG_d = G_d(Q + dQ) - G_d(Q) of the neighbour, pressure change at
-        // its frozen gamma
-        double q0[NV], q1[NV];
-#pragma unroll
-        for (int l = 0; l < NV; ++l) {
-          dn[d][l] = ext[XQ + l][en];
-          q0[l] = ext[XC + l][en];
-          q1[l] = q0[l] + dn[d][l];
-        }
-        const double Sx = ext[XS + 3 * d][en], Sy = ext[XS + 3 * d + 1][en], Sz = ext[XS + 3 * d + 2][en];
-        const double gam = ext[XG][en], p0 = ext[XP][en];
-        const double r0 = frcp<double>(q0[0]), r1 = frcp<double>(q1[0]);
-        const double ke0 = lu_div(0.5 * (q0[1] * q0[1] + q0[2] * q0[2] + q0[3] * q0[3]), q0[0], r0);
-        const double ke1 = lu_div(0.5 * (q1[1] * q1[1] + q1[2] * q1[2] + q1[3] * q1[3]), q1[0], r1);
-        const double p1 = p0 + (gam - 1) * (dn[d][4] - (ke1 - ke0));
-        const double U0 = lu_div(Sx * q0[1] + Sy * q0[2] + Sz * q0[3], q0[0], r0);
-        const double U1 = lu_div(Sx * q1[1] + Sy * q1[2] + Sz * q1[3], q1[0], r1);
-        dG[d][0] = q1[0] * U1 - q0[0] * U0;
-        dG[d][1] = (q1[1] * U1 + Sx * p1) - (q0[1] * U0 + Sx * p0);
-        dG[d][2] = (q1[2] * U1 + Sy * p1) - (q0[2] * U0 + Sy * p0);
-        dG[d][3] = (q1[3] * U1 + Sz * p1) - (q0[3] * U0 + Sz * p0);
-        dG[d][4] = (q1[4] + p1) * U1 - (q0[4] + p0) * U0;
-#pragma unroll
-        for (int l = 5; l < NV; ++l) dG[d][l] = q1[l] * U1 - q0[l] * U0;
-        an[d] = ext[XA + d][en];
+      for (int l = 0; l < NV; ++l) {
+        dn[l] = ext[XQ + l][en];
+        q0[l] = ext[XC + l][en];
+        q1[l] = q0[l] + dn[l];
       }
+      const double Sx = ext[XS + 3 * d][en], Sy = ext[XS + 3 * d + 1][en], Sz = ext[XS + 3 * d + 2][en];
+      const double gam = ext[XG][en], p0 = ext[XP][en];
+      const double r0 = frcp<double>(q0[0]), r1 = frcp<double>(q1[0]);
+      const double ke0 = lu_div(0.5 * (q0[1] * q0[1] + q0[2] * q0[2] + q0[3] * q0[3]), q0[0], r0);
+      const double ke1 = lu_div(0.5 * (q1[1] * q1[1] + q1[2] * q1[2] + q1[3] * q1[3]), q1[0], r1);
+      const double p1 = p0 + (gam - 1) * (dn[4] - (ke1 - ke0));
+      const double U0 = lu_div(Sx * q0[1] + Sy * q0[2] + Sz * q0[3], q0[0], r0);
+      const double U1 = lu_div(Sx * q1[1] + Sy * q1[2] + Sz * q1[3], q1[0], r1);
+      dG[0] = q1[0] * U1 - q0[0] * U0;
+      dG[1] = (q1[1] * U1 + Sx * p1) - (q0[1] * U0 + Sx * p0);
+      dG[2] = (q1[2] * U1 + Sy * p1) - (q0[2] * U0 + Sy * p0);
+      dG[3] = (q1[3] * U1 + Sz * p1) - (q0[3] * U0 + Sz * p0);
+      dG[4] = (q1[4] + p1) * U1 - (q0[4] + p0) * U0;
+#pragma unroll
+      for (int l = 5; l < NV; ++l) dG[l] = q1[l] * U1 - q0[l] * U0;
+      const double an = ext[XA + d][en];
+#pragma unroll
+      for (int l = 0; l < NV; ++l) term[d][l][cl] = FWD ? 0.5 * dG[l] + an * dn[l] : an * dn[l] - 0.5 * dG[l];
+    }
+    __syncthreads();
+    if (act && wv == 0) {
       double acc[NV];
 #pragma unroll
       for (int l = 0; l < NV; ++l) acc[l] = FWD ? ext[XQ + l][e] : 0.0;
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        if (!has[d]) continue;
+        if (!(d < S.dim && (FWD ? ijk[d] > 0 : ijk[d] < m[d] - 1))) continue;
 #pragma unroll
-        for (int l = 0; l < NV; ++l)
-          acc[l] += FWD ? 0.5 * dG[d][l] + an[d] * dn[d][l] : an[d] * dn[d][l] - 0.5 * dG[d][l];
+        for (int l = 0; l < NV; ++l) acc[l] += term[d][l][cl];
       }
       const double D = ext[XD][e], rD = frcp<double>(D);
 #pragma unroll
