@@ -194,17 +194,34 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
     constexpr int NVB = (NVV + KV - 1) / KV;       // viscous batches
     const bool viscous = S.viscous == 2;
     R *const buf0 = stg, *const buf1 = stg + P::STG;
-    // issue variables [v0, v0 + K) of box BX (origin bo, LDS stride VP)
-    // into buf: lane e of the box loads element e of each variable, the
-    // wavefront writing 64 consecutive slots from its first element on
-    // (FP64: lane pairs share an element, one loading each word)
-    auto issue = [&](auto box, auto kc, auto vpc, auto srcf, const int bo[3], R *buf) {
+    // The byte offsets of this lane's staged elements: lane e of a box
+    // loads element e of each variable, the wavefront writing 64 consecutive
+    // slots from its first element on (FP64: lane pairs share an element,
+    // one loading each word).  Computed once per tile for each box and kept
+    // for all its batches.
+    constexpr int W = P::W, EPW = P::EPW, NW = NT / 64;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    auto offsets = [&](auto box, const int bo[3], unsigned *gb) {
       using BX = decltype(box);
-      constexpr int K = decltype(kc)::value, VP = decltype(vpc)::value;
-      constexpr int W = P::W, EPW = P::EPW, NW = NT / 64;
       constexpr int NC = (BX::V + EPW - 1) / EPW;  // wavefront loads per variable
       constexpr int IT = (NC + NW - 1) / NW;
-      const int wave = tid >> 6, lane = tid & 63;
+#pragma unroll
+      for (int it = 0; it < IT; ++it) {
+        const int e0 = (it * NW + wave) * EPW + lane / W;
+        const int e = e0 < BX::V ? e0 : BX::V - 1;
+        const int x = min(bo[0] + e % BX::L0, m[0] + 1);
+        const int y = min(bo[1] + (e / BX::L0) % BX::L1, m[1] + 1);
+        const int z = min(bo[2] + e / (BX::L0 * BX::L1), m[2] + 1);
+        gb[it] = (unsigned)((z + kNgg) * szi + (y + kNgg) * syi + (x + kNgg)) * (unsigned)sizeof(R) +
+                 (unsigned)(lane % W) * 4u;
+      }
+    };
+    // issue variables [v0, v0 + K) of box BX (LDS stride VP) into buf
+    auto issue = [&](auto box, auto kc, auto vpc, auto srcf, const unsigned *gb, R *buf) {
+      using BX = decltype(box);
+      constexpr int K = decltype(kc)::value, VP = decltype(vpc)::value;
+      constexpr int NC = (BX::V + EPW - 1) / EPW;
+      constexpr int IT = (NC + NW - 1) / NW;
       // raised wavefront priority while the batch's loads are issued, so that
       // they go out ahead of other wavefronts' arithmetic (FP32 step -0.9%)
       __builtin_amdgcn_s_setprio(2);
@@ -212,17 +229,15 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
       for (int it = 0; it < IT; ++it) {
         const int chunk = it * NW + wave;
         if (chunk < NC) {  // uniform per wavefront
-          const int e0 = chunk * EPW + lane / W;
-          const int e = e0 < BX::V ? e0 : BX::V - 1;
-          const int x = min(bo[0] + e % BX::L0, m[0] + 1);
-          const int y = min(bo[1] + (e / BX::L0) % BX::L1, m[1] + 1);
-          const int z = min(bo[2] + e / (BX::L0 * BX::L1), m[2] + 1);
-          const unsigned gb = (unsigned)((z + kNgg) * szi + (y + kNgg) * syi + (x + kNgg)) * (unsigned)sizeof(R) +
-                              (unsigned)(lane % W) * 4u;
+          // the offset enters this block as a 32-bit value, so the loads
+          // use the scalar-base + 32-bit-offset form (a 64-bit offset kept
+          // from the start of the tile would need a 64-bit add per load)
+          unsigned go = gb[it];
+          asm volatile("" : "+v"(go));
 #pragma unroll
           for (int k = 0; k < K; ++k)
             __builtin_amdgcn_global_load_lds(
-                (const __attribute__((address_space(1))) void *)((gptr<const char>)srcf(k) + gb),
+                (const __attribute__((address_space(1))) void *)((gptr<const char>)srcf(k) + go),
                 (__attribute__((address_space(3))) void *)(buf + k * VP + chunk * EPW), 4, 0, 0);
         }
       }
@@ -237,18 +252,22 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
     int boi[3], bov[3];
     boi[A] = o[A] - 2, boi[Bx] = o[Bx], boi[Cx] = o[Cx];
     bov[A] = o[A] - 1, bov[Bx] = o[Bx] - 1, bov[Cx] = o[Cx] - 1;
+    constexpr int ITI = ((BI::V + EPW - 1) / EPW + NW - 1) / NW, ITV = ((BV::V + EPW - 1) / EPW + NW - 1) / NW;
+    unsigned gbi[ITI], gbv[ITV];
+    offsets(BI{}, boi, gbi);
+    if (viscous) offsets(BV{}, bov, gbv);
     auto issue_inv = [&](auto nc, R *buf) {
       constexpr int v0 = decltype(nc)::value * KI;
       constexpr int K = NV - v0 < KI ? NV - v0 : KI;
       issue(BI{}, std::integral_constant<int, K>{}, std::integral_constant<int, VPI>{},
-            [&](int k) { return v0 + k < 5 ? BV_ + (v0 + k) * np : YK_ + (v0 + k - 5) * np; }, boi, buf);
+            [&](int k) { return v0 + k < 5 ? BV_ + (v0 + k) * np : YK_ + (v0 + k - 5) * np; }, gbi, buf);
     };
     auto issue_vis = [&](auto nc, R *buf) {
       constexpr int j0 = decltype(nc)::value * KV;
       constexpr int K = NVV - j0 < KV ? NVV - j0 : KV;
       issue(BV{}, std::integral_constant<int, K>{}, std::integral_constant<int, VPV>{},
             [&](int k) { return j0 + k < 4 ? BV_ + (j0 + k < 3 ? j0 + k + 1 : 5) * np : YK_ + (j0 + k - 4) * np; },
-            bov, buf);
+            gbv, buf);
     };
 
     // ---- inviscid phase: F_inv into this thread's face slot
