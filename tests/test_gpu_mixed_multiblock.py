@@ -38,10 +38,12 @@ def mixed_case(name):
         return cases.c1(24, 20, 16)
     if name == "c1s_walls":  # inflow, outflow, isothermal and adiabatic walls, slip walls, 9 species
         return cases.c1_walls(22, 18, 14)
+    if name.startswith("c1s_n"):  # the other compiled species counts
+        return cases.c1(16, 14, 10, n_spec=int(name[5:]))
     return cases.c0()
 
 
-@pytest.mark.parametrize("name", ["c1s", "c0", "c1s_walls"])
+@pytest.mark.parametrize("name", ["c1s", "c0", "c1s_walls", "c1s_n1", "c1s_n3", "c1s_n5"])
 def test_mixed_precision_per_field_error(name):
     from combustioncodegpu_amd.lib import Solver
     from oracle.pyoracle import Oracle

@@ -45,11 +45,13 @@ def make(name):
         return cases.c1_walls(22, 18, 14)
     if name == "c1s_first":
         return cases.c1(18, 14, 12, reconstruction=1)
+    if name.startswith("c1s_n") and name[5:].isdigit():  # the other compiled species counts
+        return cases.c1(16, 14, 10, n_spec=int(name[5:]))
     raise KeyError(name)
 
 
 CASES = ["c0", "c0_euler", "c0_va", "c1s", "c1s_minmod", "c1s_ns2", "c1s_air", "c1s_curv_ref",
-         "c1s_curv_corr", "c1s_euler", "c1s_inviscid", "c1s_walls", "c1s_first"]
+         "c1s_curv_corr", "c1s_euler", "c1s_inviscid", "c1s_walls", "c1s_first", "c1s_n1", "c1s_n3", "c1s_n5"]
 
 
 def pair(name):
