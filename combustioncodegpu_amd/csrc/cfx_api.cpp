@@ -47,7 +47,7 @@ void round_table(const A &a, B &b) {
   };
   cp1(a.mw, b.mw); cp1(a.inv_mw, b.inv_mw); cp1(a.r_sp, b.r_sp); cp1(a.t_low, b.t_low); cp1(a.t_mid, b.t_mid);
   cp1(a.cpc, b.cpc); cp1(a.hc, b.hc); cp1(a.h_tlow, b.h_tlow); cp1(a.cp_tlow, b.cp_tlow); cp1(a.lj_inv, b.lj_inv);
-  cp1(a.log_lj_inv, b.log_lj_inv); cp1(a.vis_coeff, b.vis_coeff); cp1(a.cond_add, b.cond_add); cp1(a.wj4, b.wj4);
+  cp1(a.om145, b.om145); cp1(a.vis_coeff, b.vis_coeff); cp1(a.cond_add, b.cond_add); cp1(a.wj4, b.wj4);
   cp1(a.sqw8, b.sqw8);
 }
 
@@ -111,7 +111,7 @@ int build_species(const cfx_species *sp, SpeciesDev &tables, std::string &err) {
     s.cp_tlow[l] = c[0] + tt * (c[1] + tt * (c[2] + tt * (c[3] + tt * c[4])));
     s.h_tlow[l] = tt * (e[0] + tt * (e[1] + tt * (e[2] + tt * (e[3] + tt * e[4])))) + e[5];
     s.lj_inv[l] = 1.0 / sp->eps_k[l];
-    s.log_lj_inv[l] = std::log(s.lj_inv[l]);
+    s.om145[l] = std::pow(s.lj_inv[l], -0.145);
     s.vis_coeff[l] = 2.6693e-6 * std::sqrt(W) / (sp->sigma[l] * sp->sigma[l]);  // ChemData.cpp:249-250
     s.cond_add[l] = 1.25 * rs;
   }

@@ -45,7 +45,8 @@ struct SpeciesT {
   T cpc[2][kMaxSp][5];
   T hc[2][kMaxSp][6];
   T h_tlow[kMaxSp], cp_tlow[kMaxSp];
-  T lj_inv[kMaxSp], log_lj_inv[kMaxSp], vis_coeff[kMaxSp];
+  T lj_inv[kMaxSp], vis_coeff[kMaxSp];
+  T om145[kMaxSp];            // (k/eps)^-0.145: T*^-0.145 = T^-0.145 om145
   T cond_add[kMaxSp];        // 1.25 R_u / W_l (Transport.cu:59)
   T wj4[kMaxSp][kMaxSp];     // (W_n/W_m)^(1/4)           (ChemData.cpp:257)
   T sqw8[kMaxSp][kMaxSp];    // 1/sqrt(8(1+W_m/W_n))      (ChemData.cpp:258)

@@ -82,13 +82,14 @@ __device__ __forceinline__ void mixture_gamma_c(SpeciesPtrT<R> sp, R t, const R 
 
 // compute_transport_property (Transport.cu:43-96): out = mu, kappa,
 // rho*D_1..ns.  The Wilke sums use s_l = sqrt(mu_l) so that sqrt(mu_m/mu_n)
-// = s_m/s_n (NS square roots instead of NS^2) and T*^-0.145 =
-// exp(-0.145 log T*) with one shared log T.
+// = s_m/s_n (NS square roots instead of NS^2), and T*^-0.145 =
+// T^-0.145 (k/eps_l)^-0.145: one exponential per point, the species factor
+// from the table.
 template <typename R>
 __device__ __forceinline__ void wilke_point(SpeciesPtrT<R> sp, const SchemeDev &S, R t, const R *Y, const R *cp,
                                             R winv, R *out) {
   // Chapman-Enskog species viscosity (Transport.cu:52-57)
-  const R lt = flog<R>(t), st = fsqrt<R>(t);
+  const R tm = fexp<R>(R(-0.145) * flog<R>(t)), st = fsqrt<R>(t);
   R mu_s[NSA], X[NSA], s[NSA], rs[NSA];
   const R imw = frcp<R>(winv);
 #pragma unroll
@@ -96,7 +97,7 @@ __device__ __forceinline__ void wilke_point(SpeciesPtrT<R> sp, const SchemeDev &
     X[l] = Y[l] * imw * R(sp->inv_mw[l]);
     const R tdl = t * R(sp->lj_inv[l]);
     const R a = tdl + R(0.5);
-    const R omega = R(1.147) * fexp<R>(R(-0.145) * (lt + R(sp->log_lj_inv[l]))) + frcp<R>(a * a);
+    const R omega = R(1.147) * (tm * R(sp->om145[l])) + frcp<R>(a * a);
     mu_s[l] = fdiv<R>(R(sp->vis_coeff[l]) * st, omega);
     s[l] = fsqrt<R>(mu_s[l]);
     rs[l] = frcp<R>(s[l]);
