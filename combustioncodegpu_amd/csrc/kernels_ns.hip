@@ -727,6 +727,12 @@ __global__ void __launch_bounds__(kTpb) lusgs_coef_kernel(const BlockView *__res
 // Newton-refined reciprocal per denominator; the three neighbour terms of a
 // cell are formed by three wavefronts at once and added in a fixed order.
 constexpr int kLuEi = kLuTi + 1, kLuEj = kLuTj + 1, kLuEk = kLuTk + 1, kLuExt = kLuEi * kLuEj * kLuEk;
+// LDS slot of staged cell (i, j, k): i + kLuEi j + kLuSk k, the k planes one
+// slot apart so that the lanes of a step, which differ in j and k, spread
+// over the LDS banks (measured: bank-conflict cycles 40% of the LDS-active
+// cycles with kLuSk = 45, 7% with 46); kLuSlots slots per staged variable
+constexpr int kLuSk = kLuEi * kLuEj + 1, kLuSlots = (kLuEk - 1) * kLuSk + kLuEi * kLuEj;
+__device__ __forceinline__ int lu_slot(int i, int j, int k) { return (k * kLuSk + j * kLuEi) + i; }
 // a / b with r = frcp(b) shared by the quotients over one b: one residual
 // correction of a r (within an ulp of the IEEE quotient)
 __device__ __forceinline__ double lu_div(double a, double b, double r) {
@@ -746,8 +752,8 @@ __global__ void __launch_bounds__(kLuThreads) lusgs_tile_kernel(const BlockView 
   constexpr int NV = 5 + NS, STEPS = kLuTi + kLuTj + kLuTk - 2;
   constexpr int XQ = 0, XC = NV, XP = 2 * NV, XG = 2 * NV + 1, XS = 2 * NV + 2, XA = 2 * NV + 11, XD = 2 * NV + 14;
   static_assert(kLuTj * kLuTk == 64, "one lane per (j, k) column of a tile");
-  static_assert((kLuVars * kLuExt + 3 * NV * 64) * 8 <= 160 * 1024, "the staged tile fits in LDS");
-  __shared__ double ext[kLuVars][kLuExt];
+  static_assert((kLuVars * kLuSlots + 3 * NV * 64) * 8 <= 160 * 1024, "the staged tile fits in LDS");
+  __shared__ double ext[kLuVars][kLuSlots];
   __shared__ double term[3][NV][64];  // the step's neighbour terms, by direction
   const int w = blockIdx.x + off[0];
   const int blk = find_seg(off, nb, w);
@@ -761,8 +767,10 @@ __global__ void __launch_bounds__(kLuThreads) lusgs_tile_kernel(const BlockView 
   const int org[3] = {T0 * kLuTi - (FWD ? 1 : 0), T1 * kLuTj - (FWD ? 1 : 0), T2 * kLuTk - (FWD ? 1 : 0)};
   const long long np = B.np, ni = B.ni;
   // ---- stage the tile and its upstream faces
-  for (int e = lane; e < kLuExt; e += kLuThreads) {
-    const int c[3] = {org[0] + e % kLuEi, org[1] + (e / kLuEi) % kLuEj, org[2] + e / (kLuEi * kLuEj)};
+  for (int n = lane; n < kLuExt; n += kLuThreads) {
+    const int li = n % kLuEi, lj = (n / kLuEi) % kLuEj, lk = n / (kLuEi * kLuEj);
+    const int e = lu_slot(li, lj, lk);
+    const int c[3] = {org[0] + li, org[1] + lj, org[2] + lk};
     if (c[0] < 0 || c[0] >= m[0] || c[1] < 0 || c[1] >= m[1] || c[2] < 0 || c[2] >= m[2]) continue;
     const long long q = ((long long)c[2] * m[1] + c[1]) * m[0] + c[0];
     const long long p = pidx(B, c[0], c[1], c[2]);
@@ -789,13 +797,13 @@ __global__ void __launch_bounds__(kLuThreads) lusgs_tile_kernel(const BlockView 
   const int j = T1 * kLuTj + jl, k = T2 * kLuTk + kl;
   const bool col = j < m[1] && k < m[2];
   const int o = FWD ? 1 : 0;  // position of the tile's first cell in the staged box
-  const int es[3] = {1, kLuEi, kLuEi * kLuEj};
+  const int es[3] = {1, kLuEi, kLuSk};
   for (int step = 0; step < STEPS; ++step) {
     const int il = (FWD ? step : STEPS - 1 - step) - jl - kl;
     const int i = T0 * kLuTi + il;
     const bool act = col && il >= 0 && il < kLuTi && i < m[0];
     const int ijk[3] = {i, j, k};
-    const int e = ((kl + o) * kLuEj + (jl + o)) * kLuEi + (il + o);
+    const int e = lu_slot(il + o, jl + o, kl + o);
     if (act && wv < S.dim && (FWD ? ijk[wv] > 0 : ijk[wv] < m[wv] - 1)) {
       const int d = wv;
       const int en = FWD ? e - es[d] : e + es[d];
@@ -850,7 +858,7 @@ __global__ void __launch_bounds__(kLuThreads) lusgs_tile_kernel(const BlockView 
     const int il = c % kLuTi, jc = (c / kLuTi) % kLuTj, kc = c / (kLuTi * kLuTj);
     const int i = T0 * kLuTi + il, jg = T1 * kLuTj + jc, kg = T2 * kLuTk + kc;
     if (i >= m[0] || jg >= m[1] || kg >= m[2]) continue;
-    const int e = ((kc + o) * kLuEj + (jc + o)) * kLuEi + (il + o);
+    const int e = lu_slot(il + o, jc + o, kc + o);
     const long long q = ((long long)kg * m[1] + jg) * m[0] + i;
 #pragma unroll
     for (int l = 0; l < NV; ++l) B.dq[l * ni + q] = ext[XQ + l][e];
