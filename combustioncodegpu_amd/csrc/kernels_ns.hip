@@ -314,8 +314,13 @@ __global__ void __launch_bounds__(kTpb) update_kernel(const BlockView *__restric
     c[l] = rk3 ? A * base + Bc * (cur[l] + dqv[l] * dtj) : cur[l] + dqv[l] * dtj;
   }
   if (B.mz == 1) c[3] = 0.0;
-  // every store comes after the Newton iteration: loads and stores share one
-  // wait counter, so a store issued early would make later waits stall on it
+  // the stage base goes out now (every load of the point has been issued and
+  // the Newton iteration reads only scalar memory), which frees cur; the
+  // state's stores come after the iteration
+  if (rk3 && stage == 0) {
+#pragma unroll
+    for (int l = 0; l < NV; ++l) __builtin_nontemporal_store(cur[l], &B.qn[l * ni + q]);
+  }
   const double rho = c[0];
   const double ir = 1.0 / rho;
   const double u = c[1] * ir, v = c[2] * ir, w = c[3] * ir;
@@ -354,10 +359,6 @@ __global__ void __launch_bounds__(kTpb) update_kernel(const BlockView *__restric
   } else {
     pr = (S.gamma_air - 1) * (c[4] - 0.5 * rho * v2);
     T = pr * ir / S.r_air;
-  }
-  if (rk3 && stage == 0) {
-#pragma unroll
-    for (int l = 0; l < NV; ++l) __builtin_nontemporal_store(cur[l], &B.qn[l * ni + q]);
   }
 #pragma unroll
   for (int l = 0; l < NV; ++l) B.cv[l * np + p] = c[l];
