@@ -37,6 +37,13 @@ __device__ __forceinline__ void static_for(F &&f) {
   static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
+// Two consecutive elements of a field (the two points of an x face); element
+// alignment only, so the load may straddle a 16-byte boundary.
+template <typename R>
+struct alignas(sizeof(R)) FacePair {
+  R l, r;
+};
+
 template <int DIR>
 struct Axes;
 template <>
@@ -181,6 +188,16 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
     const long long pr_i = pl_i + st[A];
     // byte offsets of the two face points (32-bit: shared by every field array)
     const unsigned pl_b = (unsigned)pl_i * (unsigned)sizeof(R), pr_b = (unsigned)pr_i * (unsigned)sizeof(R);
+    // a field at the face's two points: for x faces they are adjacent and
+    // come in one load of two elements
+    auto ld2 = [&](gptr<const R> base, R &l, R &r) {
+      if constexpr (DIR == 0) {
+        const gptr<const FacePair<R>> v = (gptr<const FacePair<R>>)((gptr<const char>)base + pl_b);
+        l = v->l, r = v->r;
+      } else {
+        l = ldg<R>(base, pl_b), r = ldg<R>(base, pr_b);
+      }
+    };
     const int szi = (int)B.sz, syi = (int)B.sy;
 
     R k1 = 0, k2 = 0, k3 = 0;
@@ -303,10 +320,14 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
       for (int v = 0; v < NV; ++v) F[v] = R(0);
       if (face_ok) {
         const gptr<const R> M = MT_ + 3 * A * np;
-        const R jl = ldg<R>(JC_, pl_b), jr = ldg<R>(JC_, pr_b);
-        k1 = R(0.5) * (jl * ldg<R>(M, pl_b) + jr * ldg<R>(M, pr_b));
-        k2 = R(0.5) * (jl * ldg<R>(M + np, pl_b) + jr * ldg<R>(M + np, pr_b));
-        k3 = R(0.5) * (jl * ldg<R>(M + 2 * np, pl_b) + jr * ldg<R>(M + 2 * np, pr_b));
+        R jl, jr, ml, mr;
+        ld2(JC_, jl, jr);
+        ld2(M, ml, mr);
+        k1 = R(0.5) * (jl * ml + jr * mr);
+        ld2(M + np, ml, mr);
+        k2 = R(0.5) * (jl * ml + jr * mr);
+        ld2(M + 2 * np, ml, mr);
+        k3 = R(0.5) * (jl * ml + jr * mr);
         ausm_from_states<NS, R>(sp, ThermoConst<R>{sp}, S, pl, pr, k1, k2, k3, F);
       }
 #pragma unroll
@@ -327,16 +348,34 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
                                 ? (unsigned)(pl_i + B.sy) * (unsigned)sizeof(R) : pr_b;
       if (face_ok) {
         const bool q5 = (S.quirks & CFX_Q_METRIC_AVG) && DIR != 1;
+        if constexpr (DIR == 0) {
 #pragma unroll
-        for (int rc = 0; rc < 9; ++rc) {
-          const R a = ldg<R>(MT_ + rc * np, pl_b);
-          mf[rc] = (q5 && rc != 0) ? R(0.5) * (a + a) : R(0.5) * (a + ldg<R>(MT_ + rc * np, pr_b));
+          for (int rc = 0; rc < 9; ++rc) {
+            R a, b;
+            ld2(MT_ + rc * np, a, b);
+            mf[rc] = (q5 && rc != 0) ? R(0.5) * (a + a) : R(0.5) * (a + b);
+          }
+        } else {
+#pragma unroll
+          for (int rc = 0; rc < 9; ++rc) {
+            const R a = ldg<R>(MT_ + rc * np, pl_b);
+            mf[rc] = (q5 && rc != 0) ? R(0.5) * (a + a) : R(0.5) * (a + ldg<R>(MT_ + rc * np, pr_b));
+          }
         }
-        vis = R(0.5) * (ldg<R>(PR_ + 2 * np, pl_b) + ldg<R>(PR_ + 2 * np, pr_b));
-        cond = R(0.5) * (ldg<R>(PR_ + 3 * np, pl_b) + ldg<R>(PR_ + 3 * np, pr_b));
+        R a, b;
+        ld2(PR_ + 2 * np, a, b);
+        vis = R(0.5) * (a + b);
+        ld2(PR_ + 3 * np, a, b);
+        cond = R(0.5) * (a + b);
 #pragma unroll
-        for (int s2 = 0; s2 < NS; ++s2)
-          dif[s2] = R(0.5) * (ldg<R>(PR_ + (4 + s2) * np, pl_b) + ldg<R>(PR_ + (4 + s2) * np, pD_b));
+        for (int s2 = 0; s2 < NS; ++s2) {
+          if constexpr (DIR == 0) {  // the rho*D quirk reads j+1 only on z faces
+            ld2(PR_ + (4 + s2) * np, a, b);
+          } else {
+            a = ldg<R>(PR_ + (4 + s2) * np, pl_b), b = ldg<R>(PR_ + (4 + s2) * np, pD_b);
+          }
+          dif[s2] = R(0.5) * (a + b);
+        }
       } else {
 #pragma unroll
         for (int rc = 0; rc < 9; ++rc) mf[rc] = R(0);
