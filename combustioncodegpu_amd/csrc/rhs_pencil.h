@@ -201,6 +201,10 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
     const int szi = (int)B.sz, syi = (int)B.sy;
 
     R k1 = 0, k2 = 0, k3 = 0;
+    // face averages of the metric row of the face direction, formed with k
+    // from the same loads and kept for the viscous flux
+    R mfa[3] = {0, 0, 0};
+    const bool q5 = (S.quirks & CFX_Q_METRIC_AVG) && DIR != 1;
     // ---- batches through two LDS buffers.  Batch n lands in buffer n % 2;
     // after the barrier that publishes batch n, batch n + 1 is issued into
     // the other buffer (whose last readers passed that barrier) and stays
@@ -324,10 +328,13 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
         ld2(JC_, jl, jr);
         ld2(M, ml, mr);
         k1 = R(0.5) * (jl * ml + jr * mr);
+        mfa[0] = (q5 && 3 * A != 0) ? R(0.5) * (ml + ml) : R(0.5) * (ml + mr);
         ld2(M + np, ml, mr);
         k2 = R(0.5) * (jl * ml + jr * mr);
+        mfa[1] = q5 ? R(0.5) * (ml + ml) : R(0.5) * (ml + mr);
         ld2(M + 2 * np, ml, mr);
         k3 = R(0.5) * (jl * ml + jr * mr);
+        mfa[2] = q5 ? R(0.5) * (ml + ml) : R(0.5) * (ml + mr);
         ausm_from_states<NS, R>(sp, ThermoConst<R>{sp}, S, pl, pr, k1, k2, k3, F);
       }
 #pragma unroll
@@ -347,17 +354,17 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
       const unsigned pD_b = (NS > 0 && DIR == 2 && (S.quirks & CFX_Q_HV_RHOD_J))
                                 ? (unsigned)(pl_i + B.sy) * (unsigned)sizeof(R) : pr_b;
       if (face_ok) {
-        const bool q5 = (S.quirks & CFX_Q_METRIC_AVG) && DIR != 1;
-        if constexpr (DIR == 0) {
 #pragma unroll
-          for (int rc = 0; rc < 9; ++rc) {
+        for (int rc = 0; rc < 9; ++rc) {
+          if (rc / 3 == A) {  // the row of the face direction, kept from the face-area vector
+            mf[rc] = mfa[rc % 3];
+            continue;
+          }
+          if constexpr (DIR == 0) {
             R a, b;
             ld2(MT_ + rc * np, a, b);
             mf[rc] = (q5 && rc != 0) ? R(0.5) * (a + a) : R(0.5) * (a + b);
-          }
-        } else {
-#pragma unroll
-          for (int rc = 0; rc < 9; ++rc) {
+          } else {
             const R a = ldg<R>(MT_ + rc * np, pl_b);
             mf[rc] = (q5 && rc != 0) ? R(0.5) * (a + a) : R(0.5) * (a + ldg<R>(MT_ + rc * np, pr_b));
           }
