@@ -462,19 +462,25 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
       }
     }
 
-    // ---- cell difference of the face slots, dq written once per direction
+    // ---- cell difference of the face slots, dq written once per direction;
+    // the dq being accumulated into is read before the barrier, so that its
+    // latency overlaps the wait for the other wavefronts
+    int c[3] = {g[0], g[1], g[2]};
+    c[A] += 1;
+    const unsigned qb = cell_ok ? (unsigned)((c[2] * B.my + c[1]) * B.mx + c[0]) * (unsigned)sizeof(R) : 0u;
+    const gptr<R> __restrict__ DQ = Fd::dq(B);
+    R dq0[NV];
+    if (accumulate && cell_ok) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) dq0[v] = ldg<R>(DQ + v * B.ni, qb);
+    }
     __syncthreads();
     if (cell_ok) {
-      int c[3] = {g[0], g[1], g[2]};
-      c[A] += 1;
-      const unsigned qb = (unsigned)((c[2] * B.my + c[1]) * B.mx + c[0]) * (unsigned)sizeof(R);
-      const gptr<R> __restrict__ DQ = Fd::dq(B);
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         const R d = -(fbuf[v * NT + tid + ta_stride] - fbuf[v * NT + tid]);
         const gptr<R> dst = atg<R>(DQ + v * B.ni, qb);
-        if (accumulate) *dst += d;
-        else *dst = d;
+        *dst = accumulate ? dq0[v] + d : d;
       }
     }
   }
