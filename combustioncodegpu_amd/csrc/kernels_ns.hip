@@ -428,15 +428,27 @@ __device__ __forceinline__ void bc_outflow_point(const GView &B, const FaceRange
   dir[f.face] = f.direction;
   const long long np = B.np;
   const long long p = pidx(B, i, j, k);
+  // every value of the face point is read before the first store: the
+  // stores go to the same arrays, so the compiler could not move a later
+  // load above an earlier store and each field would cost a memory round trip
+  double bvp[6], cvp[5 + NS], ykp[NSA];
+#pragma unroll
+  for (int l = 0; l < 6; ++l) bvp[l] = B.bv[l * np + p];
+#pragma unroll
+  for (int l = 0; l < 5 + NS; ++l) cvp[l] = B.cv[l * np + p];
+  if (!B.mixed) {
+#pragma unroll
+    for (int l = 0; l < NS; ++l) ykp[l] = B.yk[l * np + p];
+  }
   for (int g = 1; g <= kNgg; ++g) {
     const long long q = pidx(B, i + g * dir[0], j + g * dir[1], k + g * dir[2]);
 #pragma unroll
-    for (int l = 0; l < 6; ++l) B.bv[l * np + q] = B.bv[l * np + p];
+    for (int l = 0; l < 6; ++l) B.bv[l * np + q] = bvp[l];
 #pragma unroll
-    for (int l = 0; l < 5 + NS; ++l) B.cv[l * np + q] = B.cv[l * np + p];
+    for (int l = 0; l < 5 + NS; ++l) B.cv[l * np + q] = cvp[l];
     if (!B.mixed) {
 #pragma unroll
-      for (int l = 0; l < NS; ++l) B.yk[l * np + q] = B.yk[l * np + p];
+      for (int l = 0; l < NS; ++l) B.yk[l * np + q] = ykp[l];
     }
   }
 }
@@ -598,35 +610,45 @@ __device__ __forceinline__ void exchange_point(const GView &B, const GView &T, c
   const long long t = pidx(T, it[0], it[1], it[2]);
   const long long np = B.np, tnp = T.np;
   const int face_layer = f.direction > 0 ? f.rs[f.face] : f.re[f.face];
+  // Loads are grouped ahead of the stores (the two blocks' arrays may alias
+  // as far as the compiler knows, so it keeps every store before any later
+  // load: field by field, each would cost a memory round trip).
   if (idx[f.face] == face_layer) {
+    // the shared layer: both sides of a group of fields, then the averages
+    auto average = [&](gptr<double> bb, gptr<double> tt, auto kc) {
+      constexpr int K = decltype(kc)::value;
+      double x[K > 0 ? K : 1], y[K > 0 ? K : 1];
 #pragma unroll
-    for (int l = 0; l < 6; ++l) {
-      const double a = 0.5 * (T.bv[l * tnp + t] + B.bv[l * np + p]);
-      B.bv[l * np + p] = a;
-      T.bv[l * tnp + t] = a;
-    }
+      for (int l = 0; l < K; ++l) x[l] = tt[l * tnp + t], y[l] = bb[l * np + p];
 #pragma unroll
-    for (int l = 0; l < 5 + NS; ++l) {
-      const double a = 0.5 * (T.cv[l * tnp + t] + B.cv[l * np + p]);
-      B.cv[l * np + p] = a;
-      T.cv[l * tnp + t] = a;
-    }
-    if (!B.mixed) {
-#pragma unroll
-      for (int l = 0; l < NS; ++l) {
-        const double a = 0.5 * (T.yk[l * tnp + t] + B.yk[l * np + p]);
-        B.yk[l * np + p] = a;
-        T.yk[l * tnp + t] = a;
+      for (int l = 0; l < K; ++l) {
+        const double a = 0.5 * (x[l] + y[l]);
+        bb[l * np + p] = a;
+        tt[l * tnp + t] = a;
       }
-    }
+    };
+    average(B.bv, T.bv, std::integral_constant<int, 6>{});
+    average(B.cv, T.cv, std::integral_constant<int, 5 + NS>{});
+    if constexpr (NS > 0)
+      if (!B.mixed) average(B.yk, T.yk, std::integral_constant<int, NS>{});
   } else {
+    // a ghost point: the neighbour's values, all read before any store
+    double bvt[6], cvt[5 + NS], ykt[NSA];
 #pragma unroll
-    for (int l = 0; l < 6; ++l) B.bv[l * np + p] = T.bv[l * tnp + t];
+    for (int l = 0; l < 6; ++l) bvt[l] = T.bv[l * tnp + t];
 #pragma unroll
-    for (int l = 0; l < 5 + NS; ++l) B.cv[l * np + p] = T.cv[l * tnp + t];
+    for (int l = 0; l < 5 + NS; ++l) cvt[l] = T.cv[l * tnp + t];
     if (!B.mixed) {
 #pragma unroll
-      for (int l = 0; l < NS; ++l) B.yk[l * np + p] = T.yk[l * tnp + t];
+      for (int l = 0; l < NS; ++l) ykt[l] = T.yk[l * tnp + t];
+    }
+#pragma unroll
+    for (int l = 0; l < 6; ++l) B.bv[l * np + p] = bvt[l];
+#pragma unroll
+    for (int l = 0; l < 5 + NS; ++l) B.cv[l * np + p] = cvt[l];
+    if (!B.mixed) {
+#pragma unroll
+      for (int l = 0; l < NS; ++l) B.yk[l * np + p] = ykt[l];
     }
   }
 }
