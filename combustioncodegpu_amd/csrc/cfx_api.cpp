@@ -847,10 +847,10 @@ int cfx_create(const cfx_params *params, const cfx_species *species, const cfx_b
   ctx->rk = S.rk;
   // default RHS tiles of a 3-D context: shapes among the fastest the
   // autotuner measures on MI355X for the 256x128x128 9-species block (FP64:
-  // x 32x8x1, y 16x16x1, z 8x32x1 with c fastest; FP32: x 32x4x2 at 4 waves,
-  // y 8x32x1, z 8x32x1 with c fastest); cfx_autotune re-picks per context
+  // x 32x4x2, y 8x32x1 and z 8x32x1 with c fastest; FP32: x 32x4x2 at 4
+  // waves, y 8x32x1, z 8x32x1 with c fastest); cfx_autotune re-picks per context
   if (ctx->dim == 3) {
-    const int d64[3] = {4, 4, 1}, d32[3] = {5, 2, 1};
+    const int d64[3] = {5, 1, 1}, d32[3] = {5, 2, 1};
     for (int d = 0; d < 3; ++d) ctx->rhs_var[d] = ctx->mixed ? d32[d] : d64[d];
   }
   ctx->lusgs = S.lusgs != 0;
