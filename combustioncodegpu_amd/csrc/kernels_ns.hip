@@ -267,6 +267,25 @@ __device__ __forceinline__ void total_energy(const GView &B, SpeciesPtrT<double>
   B.cv[4 * np + p] = e;
 }
 
+// The same total energy from values held in registers (rhoY: the partial
+// densities), in total_energy's operation order.
+__device__ __forceinline__ double total_energy_of(SpeciesPtrT<double> sp, const SchemeDev &S, double rho, double u,
+                                                  double v, double w, double t, double pr, const double *rhoY) {
+  double e = 0.5 * rho * (u * u + v * v + w * w);
+  if constexpr (NS > 0) {
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      double h, cp;
+      species_h_cp<double>(sp, l, t, h, cp);
+      e += h * rhoY[l];
+    }
+    e -= pr;
+  } else {
+    e += pr / (S.gamma_air - 1);
+  }
+  return e;
+}
+
 // --------------------------------------------------------------- update ---
 // update_cv_and_bv (TimeAdvanceFunc.cu:255-290) with the RK stage
 // combination q = a*qn + b*(q + dt*dq/J) and the Newton inversion of T
@@ -497,60 +516,76 @@ __device__ __forceinline__ void bc_wall_point(const GView &B, const FaceRange &f
   dir[f.face] = f.direction;
   const long long np = B.np;
   const long long p = pidx(B, i, j, k);
-  const long long pin = pidx(B, i - dir[0], j - dir[1], k - dir[2]);
-  double t_wall = wl.temperature;
-  if (wl.adiabatic) t_wall = B.bv[5 * np + pin];
-  const double pw = B.bv[4 * np + pin];
-  double Yw[NSA];
+  // every interior value is read before the first store (the stores could
+  // alias later loads as far as the compiler knows; one round trip each);
+  // interior point g = 1 is the one next to the wall
+  long long qi[kNgg], qg[kNgg];
+  double Yi[kNgg][NSA], ui[kNgg], vi[kNgg], wi[kNgg], pi[kNgg], ti[kNgg];
 #pragma unroll
-  for (int l = 0; l < NS; ++l) {
-    Yw[l] = mass_fraction(B, pin, l);
-    if (!B.mixed) B.yk[l * np + p] = Yw[l];
+  for (int g = 0; g < kNgg; ++g) {
+    qi[g] = pidx(B, i - (g + 1) * dir[0], j - (g + 1) * dir[1], k - (g + 1) * dir[2]);
+    qg[g] = pidx(B, i + (g + 1) * dir[0], j + (g + 1) * dir[1], k + (g + 1) * dir[2]);
+#pragma unroll
+    for (int l = 0; l < NS; ++l) Yi[g][l] = mass_fraction(B, qi[g], l);
+    ui[g] = B.bv[np + qi[g]], vi[g] = B.bv[2 * np + qi[g]], wi[g] = B.bv[3 * np + qi[g]];
+    pi[g] = B.bv[4 * np + qi[g]], ti[g] = B.bv[5 * np + qi[g]];
   }
-  double mw = mix_mw(sp, S, Yw);
-  B.bv[p] = pw * mw / (t_wall * kRu);
-  B.bv[np + p] = 0;
-  B.bv[2 * np + p] = 0;
-  B.bv[3 * np + p] = 0;
-  B.bv[4 * np + p] = pw;
-  B.bv[5 * np + p] = t_wall;
-  B.cv[p] = B.bv[p];
-  B.cv[np + p] = 0;
-  B.cv[2 * np + p] = 0;
-  B.cv[3 * np + p] = 0;
+  double t_wall = wl.temperature;
+  if (wl.adiabatic) t_wall = ti[0];
+  const double pw = pi[0];
+  double rY[NSA];
+  {
+    const double rho = pw * mix_mw(sp, S, Yi[0]) / (t_wall * kRu);
+    if (!B.mixed) {
 #pragma unroll
-  for (int l = 0; l < NS; ++l) B.cv[(l + 5) * np + p] = Yw[l] * B.bv[p];
-  total_energy(B, sp, S, p);
-  for (int g = 1; g <= kNgg; ++g) {
-    const long long qi = pidx(B, i - g * dir[0], j - g * dir[1], k - g * dir[2]);
-    const long long qg = pidx(B, i + g * dir[0], j + g * dir[1], k + g * dir[2]);
+      for (int l = 0; l < NS; ++l) B.yk[l * np + p] = Yi[0][l];
+    }
+    B.bv[p] = rho;
+    B.bv[np + p] = 0;
+    B.bv[2 * np + p] = 0;
+    B.bv[3 * np + p] = 0;
+    B.bv[4 * np + p] = pw;
+    B.bv[5 * np + p] = t_wall;
+    B.cv[p] = rho;
+    B.cv[np + p] = 0;
+    B.cv[2 * np + p] = 0;
+    B.cv[3 * np + p] = 0;
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
-      Yw[l] = mass_fraction(B, qi, l);
-      if (!B.mixed) B.yk[l * np + qg] = Yw[l];
+      rY[l] = Yi[0][l] * rho;
+      B.cv[(l + 5) * np + p] = rY[l];
     }
-    mw = mix_mw(sp, S, Yw);
-    const double u_i = B.bv[np + qi], v_i = B.bv[2 * np + qi], w_i = B.bv[3 * np + qi];
-    const double p_i = B.bv[4 * np + qi], t_i = B.bv[5 * np + qi];
-    double t_g = t_i;
+    B.cv[4 * np + p] = total_energy_of(sp, S, rho, 0.0, 0.0, 0.0, t_wall, pw, rY);
+  }
+#pragma unroll
+  for (int g = 0; g < kNgg; ++g) {
+    if (!B.mixed) {
+#pragma unroll
+      for (int l = 0; l < NS; ++l) B.yk[l * np + qg[g]] = Yi[g][l];
+    }
+    const double mw = mix_mw(sp, S, Yi[g]);
+    double t_g = ti[g];
     if (!wl.adiabatic) {
-      t_g = 2 * t_wall - t_i;
+      t_g = 2 * t_wall - ti[g];
       if (t_g <= 0.1 * t_wall) t_g = t_wall;
     }
-    const double rho_g = p_i * mw / (t_g * kRu);
-    B.bv[qg] = rho_g;
-    B.bv[np + qg] = -u_i;
-    B.bv[2 * np + qg] = -v_i;
-    B.bv[3 * np + qg] = -w_i;
-    B.bv[4 * np + qg] = p_i;
-    B.bv[5 * np + qg] = t_g;
-    B.cv[qg] = rho_g;
-    B.cv[np + qg] = -rho_g * u_i;
-    B.cv[2 * np + qg] = -rho_g * v_i;
-    B.cv[3 * np + qg] = -rho_g * w_i;
+    const double rho_g = pi[g] * mw / (t_g * kRu);
+    B.bv[qg[g]] = rho_g;
+    B.bv[np + qg[g]] = -ui[g];
+    B.bv[2 * np + qg[g]] = -vi[g];
+    B.bv[3 * np + qg[g]] = -wi[g];
+    B.bv[4 * np + qg[g]] = pi[g];
+    B.bv[5 * np + qg[g]] = t_g;
+    B.cv[qg[g]] = rho_g;
+    B.cv[np + qg[g]] = -rho_g * ui[g];
+    B.cv[2 * np + qg[g]] = -rho_g * vi[g];
+    B.cv[3 * np + qg[g]] = -rho_g * wi[g];
 #pragma unroll
-    for (int l = 0; l < NS; ++l) B.cv[(l + 5) * np + qg] = Yw[l] * rho_g;
-    total_energy(B, sp, S, qg);
+    for (int l = 0; l < NS; ++l) {
+      rY[l] = Yi[g][l] * rho_g;
+      B.cv[(l + 5) * np + qg[g]] = rY[l];
+    }
+    B.cv[4 * np + qg[g]] = total_energy_of(sp, S, rho_g, -ui[g], -vi[g], -wi[g], t_g, pi[g], rY);
   }
 }
 
@@ -563,35 +598,57 @@ __device__ __forceinline__ void bc_symmetry_point(const GView &B, const FaceRang
   dir[f.face] = f.direction;
   const long long np = B.np;
   const long long p = pidx(B, i, j, k);
+  // every value is read before the first store (as in the wall condition)
   double n[3];
   for (int c = 0; c < 3; ++c) n[c] = B.metric[(f.face * 3 + c) * np + p];
+  double bp[6], rYp[NSA];
+#pragma unroll
+  for (int l = 0; l < 6; ++l) bp[l] = B.bv[l * np + p];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) rYp[l] = B.cv[(5 + l) * np + p];
+  long long qi[kNgg], qg[kNgg];
+  double bi[kNgg][6], Yi[kNgg][NSA];
+#pragma unroll
+  for (int g = 0; g < kNgg; ++g) {
+    qi[g] = pidx(B, i - (g + 1) * dir[0], j - (g + 1) * dir[1], k - (g + 1) * dir[2]);
+    qg[g] = pidx(B, i + (g + 1) * dir[0], j + (g + 1) * dir[1], k + (g + 1) * dir[2]);
+#pragma unroll
+    for (int l = 0; l < 6; ++l) bi[g][l] = B.bv[l * np + qi[g]];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) Yi[g][l] = mass_fraction(B, qi[g], l);
+  }
   const double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
   n[0] /= nn;
   n[1] /= nn;
   n[2] /= nn;
   {
-    const double vn = B.bv[np + p] * n[0] + B.bv[2 * np + p] * n[1] + B.bv[3 * np + p] * n[2];
-    for (int c = 0; c < 3; ++c) B.bv[(1 + c) * np + p] -= vn * n[c];
-    for (int c = 0; c < 3; ++c) B.cv[(1 + c) * np + p] = B.bv[p] * B.bv[(1 + c) * np + p];
-    total_energy(B, sp, S, p);
+    const double vn = bp[1] * n[0] + bp[2] * n[1] + bp[3] * n[2];
+    for (int c = 0; c < 3; ++c) {
+      bp[1 + c] -= vn * n[c];
+      B.bv[(1 + c) * np + p] = bp[1 + c];
+    }
+    for (int c = 0; c < 3; ++c) B.cv[(1 + c) * np + p] = bp[0] * bp[1 + c];
+    B.cv[4 * np + p] = total_energy_of(sp, S, bp[0], bp[1], bp[2], bp[3], bp[5], bp[4], rYp);
   }
-  for (int g = 1; g <= kNgg; ++g) {
-    const long long qi = pidx(B, i - g * dir[0], j - g * dir[1], k - g * dir[2]);
-    const long long qg = pidx(B, i + g * dir[0], j + g * dir[1], k + g * dir[2]);
-    const double vn = B.bv[np + qi] * n[0] + B.bv[2 * np + qi] * n[1] + B.bv[3 * np + qi] * n[2];
-    B.bv[qg] = B.bv[qi];
-    for (int c = 0; c < 3; ++c) B.bv[(1 + c) * np + qg] = B.bv[(1 + c) * np + qi] - 2 * vn * n[c];
-    B.bv[4 * np + qg] = B.bv[4 * np + qi];
-    B.bv[5 * np + qg] = B.bv[5 * np + qi];
+#pragma unroll
+  for (int g = 0; g < kNgg; ++g) {
+    const double *b = bi[g];
+    const double vn = b[1] * n[0] + b[2] * n[1] + b[3] * n[2];
+    double u[3], rY[NSA];
+    for (int c = 0; c < 3; ++c) u[c] = b[1 + c] - 2 * vn * n[c];
+    B.bv[qg[g]] = b[0];
+    for (int c = 0; c < 3; ++c) B.bv[(1 + c) * np + qg[g]] = u[c];
+    B.bv[4 * np + qg[g]] = b[4];
+    B.bv[5 * np + qg[g]] = b[5];
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
-      const double y = mass_fraction(B, qi, l);
-      if (!B.mixed) B.yk[l * np + qg] = y;
-      B.cv[(5 + l) * np + qg] = B.bv[qg] * y;
+      if (!B.mixed) B.yk[l * np + qg[g]] = Yi[g][l];
+      rY[l] = b[0] * Yi[g][l];
+      B.cv[(5 + l) * np + qg[g]] = rY[l];
     }
-    B.cv[qg] = B.bv[qg];
-    for (int c = 0; c < 3; ++c) B.cv[(1 + c) * np + qg] = B.bv[qg] * B.bv[(1 + c) * np + qg];
-    total_energy(B, sp, S, qg);
+    B.cv[qg[g]] = b[0];
+    for (int c = 0; c < 3; ++c) B.cv[(1 + c) * np + qg[g]] = b[0] * u[c];
+    B.cv[4 * np + qg[g]] = total_energy_of(sp, S, b[0], u[0], u[1], u[2], b[5], b[4], rY);
   }
 }
 
