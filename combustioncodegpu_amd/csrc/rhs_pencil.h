@@ -140,7 +140,8 @@ __device__ __forceinline__ bool pencil_tile(int lid, int nta, int ntb, int ntc, 
 template <int NS, int DIR, int TA, int TB, int TC, typename R, int G, int WPE>
 __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ views, const int *__restrict__ off,
                                                 int nb, int padded, int tseq, SpeciesPtrT<R> sp, const SchemeDev &S,
-                                                int accumulate, R *__restrict__ stg, R *__restrict__ fbuf) {
+                                                int accumulate, R *__restrict__ stg0, R *__restrict__ stg1,
+                                                R *__restrict__ fbuf) {
   using P = PencilShape<NS, DIR, TA, TB, TC, R, WPE>;
   using BI = typename P::BI;
   using BV = typename P::BV;
@@ -214,7 +215,7 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
     constexpr int NVV = 4 + NS;                    // viscous variables: u, v, w, T, Y_l
     constexpr int NVB = (NVV + KV - 1) / KV;       // viscous batches
     const bool viscous = S.viscous == 2;
-    R *const buf0 = stg, *const buf1 = stg + P::STG;
+    R *const buf0 = stg0, *const buf1 = stg1;
     // The byte offsets of this lane's staged elements: lane e of a box
     // loads element e of each variable, the wavefront writing 64 consecutive
     // slots from its first element on (FP64: lane pairs share an element,
@@ -494,14 +495,14 @@ __global__ void __launch_bounds__(TA *TB *TC) __attribute__((amdgpu_waves_per_eu
     rhs_pencil_kernel(const BlockView *__restrict__ views, const int *__restrict__ off, int nb, int padded,
                       const SpeciesDev *__restrict__ spg, SchemeDev S, int accumulate) {
   using P = PencilShape<NS, DIR, TA, TB, TC, R, WPE>;
-  __shared__ R stg[2 * P::STG];        // two batches of staged variables
+  __shared__ R stg0[P::STG], stg1[P::STG];  // two batches of staged variables
   __shared__ R fbuf[P::NV * P::NT];    // each thread's face flux F_inv - F_visc
   const SpeciesPtrT<R> sp = const_table<R>(spg);
   if constexpr (PERSIST) {
     for (int t = blockIdx.x; t < padded; t += gridDim.x)
-      rhs_pencil_tile<NS, DIR, TA, TB, TC, R, G, WPE>(views, off, nb, padded, t, sp, S, accumulate, stg, fbuf);
+      rhs_pencil_tile<NS, DIR, TA, TB, TC, R, G, WPE>(views, off, nb, padded, t, sp, S, accumulate, stg0, stg1, fbuf);
   } else {
-    rhs_pencil_tile<NS, DIR, TA, TB, TC, R, G, WPE>(views, off, nb, padded, blockIdx.x, sp, S, accumulate, stg, fbuf);
+    rhs_pencil_tile<NS, DIR, TA, TB, TC, R, G, WPE>(views, off, nb, padded, blockIdx.x, sp, S, accumulate, stg0, stg1, fbuf);
   }
 }
 
