@@ -209,7 +209,11 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
     // ---- batches through two LDS buffers.  Batch n lands in buffer n % 2;
     // after the barrier that publishes batch n, batch n + 1 is issued into
     // the other buffer (whose last readers passed that barrier) and stays
-    // in flight while batch n is used.
+    // in flight while batch n is used.  Both buffers are free when a tile
+    // starts, so the first two batches go out together.  The buffers are
+    // separate shared arrays: the compiler's waits for global-to-LDS loads
+    // then tell them apart and do not hold the reads of one buffer until
+    // the loads into the other have landed.
     constexpr int VPI = P::VPI, VPV = P::VPV, KI = P::KI, KV = P::KV;
     constexpr int NI = (NV + KI - 1) / KI;         // inviscid batches
     constexpr int NVV = 4 + NS;                    // viscous variables: u, v, w, T, Y_l
@@ -223,6 +227,9 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
     // for all its batches.
     constexpr int W = P::W, EPW = P::EPW, NW = NT / 64;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    // the range readfirstlane hides: the chunks of whole rounds of NW are
+    // then issued without a test
+    __builtin_assume(wave >= 0 && wave < NW);
     auto offsets = [&](auto box, const int bo[3], unsigned *gb) {
       using BX = decltype(box);
       constexpr int NC = (BX::V + EPW - 1) / EPW;  // wavefront loads per variable
@@ -266,11 +273,28 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
       __builtin_amdgcn_s_setprio(0);
     };
     // the loads of this thread (and so of the batch, after the barrier)
-    // have written LDS: vmcnt(0) with lgkmcnt and expcnt left open
-    auto land = [&]() {
-      __builtin_amdgcn_s_waitcnt(0x0F70);
-      __syncthreads();
+    // have written LDS, except the last `later` ones it issued: vmcnt(later)
+    // with lgkmcnt and expcnt left open.  Vector memory loads complete in
+    // issue order, and the issues of different batches stay in issue order
+    // (s_setprio, which brackets each batch, is a scheduling boundary).
+    // With later > 0 the barrier goes without __syncthreads' workgroup
+    // fences, which would wait for every global-to-LDS load in flight
+    // (vmcnt(0)); the empty asm statements with a memory clobber keep the
+    // compiler from moving memory accesses across it instead.  Used only
+    // where no LDS store of this tile precedes the barrier.
+    auto land = [&](auto later) {
+      constexpr int n = decltype(later)::value;
+      static_assert(n >= 0 && n < 64, "vmcnt holds 6 bits");
+      __builtin_amdgcn_s_waitcnt((n & 15) | ((n >> 4) << 14) | 0x0F70);
+      if constexpr (n == 0) {
+        __syncthreads();
+      } else {
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+      }
     };
+    using none = std::integral_constant<int, 0>;
     int boi[3], bov[3];
     boi[A] = o[A] - 2, boi[Bx] = o[Bx], boi[Cx] = o[Cx];
     bov[A] = o[A] - 1, bov[Bx] = o[Bx] - 1, bov[Cx] = o[Cx] - 1;
@@ -302,17 +326,26 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
       constexpr int sa = BI::sa;
       R pl[NV], pr[NV];
       issue_inv(std::integral_constant<int, 0>{}, buf0);
+      if constexpr (NI > 1) issue_inv(std::integral_constant<int, 1>{}, buf1);
       static_for<NI>([&](auto bi) {
         constexpr int n = decltype(bi)::value;
         constexpr int v0 = n * KI;
         constexpr int K = NV - v0 < KI ? NV - v0 : KI;
         R *const cur = n % 2 == 0 ? buf0 : buf1;
         R *const nxt = n % 2 == 0 ? buf1 : buf0;
-        land();
-        if constexpr (n + 1 < NI) {
-          issue_inv(std::integral_constant<int, n + 1>{}, nxt);
+        if constexpr (n == 0 && NI > 1) {
+          // batch 1 stays in flight: every wavefront issued at least
+          // NC / NW of its wavefront loads for each of its variables
+          constexpr int K1 = NV - KI < KI ? NV - KI : KI;
+          constexpr int NC = (BI::V + EPW - 1) / EPW;
+          land(std::integral_constant<int, K1 * (NC / NW)>{});
         } else {
+          land(none{});
+        }
+        if constexpr (n + 1 == NI) {
           if (viscous) issue_vis(std::integral_constant<int, 0>{}, nxt);
+        } else if constexpr (n > 0) {
+          issue_inv(std::integral_constant<int, n + 1>{}, nxt);
         }
 #pragma unroll
         for (int k = 0; k < K; ++k) {
@@ -406,7 +439,7 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
         constexpr int K = NVV - j0 < KV ? NVV - j0 : KV;
         R *const cur = n % 2 == 0 ? buf0 : buf1;
         R *const nxt = n % 2 == 0 ? buf1 : buf0;
-        land();
+        land(none{});
         if constexpr (nb2 + 1 < NVB) issue_vis(std::integral_constant<int, nb2 + 1>{}, nxt);
         static_for<K>([&](auto kk) {
           constexpr int k = decltype(kk)::value;
