@@ -325,6 +325,14 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
       const int li = l[0] + BI::L0 * (l[1] + BI::L1 * l[2]);
       constexpr int sa = BI::sa;
       R pl[NV], pr[NV];
+      // the face-area data (Jacobian and metric row of the face direction
+      // at the face's two points), loaded ahead of the staged batches so
+      // that its wait is the first batch's (lanes off the block read at
+      // offset 0 and do not use it)
+      R fj[2], fm[3][2];
+      ld2(JC_, fj[0], fj[1]);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) ld2(MT_ + (3 * A + r) * np, fm[r][0], fm[r][1]);
       issue_inv(std::integral_constant<int, 0>{}, buf0);
       if constexpr (NI > 1) issue_inv(std::integral_constant<int, 1>{}, buf1);
       static_for<NI>([&](auto bi) {
@@ -357,18 +365,13 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
 #pragma unroll
       for (int v = 0; v < NV; ++v) F[v] = R(0);
       if (face_ok) {
-        const gptr<const R> M = MT_ + 3 * A * np;
-        R jl, jr, ml, mr;
-        ld2(JC_, jl, jr);
-        ld2(M, ml, mr);
-        k1 = R(0.5) * (jl * ml + jr * mr);
-        mfa[0] = (q5 && 3 * A != 0) ? R(0.5) * (ml + ml) : R(0.5) * (ml + mr);
-        ld2(M + np, ml, mr);
-        k2 = R(0.5) * (jl * ml + jr * mr);
-        mfa[1] = q5 ? R(0.5) * (ml + ml) : R(0.5) * (ml + mr);
-        ld2(M + 2 * np, ml, mr);
-        k3 = R(0.5) * (jl * ml + jr * mr);
-        mfa[2] = q5 ? R(0.5) * (ml + ml) : R(0.5) * (ml + mr);
+        const R jl = fj[0], jr = fj[1];
+        k1 = R(0.5) * (jl * fm[0][0] + jr * fm[0][1]);
+        mfa[0] = (q5 && 3 * A != 0) ? R(0.5) * (fm[0][0] + fm[0][0]) : R(0.5) * (fm[0][0] + fm[0][1]);
+        k2 = R(0.5) * (jl * fm[1][0] + jr * fm[1][1]);
+        mfa[1] = q5 ? R(0.5) * (fm[1][0] + fm[1][0]) : R(0.5) * (fm[1][0] + fm[1][1]);
+        k3 = R(0.5) * (jl * fm[2][0] + jr * fm[2][1]);
+        mfa[2] = q5 ? R(0.5) * (fm[2][0] + fm[2][0]) : R(0.5) * (fm[2][0] + fm[2][1]);
         ausm_from_states<NS, R>(sp, ThermoConst<R>{sp}, S, pl, pr, k1, k2, k3, F);
       }
 #pragma unroll
@@ -399,8 +402,13 @@ __device__ __forceinline__ void rhs_pencil_tile(const BlockView *__restrict__ vi
             ld2(MT_ + rc * np, a, b);
             mf[rc] = (q5 && rc != 0) ? R(0.5) * (a + a) : R(0.5) * (a + b);
           } else {
+            // the quirk's average (a + a) / 2 reads the left point twice: a
+            // select of the address, not of the value, so that both loads
+            // stay unconditional (a value select turns into a branch with
+            // a wait for the first load)
             const R a = ldg<R>(MT_ + rc * np, pl_b);
-            mf[rc] = (q5 && rc != 0) ? R(0.5) * (a + a) : R(0.5) * (a + ldg<R>(MT_ + rc * np, pr_b));
+            const R b = ldg<R>(MT_ + rc * np, (q5 && rc != 0) ? pl_b : pr_b);
+            mf[rc] = R(0.5) * (a + b);
           }
         }
         R a, b;
