@@ -848,9 +848,10 @@ int cfx_create(const cfx_params *params, const cfx_species *species, const cfx_b
   // default RHS tiles of a 3-D context: shapes among the fastest the
   // autotuner measures on MI355X for the 256x128x128 9-species block (FP64:
   // x 32x4x2, y 8x32x1 and z 8x32x1 with c fastest; FP32: x 32x4x2 at 4
-  // waves, y 8x32x1, z 8x32x1 with c fastest); cfx_autotune re-picks per context
+  // waves, y 8x32x1, z 8x64x1 (512 threads) with c fastest); cfx_autotune
+  // re-picks per context
   if (ctx->dim == 3) {
-    const int d64[3] = {5, 1, 1}, d32[3] = {5, 2, 1};
+    const int d64[3] = {5, 1, 1}, d32[3] = {5, 2, 3};
     for (int d = 0; d < 3; ++d) ctx->rhs_var[d] = ctx->mixed ? d32[d] : d64[d];
   }
   ctx->lusgs = S.lusgs != 0;

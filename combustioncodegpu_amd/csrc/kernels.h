@@ -44,8 +44,8 @@ struct PencilTile {
 constexpr int kNVar3 = 6, kNVar2 = 2;
 constexpr PencilTile kTiles3[3][kNVar3] = {
     {{32, 4, 2, 1, 3}, {32, 4, 4, 1, 3}, {64, 2, 2, 1, 3}, {32, 8, 1, 1, 3}, {32, 8, 1, 1, 4}, {32, 4, 2, 1, 4}},
-    {{16, 8, 2, 1, 3}, {8, 32, 1, 0, 4}, {8, 32, 1, 1, 4}, {16, 16, 1, 0, 4}, {16, 16, 1, 1, 4}, {16, 8, 2, 1, 4}},
-    {{16, 8, 2, 1, 3}, {8, 32, 1, 0, 4}, {8, 32, 1, 1, 4}, {16, 16, 1, 0, 4}, {16, 16, 1, 1, 4}, {16, 8, 2, 1, 4}}};
+    {{8, 64, 1, 1, 4}, {8, 32, 1, 0, 4}, {8, 32, 1, 1, 4}, {8, 64, 1, 0, 4}, {16, 16, 1, 1, 4}, {16, 8, 2, 1, 4}},
+    {{8, 64, 1, 1, 4}, {8, 32, 1, 0, 4}, {8, 32, 1, 1, 4}, {8, 64, 1, 0, 4}, {16, 16, 1, 1, 4}, {16, 8, 2, 1, 4}}};
 constexpr PencilTile kTiles2[2][kNVar2] = {{{32, 8, 1, 1, 3}, {64, 4, 1, 1, 3}}, {{16, 16, 1, 1, 3}, {32, 8, 1, 1, 3}}};
 
 inline int rhs_variants(int dim) { return dim == 3 ? kNVar3 : kNVar2; }
